@@ -1,0 +1,180 @@
+"""Benchmark of the relaxation loop: image*cells per second per NLCG step.
+
+Workload (default): vortex_entry_256 - one-component string, 256x256 grid
+(dx = 0.125), 64 images, H = 1.0, seeded vortex-entry initial guess; mode L
+(linear reparametrisation every R = 10 NLCG steps, included in the timing).
+A "step" is one NLCG iteration over all images; the timed region runs
+`--steps` of them (default 1000 = 100 outer iterations) on fields already
+resident in HBM.  `value` is the mixed-precision path's throughput; the FP64
+path is timed on identical inputs and reported beside it.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+R = 10
+
+
+def barrier(e):
+    """post.py:188-189: max(F - F_0) and max(F - F_last)."""
+    e = np.asarray(e)
+    return float(np.max(e - e[0])), float(np.max(e - e[-1]))
+
+
+def run_path(strings, precision, steps, warmup, profile_iters=3):
+    from mepgl_amd.solver import Solver
+
+    sv = Solver(strings, precision=precision)
+    iters = max(steps // R, 1)
+    w_iters = max(warmup // R, 1)
+    sv.loop(w_iters, R, "L")
+    sv.synchronize()
+    # Re-start from the initial guess so that both precisions time the same
+    # trajectory; warm-up only compiled graphs and touched memory.
+    for i, s in enumerate(strings):
+        sv.upload(i, s)
+    sv.reset_solver()
+    sv.synchronize()
+    t0 = time.perf_counter()
+    sv.loop(iters, R, "L")
+    sv.synchronize()
+    dt = time.perf_counter() - t0
+    energies = sv.computeFreeEnergy()
+    # Per-kernel timing with HIP events on the library's stream (eager
+    # launches, same sequence), for the roofline of the dominant kernel.
+    sv.profile(True)
+    sv.loop(profile_iters, R, "L")
+    sv.synchronize()
+    times = {}
+    for k, name in ((0, "grad_update"), (1, "poly_linesearch"), (2, "grad_first")):
+        ms, cnt = sv.profile_read(k)
+        times[name] = (ms, cnt)
+    sv.profile(False)
+    info = sv.info(0)
+    sv.close()
+    return dict(seconds=dt, steps=iters * R, energies=energies, times=times, info=info)
+
+
+def cpu_baseline(strings, sample_steps):
+    """The FP64 CPU oracle on a bounded sample of the same workload."""
+    from oracle import oracle as O
+
+    O.build()
+    s = strings[0]
+    ref = O.OracleString(s)
+    ref.nlcg_steps(1)  # warm-up (page-in, thread pool)
+    t0 = time.perf_counter()
+    ref.nlcg_steps(sample_steps)
+    dt = time.perf_counter() - t0
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {
+        "value": s.image_cells * sample_steps / dt,
+        "unit": "image*cells/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{sample_steps} NLCG steps of string 0 ({s.F} images x {s.N}^2), "
+                  f"oracle/mepgl_oracle.c FP64, OpenMP {threads} threads, "
+                  f"{platform.processor() or platform.machine()}",
+    }
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="vortex_entry_256")
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU: multi-GPU runs are not part of "
+                                   "it (strings and image ranges would shard with no exchange "
+                                   "between reparametrisations)"}))
+        return 2
+    from mepgl_amd import configs
+
+    strings = configs.WORKLOADS[args.workload]()
+    cells = sum(s.image_cells for s in strings)
+    C_f = [4 if s.no_self_field else 2 + 2 * s.ncomp for s in strings]
+    cf = C_f[0]
+
+    res = {}
+    for prec in ("fp64", "mixed"):
+        res[prec] = run_path(strings, prec, args.steps, args.warmup)
+        print(f"# {prec}: {res[prec]['steps']} steps in {res[prec]['seconds']:.4f} s", flush=True)
+
+    mixed, fp64 = res["mixed"], res["fp64"]
+    value = cells * mixed["steps"] / mixed["seconds"]
+    value64 = cells * fp64["steps"] / fp64["seconds"]
+
+    # Roofline of the dominant kernel (mixed path), algorithmic bytes:
+    # G with update reads x, d, g_old and writes x, g: 5 C_f s per image*cell;
+    # P reads x, g, d and writes d: 4 C_f s.
+    def roof(r, sbytes):
+        g_ms, g_n = r["times"]["grad_update"]
+        p_ms, p_n = r["times"]["poly_linesearch"]
+        cand = [("grad_update", g_ms, g_n, 5 * cf * sbytes), ("poly_linesearch", p_ms, p_n, 4 * cf * sbytes)]
+        name, ms, n, per = max(cand, key=lambda t: t[1] * t[2])
+        ach = cells * per / (ms * 1e-3) / 1e9
+        return name, ms, ach, per
+
+    kname, kms, ach, per = roof(mixed, 4)
+    kname64, kms64, ach64, per64 = roof(fp64, 8)
+    b_m = barrier(mixed["energies"][0])
+    b_d = barrier(fp64["energies"][0])
+    rel_b = abs(b_m[0] - b_d[0]) / abs(b_d[0]) if b_d[0] else None
+    e_rel = float(np.max(np.abs(mixed["energies"][0] - fp64["energies"][0]) /
+                         np.abs(fp64["energies"][0])))
+
+    out = {
+        "metric": "image*cells/s per NLCG step (mixed precision)",
+        "value": value,
+        "unit": "image*cells/s",
+        "n_gpus": 1,
+        "steps": mixed["steps"],
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * mixed["seconds"] / mixed["steps"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 fields / f64 reductions",
+        "data": "synthetic seeded vortex-entry initial guess",
+        "config": {"workload": args.workload, "strings": len(strings),
+                   "images": sum(s.F for s in strings), "N": strings[0].N,
+                   "components": strings[0].ncomp, "mode": "L", "reparam_every": R,
+                   "tile_rows": mixed["info"]["tile_rows"]},
+        "fp64": {"value": value64, "ms_per_step": 1e3 * fp64["seconds"] / fp64["steps"],
+                 "roofline": {"kernel": kname64, "achieved": ach64, "peak": HBM_PEAK_GBS,
+                              "unit": "GB/s", "frac": ach64 / HBM_PEAK_GBS,
+                              "avg_ms": kms64}},
+        "mixed_over_fp64": value / value64,
+        "energy_rel_err_mixed_vs_fp64": e_rel,
+        "barrier_rel_err_mixed_vs_fp64": rel_b,
+        "barrier_fp64": b_d[0],
+        "roofline": {"bound": "hbm", "kernel": kname, "achieved": ach, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_image_cell": per, "avg_ms": kms},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(strings, args.cpu_steps)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

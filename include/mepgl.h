@@ -1,0 +1,185 @@
+/*
+ * mepgl.h - C ABI of the MI355X gauged-string-method relaxation library
+ * (libmepgl.so, HIP kernels for gfx950).
+ *
+ * The library replaces the solver layer of the reference (mepgl/src/nlcg.cu,
+ * interpolation.cu, parallel_sum.cu) and its relaxation loop (loop() in
+ * mepgl/src/mepgl.cu:38-391).  The reference passes raw device pointers and
+ * compile-time sizes between host functions; here an opaque context owns all
+ * device memory for one or many strings, sizes are runtime values, and every
+ * entry point returns a status code instead of exiting.
+ *
+ * Host arrays use the reference's device layout (io.cu:23-30): per-site
+ * planes are N*N values with x fastest ([j][i]), fields are F*N*N pairs
+ * (real, imaginary / x, y) with x fastest then y then image, i.e. the
+ * layout of a C-contiguous numpy array of shape (F, N, N, 2).  Host arrays
+ * are borrowed for the duration of a call only.
+ *
+ * Threading: a context is driven by one host thread; mepgl_set_control may
+ * be called from any thread (the stdin thread of the CLI).
+ */
+#ifndef MEPGL_H
+#define MEPGL_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes.  0 is success; mepgl_last_error() describes failures. */
+#define MEPGL_OK 0
+#define MEPGL_ERR_ARG (-1)       /* invalid argument or size            */
+#define MEPGL_ERR_HIP (-2)       /* HIP runtime error                   */
+#define MEPGL_ERR_NONFINITE (-3) /* NaN/Inf in a per-image energy or |g|^2 */
+#define MEPGL_ERR_STATE (-4)     /* call not valid in the current state */
+#define MEPGL_ERR_NOMEM (-5)     /* device allocation failed            */
+
+/* Precision policies.  FP64: every stored field and every operation in
+ * double (DOUBLE_PRECISION build of the reference, real.cuh:7-14).
+ * MIXED: fields, directions and gradients stored and updated in float; all
+ * reductions, NLCG scalars, line search, arclength and reparametrisation in
+ * double (the reductions with two-sum compensation). */
+#define MEPGL_FP64 0
+#define MEPGL_MIXED 1
+
+/* Flags for mepgl_create. */
+#define MEPGL_POLY_EXACT 1u     /* exact line-search quartic instead of the
+                                   reference's (nlcg.cu:1162-1165, 1251)   */
+#define MEPGL_CUBIC_TEXTBOOK 2u /* (1-w)^3-(1-w) spline weight instead of the
+                                   reference's (interpolation.cu:394-400)  */
+#define MEPGL_NO_GRAPH 4u       /* mepgl_loop launches eagerly instead of
+                                   replaying a captured hipGraph           */
+
+/* A per-site parameter: a uniform value, or an N*N plane when plane != NULL. */
+typedef struct {
+  double value;
+  const double *plane;
+} mepgl_param;
+
+/* One string of images.  Replaces the compile-time configuration of the
+ * reference (config.cuh:14-34: N, dx, MULTICOMPONENT, NO_SELF_FIELD) and the
+ * parameter files loaded by loadInput (io.cu:10-126). */
+typedef struct {
+  int N;                 /* grid points per side                          */
+  int F;                 /* images                                        */
+  int ncomp;             /* 1 or 2 order-parameter components             */
+  int no_self_field;     /* 1: A fixed, magnetic term and dF/dA omitted   */
+  double dx;             /* lattice spacing                               */
+  mepgl_param a_1, b_1, m_xx_1, m_yy_1;
+  mepgl_param a_2, b_2, m_xx_2, m_yy_2;
+  mepgl_param h;         /* applied field                                 */
+  double q_1, q_2, eta, gamma, delta; /* q.npy (io.cu:60-67)              */
+  const int32_t *comp_domain; /* N*N, computational domain (nonzero = in) */
+  const int32_t *sc_domain;   /* N*N, superconducting domain              */
+  int rows_per_thread;   /* tile height / 4; 0 lets the library choose    */
+} mepgl_string_desc;
+
+typedef struct mepgl_ctx mepgl_ctx;
+
+/* Creates a context for n_strings strings on HIP device `device`, builds the
+ * neighbour maps (buildNearestNeighbourMap, nlcg.cu:48-103, launched from
+ * mepgl.cu:598-599) and puts the solver in the state loop() starts from
+ * (mepgl.cu:124-140).  Fields start at zero until mepgl_upload. */
+int mepgl_create(const mepgl_string_desc *strings, int n_strings, int precision,
+                 unsigned flags, int device, mepgl_ctx **out);
+void mepgl_destroy(mepgl_ctx *ctx);
+const char *mepgl_last_error(const mepgl_ctx *ctx);
+
+/* Host -> device copy of one string's fields (the host-to-device copies of
+ * mepgl.cu:567-595).  psi_2 may be NULL for one component. */
+int mepgl_upload(mepgl_ctx *ctx, int string, const double *a, const double *psi_1,
+                 const double *psi_2);
+/* Device -> host copy of the current fields (mepgl.cu:791-796). */
+int mepgl_download(mepgl_ctx *ctx, int string, double *a, double *psi_1, double *psi_2);
+
+/* Solver state of loop()'s start (mepgl.cu:124-140): zero gradients and
+ * direction, infinite gradient norms. */
+int mepgl_reset_solver(mepgl_ctx *ctx);
+
+/* computeFreeEnergy (nlcg.cu:217-240) for every image of every string,
+ * evaluated in double; read with MEPGL_SCALAR_ENERGY. */
+int mepgl_free_energy(mepgl_ctx *ctx);
+
+/* The gradient at the current fields (gradientPsi + gradientA,
+ * nlcg.cu:504-888) into host arrays of one string; does not disturb the
+ * solver state.  psi_2_g may be NULL for one component. */
+int mepgl_gradient(mepgl_ctx *ctx, int string, double *a_g, double *psi_1_g, double *psi_2_g);
+
+/* nlcgSteps (nlcg.cu:1462-1586) on every string: iter_n NLCG steps,
+ * Polak-Ribiere beta when nlcg != 0, psi frozen when maxwell_solver != 0.
+ * Returns MEPGL_ERR_NONFINITE if an energy or gradient norm is not finite. */
+int mepgl_nlcg_steps(mepgl_ctx *ctx, int iter_n, int nlcg, int maxwell_solver);
+
+/* computeJB (nlcg.cu:243-380): magnetic field b and supercurrents j_1, j_2
+ * of one string into host arrays (F*N*N, F*N*N*2, F*N*N*2). */
+int mepgl_compute_jb(mepgl_ctx *ctx, int string, double *b, double *j_1, double *j_2);
+
+/* arcLength + computeJB + arcLengthJB (interpolation.cu:143-169,
+ * mepgl.cu:193-201) on every string; read with MEPGL_SCALAR_S and
+ * MEPGL_SCALAR_S_JB. */
+int mepgl_arc_length(mepgl_ctx *ctx);
+
+/* redistributeLinear / redistributeCubic (interpolation.cu:229-250,
+ * 404-481) of every string on its last s_jb, keeping F images. */
+int mepgl_redistribute(mepgl_ctx *ctx, int cubic);
+
+/* The end-of-run resampling of mepgl.cu:699-766: arclength, then linear
+ * redistribution of one string to F_new images (reallocates its buffers). */
+int mepgl_resample(mepgl_ctx *ctx, int string, int F_new);
+
+/* Resets the Polak-Ribiere memory of the interior images
+ * (mepgl.cu:231-234). */
+int mepgl_reset_interior_beta(mepgl_ctx *ctx);
+
+/* loop() (mepgl.cu:148-306): `iterations` outer iterations of mode
+ * 'S' (relaxation), 'M' (Maxwell: psi frozen), 'L' (linear
+ * reparametrisation then relaxation) or 'C' (cubic).  Each outer iteration
+ * is one hipGraph replay unless MEPGL_NO_GRAPH.  Stops early when the stop
+ * control is set. */
+int mepgl_loop(mepgl_ctx *ctx, int iterations, int relaxation_steps, char mode, int nlcg);
+
+/* Per-image scalars of one string (F values, 4F for MEPGL_SCALAR_POLY). */
+#define MEPGL_SCALAR_FENERGY 0   /* line-search A: energy before the update */
+#define MEPGL_SCALAR_G_NORM2 1   /* caller's g_norm2                        */
+#define MEPGL_SCALAR_ALPHA 2
+#define MEPGL_SCALAR_S 3
+#define MEPGL_SCALAR_S_JB 4
+#define MEPGL_SCALAR_BETA 5
+#define MEPGL_SCALAR_G_DELTA_G 6
+#define MEPGL_SCALAR_POLY 7      /* B, C, D, E per image                    */
+#define MEPGL_SCALAR_ENERGY 8    /* last mepgl_free_energy                  */
+#define MEPGL_SCALAR_G_NORM2_OLD 9
+#define MEPGL_SCALAR_LS_INDEX 10 /* scan index where the line search stopped */
+int mepgl_get_scalars(mepgl_ctx *ctx, int string, int which, double *out);
+
+/* Internal solver buffers of one string, for state checks:
+ * 0 = caller's gradient g, 1 = caller's g_old, 2 = direction d. */
+int mepgl_download_state(mepgl_ctx *ctx, int string, int which, double *a, double *psi_1,
+                         double *psi_2);
+
+/* Controls of the stdin thread (mepgl.cu:635-671), safe from any thread. */
+#define MEPGL_CONTROL_STOP 0
+#define MEPGL_CONTROL_RELAXATION_STEPS 1
+#define MEPGL_CONTROL_NLCG 2
+int mepgl_set_control(mepgl_ctx *ctx, int key, int value);
+int mepgl_get_control(mepgl_ctx *ctx, int key, int *value);
+
+/* Per-kernel timing with HIP events on the context's stream.  enable != 0
+ * brackets every launch with events and disables graph replay; read returns
+ * the average duration in ms and the launch count of kernel `kernel`
+ * (0 = G: update + gradient, 1 = P: direction + quartic + line search). */
+int mepgl_profile(mepgl_ctx *ctx, int enable);
+int mepgl_profile_read(mepgl_ctx *ctx, int kernel, double *avg_ms, int *count);
+
+/* Waits for all work of the context. */
+int mepgl_synchronize(mepgl_ctx *ctx);
+
+/* Geometry and memory facts: work items per NLCG launch, bytes of device
+ * memory held, and the tile height chosen for a string. */
+int mepgl_info(mepgl_ctx *ctx, int string, int *tile_rows, int *work_items, long long *bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MEPGL_H */

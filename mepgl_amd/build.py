@@ -1,0 +1,73 @@
+"""Build libmepgl.so (HIP kernels for gfx950 + host context + C ABI) in-tree.
+
+    python -m mepgl_amd.build            # library
+    python -m mepgl_amd.build --cli      # library and the mepgl CLI
+"""
+from __future__ import annotations
+
+import argparse
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libmepgl.so")
+CLI = os.path.join(HERE, "mepgl")
+
+ARCH = "gfx950"
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+# Packed FP32 instructions are disabled for every HIP compile of this project.
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wall", "-Wno-unused-result", "-I", os.path.join(ROOT, "include")]
+
+SOURCES = ["kernels.hip", "context.cpp"]
+CLI_SOURCES = ["cli.cpp", "npyio.cpp"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("compile failed: " + " ".join(cmd[:3]) + " ...")
+    return r
+
+
+def build_library(verbose=False) -> str:
+    objs = []
+    os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
+    for src in SOURCES:
+        obj = os.path.join(HERE, "_build", src + ".o")
+        lang = [] if src.endswith(".hip") else ["-x", "hip"]
+        _run([HIPCC, *HIP_FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    if verbose:
+        print("built", LIB)
+    return LIB
+
+
+def build_cli(verbose=False) -> str:
+    srcs = [os.path.join(CSRC, s) for s in CLI_SOURCES]
+    _run([HIPCC, *HIP_FLAGS, "-x", "hip", *srcs, "-o", CLI, "-L", HERE, "-lmepgl",
+          "-Wl,-rpath,$ORIGIN", "-lpthread"])
+    if verbose:
+        print("built", CLI)
+    return CLI
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--cli", action="store_true")
+    args = ap.parse_args(argv)
+    build_library(verbose=True)
+    if args.cli:
+        build_cli(verbose=True)
+
+
+if __name__ == "__main__":
+    main()

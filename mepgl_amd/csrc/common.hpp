@@ -1,0 +1,97 @@
+// Shared host/device definitions of the MI355X gauged-string-method library.
+//
+// Data layout in HBM (one allocation set per string):
+//   field sets x[2], d[2], g[2]: [image n][plane p][row j][col i], row pitch
+//   `pitch` elements (a multiple of 64), planes p = 0 A_x, 1 A_y, 2 u_1,
+//   3 v_1, 4 u_2, 5 v_2 (2 + 2*ncomp planes).  Column i runs along x, so a
+//   64-lane wavefront reads one contiguous row segment.
+//   nn: uint16 per site, low byte = superconducting-domain neighbour bits,
+//   high byte = computational-domain neighbour bits (bit k: neighbour k,
+//   3 2 1 / 4 X 0 / 5 6 7).
+//   parameter planes (double) only for non-uniform parameters.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mepgl {
+
+constexpr int kWave = 64;          // CDNA wavefront width
+constexpr int kTileX = 64;         // tile width: one wavefront row
+constexpr int kWavesPerBlock = 4;  // 256-thread workgroups
+constexpr int kBlock = kWave * kWavesPerBlock;
+constexpr int kLsSteps = 2000;     // line-search scan length (nlcg.cu:1421)
+constexpr int kNumXcd = 8;         // MI355X accelerator complex dies
+
+enum Plane { PAX = 0, PAY = 1, PU1 = 2, PV1 = 3, PU2 = 4, PV2 = 5 };
+
+struct DevParam {
+  double v;          // uniform value (used when p == nullptr)
+  const double* p;   // per-site plane [j][i] with the string's pitch, or null
+};
+
+// Per-image scalar block (double), indexed by the global image id.
+struct ImageScalars {
+  double g_norm2[2];   // [0]: caller's g_norm2, [1]: caller's g_norm2_old
+  double g_delta_g;
+  double beta;
+  double alpha;
+  double fenergy;      // A coefficient of the last line search
+  double poly[4];      // B, C, D, E
+  double s, s_jb;      // arclength (plain metric, gauge-invariant metric)
+  double ds2, ds2_jb;  // squared distance to the previous image
+  double lin_w;        // linear redistribution weight of this new image
+  int lin_k;           // linear redistribution segment end node
+  int ls_index;        // scan index at which the line search stopped
+};
+
+struct DevString {
+  int N, pitch, F, nc;      // grid side, row pitch, images, components
+  int nplanes;              // 2 + 2*nc
+  int nsf;                  // NO_SELF_FIELD: A is fixed, no A gradient
+  int img0;                 // global image id of image 0
+  int tiles_x, tiles_y;     // tiles per image
+  int rows_per_thread;      // tile height = rows_per_thread * kWavesPerBlock
+  int poly_exact;           // 0: reference line-search quartic (Q1, Q2)
+  long long plane_stride;   // elements per plane (rows * pitch)
+  long long image_stride;   // elements per image (nplanes * plane_stride)
+  double dx, inv_dx, dx2;
+  double q[2], eta, gamma, delta;
+  DevParam a[2], b[2], mxx[2], myy[2], h;
+  const uint16_t* nn;
+  void* x[2];
+  void* d[2];
+  void* g[2];
+  void* gs;                 // scratch gradient (diagnostic entry point)
+  double* part_g;           // [F][ntiles][kPartG] workgroup partials
+  double* part_p;           // [F][ntiles][kPartP]
+  double* part_a;           // [F][ntiles][kPartA] arclength partials
+  unsigned* count_g;        // [F] arrival counters (self-resetting)
+  unsigned* count_p;        // [F]
+  double* cubic;            // [F][4] Thomas factors c', m, sub-diagonal, h
+  void* cubic_rhs;          // cubic right-hand sides [F][plane][rows][pitch]
+  int* plan_k;              // [F] linear redistribution: segment end node
+  double* plan_w;           // [F] linear redistribution: weight of node k
+  int* cplan_k;             // [F] cubic redistribution: segment end node
+  double* cplan_w;          // [F] cubic redistribution: weight of node k
+};
+
+// One work item = one tile of one image of one string.
+struct WorkItem {
+  int string;
+  int image;
+  int tile;   // tx + tiles_x * ty
+  int pad;
+};
+
+constexpr int kPartG = 4;   // g_norm2, g_delta_g (+ compensation terms)
+constexpr int kPartP = 10;  // A..E (+ compensation terms)
+constexpr int kPartA = 4;   // ds2, ds2_jb (+ compensation terms)
+
+struct LineSearchTable {
+  double x[kLsSteps + 1];      // x_k = 1e-10 * m^k, sequential product
+  double x4[kLsSteps + 1];     // pow(x_k, 4)
+  double alpha[kLsSteps];      // x_{k+1} / m
+};
+
+}  // namespace mepgl

@@ -1,0 +1,1044 @@
+// Host side of libmepgl: context, memory, work lists, the NLCG step
+// sequence and the relaxation loop (loop(), mepgl.cu:38-391), behind the C
+// ABI declared in include/mepgl.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/mepgl.h"
+#include "common.hpp"
+#include "kernels.hpp"
+
+using namespace mepgl;
+
+namespace {
+
+struct Alloc {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+struct StringHost {
+  mepgl_string_desc desc;
+  DevString dev;            // host copy of the device descriptor
+  std::vector<Alloc> owned; // device allocations of this string
+  int ntiles = 0;
+  double* jb = nullptr;     // JB planes (lazy)
+  size_t elem = 8;          // bytes per stored real
+};
+
+struct Timing {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  size_t used = 0;
+  std::vector<std::pair<int, std::pair<size_t, size_t>>> marks;  // kernel, (start, stop)
+  double total_ms[kKernelCount] = {0};
+  int count[kKernelCount] = {0};
+};
+
+}  // namespace
+
+struct mepgl_ctx {
+  int precision = MEPGL_FP64;
+  unsigned flags = 0;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<StringHost> strings;
+  int total_images = 0;
+  DevString* d_strings = nullptr;
+  ImageScalars* d_sc = nullptr;
+  LineSearchTable* d_ls = nullptr;
+  WorkItem* d_items = nullptr;
+  int nitems = 0;
+  double** d_jb = nullptr;
+  double* d_out = nullptr;
+  std::string err;
+  int xp = 0, dp = 0;        // current field / direction buffers
+  bool pending = false;      // an update x += alpha d is owed
+  std::atomic<int> stop{0}, relax_steps{25}, nlcg{1};
+  Timing timing;
+  bool dry = false;          // advance the parity state without launching
+  std::map<std::tuple<int, int, int, int, int, char>, hipGraphExec_t> graphs;
+  long long bytes = 0;
+
+  bool fp32() const { return precision == MEPGL_MIXED; }
+};
+
+namespace {
+
+int fail(mepgl_ctx* c, int code, const std::string& msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+#define HIP_TRY(ctx, expr)                                                              \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(ctx, MEPGL_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int dev_alloc(mepgl_ctx* c, StringHost* s, void** p, size_t bytes) {
+  *p = nullptr;
+  if (bytes == 0) return MEPGL_OK;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(c, MEPGL_ERR_NOMEM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " +
+                                        hipGetErrorString(e));
+  }
+  if (s) s->owned.push_back({*p, bytes});
+  c->bytes += (long long)bytes;
+  return MEPGL_OK;
+}
+
+void free_string(mepgl_ctx* c, StringHost& s) {
+  for (auto& a : s.owned) {
+    hipFree(a.p);
+    c->bytes -= (long long)a.bytes;
+  }
+  s.owned.clear();
+}
+
+// Tile height: 4 * rows_per_thread rows.  Aim for enough workgroups per
+// launch to cover the 256 CUs several times over while keeping tiles tall
+// enough that halo rows stay a small fraction of the rows read.
+int choose_rows(const mepgl_string_desc& d, int total_images_hint) {
+  if (d.rows_per_thread > 0) return d.rows_per_thread;
+  const long long cols = (d.N + kTileX - 1) / kTileX;
+  for (int r : {8, 4, 2}) {
+    const long long th = 4LL * r;
+    const long long tiles = cols * ((d.N + th - 1) / th) * (long long)total_images_hint;
+    if (th <= d.N && tiles >= 4096) return r;
+  }
+  return 1;
+}
+
+bool param_ok(const mepgl_param& p) { return p.plane != nullptr || std::isfinite(p.value); }
+
+void build_line_search_table(LineSearchTable* t) {
+  // lineSearch, nlcg.cu:1414-1436: x_0 = 1e-10, x_{k+1} = x_k * 1.025f.
+  const double m = (double)1.025f;
+  double x = 1e-10;
+  for (int k = 0; k <= kLsSteps; ++k) {
+    t->x[k] = x;
+    t->x4[k] = std::pow(x, 4);
+    if (k < kLsSteps) {
+      const double xn = x * m;
+      t->alpha[k] = xn / m;
+      x = xn;
+    }
+  }
+}
+
+StepArgs step_args(mepgl_ctx* c) {
+  StepArgs a{};
+  a.strings = c->d_strings;
+  a.items = c->d_items;
+  a.nitems = c->nitems;
+  a.sc = c->d_sc;
+  a.ls = c->d_ls;
+  a.nlcg = 1;
+  return a;
+}
+
+AuxArgs aux_args(mepgl_ctx* c) {
+  AuxArgs a{};
+  a.strings = c->d_strings;
+  a.items = c->d_items;
+  a.nitems = c->nitems;
+  a.nstrings = (int)c->strings.size();
+  a.sc = c->d_sc;
+  a.xin = c->xp;
+  a.xout = 1 - c->xp;
+  a.out = c->d_out;
+  a.jb = c->d_jb;
+  return a;
+}
+
+hipEvent_t take_event(mepgl_ctx* c) {
+  Timing& t = c->timing;
+  if (t.used == t.pool.size()) {
+    hipEvent_t e;
+    hipEventCreate(&e);
+    t.pool.push_back(e);
+  }
+  return t.pool[t.used++];
+}
+
+int run_step(mepgl_ctx* c, const StepArgs& a, int which) {
+  if (c->dry) return MEPGL_OK;
+  if (c->timing.on && which != kKernelUpdate) {
+    const size_t s0 = c->timing.used;
+    hipEventRecord(take_event(c), c->stream);
+    hipError_t e = launch_step(a, which, c->fp32(), c->stream);
+    hipEventRecord(take_event(c), c->stream);
+    // Timing ids: 0 = G with the pending update, 1 = P, 2 = G without it.
+    const int tid = (which == kKernelGrad && !a.upd) ? (int)kKernelUpdate : which;
+    c->timing.marks.push_back({tid, {s0, s0 + 1}});
+    if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, std::string("launch: ") + hipGetErrorString(e));
+    return MEPGL_OK;
+  }
+  hipError_t e = launch_step(a, which, c->fp32(), c->stream);
+  if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, std::string("launch: ") + hipGetErrorString(e));
+  return MEPGL_OK;
+}
+
+int run_aux(mepgl_ctx* c, const AuxArgs& a, int which, int arg) {
+  if (c->dry) return MEPGL_OK;
+  hipError_t e = launch_aux(a, which, arg, c->fp32(), c->stream);
+  if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, std::string("launch: ") + hipGetErrorString(e));
+  return MEPGL_OK;
+}
+
+// Applies an owed update in place.
+int flush(mepgl_ctx* c) {
+  if (!c->pending) return MEPGL_OK;
+  StepArgs a = step_args(c);
+  a.xin = c->xp;
+  a.din = c->dp;
+  int rc = run_step(c, a, kKernelUpdate);
+  if (rc) return rc;
+  c->pending = false;
+  return MEPGL_OK;
+}
+
+// One NLCG step k of a call (nlcg.cu:1478-1583).  The reference swaps local
+// copies of the gradient pointers (nlcg.cu:1484-1498), so step k writes the
+// caller's g_old buffer when k is even and its g buffer when k is odd.
+int nlcg_step(mepgl_ctx* c, int k, int nlcg, int maxwell) {
+  const int gcur = (k % 2 == 0) ? 1 : 0;
+  StepArgs a = step_args(c);
+  a.gcur = gcur;
+  a.gold = 1 - gcur;
+  a.scur = gcur;
+  a.sold = 1 - gcur;
+  a.nlcg = nlcg;
+  a.maxwell = maxwell;
+  a.xin = c->xp;
+  a.din = c->dp;
+  a.upd = c->pending ? 1 : 0;
+  a.xout = 1 - c->xp;
+  int rc = run_step(c, a, kKernelGrad);
+  if (rc) return rc;
+  if (c->pending) {
+    c->xp = 1 - c->xp;
+    c->pending = false;
+  }
+  a.upd = 0;
+  a.xin = c->xp;
+  a.din = c->dp;
+  a.dout = 1 - c->dp;
+  rc = run_step(c, a, kKernelPoly);
+  if (rc) return rc;
+  c->dp = 1 - c->dp;
+  c->pending = true;
+  return MEPGL_OK;
+}
+
+int check_finite(mepgl_ctx* c) {
+  std::vector<ImageScalars> h(c->total_images);
+  HIP_TRY(c, hipMemcpyAsync(h.data(), c->d_sc, sizeof(ImageScalars) * h.size(),
+                            hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (size_t si = 0; si < c->strings.size(); ++si) {
+    const DevString& s = c->strings[si].dev;
+    for (int n = 0; n < s.F; ++n) {
+      const ImageScalars& x = h[s.img0 + n];
+      if (!std::isfinite(x.fenergy) || std::isnan(x.g_norm2[0]) || std::isnan(x.g_norm2[1]))
+        return fail(c, MEPGL_ERR_NONFINITE, "non-finite energy or gradient norm in string " +
+                                                std::to_string(si) + " image " + std::to_string(n));
+    }
+  }
+  return MEPGL_OK;
+}
+
+int build_items(mepgl_ctx* c) {
+  std::vector<WorkItem> items;
+  for (size_t si = 0; si < c->strings.size(); ++si) {
+    const DevString& s = c->strings[si].dev;
+    for (int n = 0; n < s.F; ++n)
+      for (int t = 0; t < s.tiles_x * s.tiles_y; ++t) items.push_back({(int)si, n, t, 0});
+  }
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (c->d_items) hipFree(c->d_items);
+  c->d_items = nullptr;
+  c->nitems = (int)items.size();
+  HIP_TRY(c, hipMalloc(&c->d_items, sizeof(WorkItem) * std::max<size_t>(items.size(), 1)));
+  HIP_TRY(c, hipMemcpyAsync(c->d_items, items.data(), sizeof(WorkItem) * items.size(),
+                            hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return MEPGL_OK;
+}
+
+int upload_descriptors(mepgl_ctx* c) {
+  // The descriptor arrays keep their addresses for the life of the context,
+  // so captured graphs stay valid when a descriptor changes.
+  std::vector<DevString> ds;
+  for (auto& s : c->strings) ds.push_back(s.dev);
+  if (!c->d_strings) HIP_TRY(c, hipMalloc(&c->d_strings, sizeof(DevString) * ds.size()));
+  HIP_TRY(c, hipMemcpyAsync(c->d_strings, ds.data(), sizeof(DevString) * ds.size(),
+                            hipMemcpyHostToDevice, c->stream));
+  std::vector<double*> jb;
+  for (auto& s : c->strings) jb.push_back(s.jb);
+  if (!c->d_jb) HIP_TRY(c, hipMalloc(&c->d_jb, sizeof(double*) * jb.size()));
+  HIP_TRY(c, hipMemcpyAsync(c->d_jb, jb.data(), sizeof(double*) * jb.size(),
+                            hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return MEPGL_OK;
+}
+
+int upload_param(mepgl_ctx* c, StringHost& s, const mepgl_param& p, DevParam& out) {
+  out.v = p.value;
+  out.p = nullptr;
+  if (!p.plane) return MEPGL_OK;
+  const int N = s.desc.N, P = s.dev.pitch;
+  std::vector<double> h((size_t)N * P, 0.0);
+  for (int j = 0; j < N; ++j)
+    for (int i = 0; i < N; ++i) h[(size_t)j * P + i] = p.plane[(size_t)j * N + i];
+  void* d;
+  int rc = dev_alloc(c, &s, &d, h.size() * sizeof(double));
+  if (rc) return rc;
+  HIP_TRY(c, hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                            c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  out.p = (const double*)d;
+  return MEPGL_OK;
+}
+
+// Allocates the per-image buffers of a string for F images.
+int alloc_images(mepgl_ctx* c, StringHost& s, int F) {
+  DevString& d = s.dev;
+  d.F = F;
+  const size_t fs = (size_t)F * d.image_stride * s.elem;
+  for (int b = 0; b < 2; ++b) {
+    int rc;
+    if ((rc = dev_alloc(c, &s, &d.x[b], fs))) return rc;
+    if ((rc = dev_alloc(c, &s, &d.d[b], fs))) return rc;
+    if ((rc = dev_alloc(c, &s, &d.g[b], fs))) return rc;
+    HIP_TRY(c, hipMemsetAsync(d.x[b], 0, fs, c->stream));
+    HIP_TRY(c, hipMemsetAsync(d.d[b], 0, fs, c->stream));
+    HIP_TRY(c, hipMemsetAsync(d.g[b], 0, fs, c->stream));
+  }
+  d.gs = nullptr;
+  d.cubic_rhs = nullptr;
+  s.jb = nullptr;
+  const size_t nt = (size_t)F * s.ntiles;
+  void* p;
+  int rc;
+  if ((rc = dev_alloc(c, &s, &p, nt * kPartG * sizeof(double)))) return rc;
+  d.part_g = (double*)p;
+  if ((rc = dev_alloc(c, &s, &p, nt * kPartP * sizeof(double)))) return rc;
+  d.part_p = (double*)p;
+  if ((rc = dev_alloc(c, &s, &p, nt * kPartA * sizeof(double)))) return rc;
+  d.part_a = (double*)p;
+  if ((rc = dev_alloc(c, &s, &p, (size_t)F * sizeof(unsigned) * 2))) return rc;
+  HIP_TRY(c, hipMemsetAsync(p, 0, (size_t)F * sizeof(unsigned) * 2, c->stream));
+  d.count_g = (unsigned*)p;
+  d.count_p = d.count_g + F;
+  if ((rc = dev_alloc(c, &s, &p, (size_t)F * 4 * sizeof(double)))) return rc;
+  HIP_TRY(c, hipMemsetAsync(p, 0, (size_t)F * 4 * sizeof(double), c->stream));
+  d.cubic = (double*)p;
+  if ((rc = dev_alloc(c, &s, &p, (size_t)F * (2 * sizeof(int) + 2 * sizeof(double))))) return rc;
+  HIP_TRY(c, hipMemsetAsync(p, 0, (size_t)F * (2 * sizeof(int) + 2 * sizeof(double)), c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  d.plan_w = (double*)p;
+  d.cplan_w = d.plan_w + F;
+  d.plan_k = (int*)(d.cplan_w + F);
+  d.cplan_k = d.plan_k + F;
+  return MEPGL_OK;
+}
+
+int alloc_scalars(mepgl_ctx* c) {
+  int img = 0;
+  for (auto& s : c->strings) {
+    s.dev.img0 = img;
+    img += s.dev.F;
+  }
+  c->total_images = img;
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (c->d_sc) hipFree(c->d_sc);
+  if (c->d_out) hipFree(c->d_out);
+  c->d_sc = nullptr;
+  c->d_out = nullptr;
+  HIP_TRY(c, hipMalloc(&c->d_sc, sizeof(ImageScalars) * img));
+  HIP_TRY(c, hipMemsetAsync(c->d_sc, 0, sizeof(ImageScalars) * img, c->stream));
+  HIP_TRY(c, hipMalloc(&c->d_out, sizeof(double) * img));
+  HIP_TRY(c, hipMemsetAsync(c->d_out, 0, sizeof(double) * img, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return MEPGL_OK;
+}
+
+void destroy_graphs(mepgl_ctx* c) {
+  for (auto& g : c->graphs) hipGraphExecDestroy(g.second);
+  c->graphs.clear();
+}
+
+int reset_solver(mepgl_ctx* c) {
+  for (auto& s : c->strings) {
+    const long long n = (long long)s.dev.F * s.dev.image_stride;
+    for (int b = 0; b < 2; ++b) {
+      HIP_TRY(c, launch_fill(s.dev.g[b], n, c->fp32(), 0.0, c->stream));
+      HIP_TRY(c, launch_fill(s.dev.d[b], n, c->fp32(), 0.0, c->stream));
+    }
+  }
+  AuxArgs a = aux_args(c);
+  int rc = run_aux(c, a, kKernelResetNorms, 0);
+  if (rc) return rc;
+  c->pending = false;
+  return MEPGL_OK;
+}
+
+bool valid_string(mepgl_ctx* c, int s) { return c && s >= 0 && s < (int)c->strings.size(); }
+
+// Field conversion between the host layout (F, N, N, 2) and the device planes.
+template <class T>
+void pack_fields(const StringHost& s, const double* a, const double* p1, const double* p2,
+                 std::vector<T>& out) {
+  const DevString& d = s.dev;
+  const int N = d.N, P = d.pitch, F = d.F;
+  out.assign((size_t)F * d.image_stride, (T)0);
+  const double* src[3] = {a, p1, p2};
+  for (int n = 0; n < F; ++n)
+    for (int f = 0; f < 1 + d.nc; ++f) {
+      if (!src[f]) continue;
+      for (int comp = 0; comp < 2; ++comp) {
+        T* dst = out.data() + (size_t)n * d.image_stride + (size_t)(2 * f + comp) * d.plane_stride;
+        const double* sp = src[f] + (size_t)n * N * N * 2;
+        for (int j = 0; j < N; ++j)
+          for (int i = 0; i < N; ++i) dst[(size_t)j * P + i] = (T)sp[((size_t)j * N + i) * 2 + comp];
+      }
+    }
+}
+
+template <class T>
+void unpack_fields(const StringHost& s, const std::vector<T>& in, double* a, double* p1,
+                   double* p2) {
+  const DevString& d = s.dev;
+  const int N = d.N, P = d.pitch, F = d.F;
+  double* dst[3] = {a, p1, p2};
+  for (int n = 0; n < F; ++n)
+    for (int f = 0; f < 3; ++f) {
+      if (!dst[f]) continue;
+      for (int comp = 0; comp < 2; ++comp) {
+        double* dp = dst[f] + (size_t)n * N * N * 2;
+        if (f > d.nc) {
+          for (int j = 0; j < N; ++j)
+            for (int i = 0; i < N; ++i) dp[((size_t)j * N + i) * 2 + comp] = 0.0;
+          continue;
+        }
+        const T* sp = in.data() + (size_t)n * d.image_stride + (size_t)(2 * f + comp) * d.plane_stride;
+        for (int j = 0; j < N; ++j)
+          for (int i = 0; i < N; ++i) dp[((size_t)j * N + i) * 2 + comp] = (double)sp[(size_t)j * P + i];
+      }
+    }
+}
+
+int copy_out_fields(mepgl_ctx* c, int si, const void* dev, double* a, double* p1, double* p2) {
+  StringHost& s = c->strings[si];
+  const size_t n = (size_t)s.dev.F * s.dev.image_stride;
+  if (c->fp32()) {
+    std::vector<float> h(n);
+    HIP_TRY(c, hipMemcpyAsync(h.data(), dev, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    unpack_fields(s, h, a, p1, p2);
+  } else {
+    std::vector<double> h(n);
+    HIP_TRY(c, hipMemcpyAsync(h.data(), dev, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    unpack_fields(s, h, a, p1, p2);
+  }
+  return MEPGL_OK;
+}
+
+int arc_length_all(mepgl_ctx* c) {
+  int rc = flush(c);
+  if (rc) return rc;
+  AuxArgs a = aux_args(c);
+  if ((rc = run_aux(c, a, kKernelArclen, 0))) return rc;
+  return run_aux(c, a, kKernelArcFinalize, (int)c->strings.size());
+}
+
+int ensure_cubic_rhs(mepgl_ctx* c) {
+  bool changed = false;
+  for (auto& s : c->strings) {
+    if (s.dev.cubic_rhs) continue;
+    int rc = dev_alloc(c, &s, &s.dev.cubic_rhs,
+                       (size_t)s.dev.F * s.dev.image_stride * sizeof(double));
+    if (rc) return rc;
+    changed = true;
+  }
+  return changed ? upload_descriptors(c) : MEPGL_OK;
+}
+
+// Redistribution of all strings onto the other field buffer (uses the plan
+// computed by the last arclength pass).
+int redistribute_all(mepgl_ctx* c, int cubic) {
+  int rc = flush(c);
+  if (rc) return rc;
+  AuxArgs a = aux_args(c);
+  if (cubic) {
+    for (auto& s : c->strings)
+      if (s.dev.F < 4)
+        return fail(c, MEPGL_ERR_ARG, "cubic reparametrisation needs at least 4 images");
+    if ((rc = run_aux(c, a, kKernelCubic, (c->flags & MEPGL_CUBIC_TEXTBOOK) ? 1 : 0))) return rc;
+  } else {
+    if ((rc = run_aux(c, a, kKernelLinear, 0))) return rc;
+  }
+  // NO_SELF_FIELD strings keep the same fixed A in both field buffers: copy
+  // the new A (buffer xout) over the old one (buffer xin).
+  if ((rc = run_aux(c, a, kKernelCopyA, 0))) return rc;
+  c->xp = 1 - c->xp;
+  return MEPGL_OK;
+}
+
+// One outer iteration of loop() (mepgl.cu:153-303).
+int outer_iteration(mepgl_ctx* c, int relax, char mode, int nlcg) {
+  int rc;
+  if (mode == 'L' || mode == 'C') {
+    if ((rc = arc_length_all(c))) return rc;
+    if ((rc = redistribute_all(c, mode == 'C'))) return rc;
+    AuxArgs a = aux_args(c);
+    if ((rc = run_aux(c, a, kKernelResetNorms, 1))) return rc;
+  }
+  const bool maxwell = mode == 'M';
+  for (int k = 0; k < relax; ++k)
+    if ((rc = nlcg_step(c, k, nlcg, maxwell ? 1 : 0))) return rc;
+  return MEPGL_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C ABI
+extern "C" {
+
+int mepgl_create(const mepgl_string_desc* strings, int n_strings, int precision, unsigned flags,
+                 int device, mepgl_ctx** out) {
+  if (!out) return MEPGL_ERR_ARG;
+  *out = nullptr;
+  if (!strings || n_strings <= 0) return MEPGL_ERR_ARG;
+  if (precision != MEPGL_FP64 && precision != MEPGL_MIXED) return MEPGL_ERR_ARG;
+  std::unique_ptr<mepgl_ctx> c(new mepgl_ctx());
+  c->precision = precision;
+  c->flags = flags;
+  c->device = device;
+  hipError_t e = hipSetDevice(device);
+  if (e != hipSuccess) return MEPGL_ERR_HIP;
+  e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) return MEPGL_ERR_HIP;
+
+  long long total_img = 0;
+  for (int i = 0; i < n_strings; ++i) total_img += std::max(strings[i].F, 0);
+
+  c->strings.resize(n_strings);
+  for (int si = 0; si < n_strings; ++si) {
+    const mepgl_string_desc& in = strings[si];
+    const mepgl_param* ps[] = {&in.a_1, &in.b_1, &in.m_xx_1, &in.m_yy_1, &in.a_2,
+                               &in.b_2, &in.m_xx_2, &in.m_yy_2, &in.h};
+    bool ok = in.N >= 2 && in.F >= 2 && (in.ncomp == 1 || in.ncomp == 2) && in.dx > 0 &&
+              std::isfinite(in.dx) && in.comp_domain && in.sc_domain;
+    for (auto* p : ps) ok = ok && param_ok(*p);
+    if (!ok) {
+      mepgl_ctx* raw = c.release();
+      *out = raw;
+      return fail(raw, MEPGL_ERR_ARG, "invalid string descriptor " + std::to_string(si));
+    }
+    StringHost& s = c->strings[si];
+    s.desc = in;
+    s.elem = c->fp32() ? 4 : 8;
+    DevString& d = s.dev;
+    std::memset(&d, 0, sizeof(d));
+    d.N = in.N;
+    d.pitch = (in.N + 63) / 64 * 64;
+    d.nc = in.ncomp;
+    d.nplanes = 2 + 2 * in.ncomp;
+    d.nsf = in.no_self_field ? 1 : 0;
+    d.poly_exact = (flags & MEPGL_POLY_EXACT) ? 1 : 0;
+    d.rows_per_thread = choose_rows(in, (int)total_img);
+    d.tiles_x = (in.N + kTileX - 1) / kTileX;
+    d.tiles_y = (in.N + 4 * d.rows_per_thread - 1) / (4 * d.rows_per_thread);
+    s.ntiles = d.tiles_x * d.tiles_y;
+    d.plane_stride = (long long)in.N * d.pitch;
+    d.image_stride = (long long)d.nplanes * d.plane_stride;
+    d.dx = in.dx;
+    d.inv_dx = 1.0 / in.dx;
+    d.dx2 = in.dx * in.dx;
+    d.q[0] = in.q_1;
+    d.q[1] = in.q_2;
+    d.eta = in.eta;
+    d.gamma = in.gamma;
+    d.delta = in.delta;
+    int rc;
+    mepgl_ctx* cp = c.get();
+#define UP(field, dst)                                          \
+  if ((rc = upload_param(cp, s, in.field, dst))) {              \
+    *out = c.release();                                         \
+    return rc;                                                  \
+  }
+    UP(a_1, d.a[0]) UP(b_1, d.b[0]) UP(m_xx_1, d.mxx[0]) UP(m_yy_1, d.myy[0])
+    UP(a_2, d.a[1]) UP(b_2, d.b[1]) UP(m_xx_2, d.mxx[1]) UP(m_yy_2, d.myy[1])
+    UP(h, d.h)
+#undef UP
+    // Neighbour maps.
+    void *dsc = nullptr, *dcm = nullptr, *nn = nullptr;
+    const size_t db = (size_t)in.N * in.N * sizeof(int32_t);
+    if ((rc = dev_alloc(cp, &s, &nn, (size_t)in.N * d.pitch * sizeof(uint16_t))) ||
+        (rc = dev_alloc(cp, nullptr, &dsc, db)) || (rc = dev_alloc(cp, nullptr, &dcm, db))) {
+      *out = c.release();
+      return rc;
+    }
+    hipMemsetAsync(nn, 0, (size_t)in.N * d.pitch * sizeof(uint16_t), c->stream);
+    hipMemcpyAsync(dsc, in.sc_domain, db, hipMemcpyHostToDevice, c->stream);
+    hipMemcpyAsync(dcm, in.comp_domain, db, hipMemcpyHostToDevice, c->stream);
+    launch_nn_map((const int*)dsc, (const int*)dcm, in.N, d.pitch, (uint16_t*)nn, c->stream);
+    hipStreamSynchronize(c->stream);
+    hipFree(dsc);
+    hipFree(dcm);
+    c->bytes -= 2 * (long long)db;
+    d.nn = (const uint16_t*)nn;
+    if ((rc = alloc_images(cp, s, in.F))) {
+      *out = c.release();
+      return rc;
+    }
+  }
+  mepgl_ctx* cp = c.get();
+  int rc;
+  if ((rc = alloc_scalars(cp)) || (rc = upload_descriptors(cp)) || (rc = build_items(cp))) {
+    *out = c.release();
+    return rc;
+  }
+  LineSearchTable* t = new LineSearchTable;
+  build_line_search_table(t);
+  if (hipMalloc(&c->d_ls, sizeof(LineSearchTable)) != hipSuccess) {
+    delete t;
+    *out = c.release();
+    return fail(*out, MEPGL_ERR_NOMEM, "line-search table");
+  }
+  hipMemcpyAsync(c->d_ls, t, sizeof(LineSearchTable), hipMemcpyHostToDevice, c->stream);
+  hipStreamSynchronize(c->stream);
+  delete t;
+  if ((rc = reset_solver(cp))) {
+    *out = c.release();
+    return rc;
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) {
+    *out = c.release();
+    return fail(*out, MEPGL_ERR_HIP, "initialisation");
+  }
+  *out = c.release();
+  return MEPGL_OK;
+}
+
+void mepgl_destroy(mepgl_ctx* c) {
+  if (!c) return;
+  hipSetDevice(c->device);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  destroy_graphs(c);
+  for (auto& s : c->strings) free_string(c, s);
+  hipFree(c->d_strings);
+  hipFree(c->d_sc);
+  hipFree(c->d_ls);
+  hipFree(c->d_items);
+  hipFree(c->d_jb);
+  hipFree(c->d_out);
+  for (auto e : c->timing.pool) hipEventDestroy(e);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+const char* mepgl_last_error(const mepgl_ctx* c) { return c ? c->err.c_str() : "no context"; }
+
+int mepgl_upload(mepgl_ctx* c, int si, const double* a, const double* p1, const double* p2) {
+  if (!valid_string(c, si) || !a || !p1) return fail(c, MEPGL_ERR_ARG, "upload: bad arguments");
+  StringHost& s = c->strings[si];
+  if (s.dev.nc == 2 && !p2) return fail(c, MEPGL_ERR_ARG, "upload: psi_2 required");
+  hipSetDevice(c->device);
+  int rc = flush(c);  // other strings keep their owed update
+  if (rc) return rc;
+  const size_t n = (size_t)s.dev.F * s.dev.image_stride;
+  for (int b = 0; b < 2; ++b) {
+    if (c->fp32()) {
+      std::vector<float> h;
+      pack_fields(s, a, p1, s.dev.nc == 2 ? p2 : nullptr, h);
+      HIP_TRY(c, hipMemcpyAsync(s.dev.x[b], h.data(), n * sizeof(float), hipMemcpyHostToDevice,
+                                c->stream));
+      HIP_TRY(c, hipStreamSynchronize(c->stream));
+    } else {
+      std::vector<double> h;
+      pack_fields(s, a, p1, s.dev.nc == 2 ? p2 : nullptr, h);
+      HIP_TRY(c, hipMemcpyAsync(s.dev.x[b], h.data(), n * sizeof(double), hipMemcpyHostToDevice,
+                                c->stream));
+      HIP_TRY(c, hipStreamSynchronize(c->stream));
+    }
+  }
+  return MEPGL_OK;
+}
+
+int mepgl_download(mepgl_ctx* c, int si, double* a, double* p1, double* p2) {
+  if (!valid_string(c, si) || !a || !p1) return fail(c, MEPGL_ERR_ARG, "download: bad arguments");
+  hipSetDevice(c->device);
+  int rc = flush(c);
+  if (rc) return rc;
+  return copy_out_fields(c, si, c->strings[si].dev.x[c->xp], a, p1, p2);
+}
+
+int mepgl_reset_solver(mepgl_ctx* c) {
+  if (!c) return MEPGL_ERR_ARG;
+  hipSetDevice(c->device);
+  int rc = flush(c);
+  if (rc) return rc;
+  return reset_solver(c);
+}
+
+int mepgl_free_energy(mepgl_ctx* c) {
+  if (!c) return MEPGL_ERR_ARG;
+  hipSetDevice(c->device);
+  int rc = flush(c);
+  if (rc) return rc;
+  AuxArgs a = aux_args(c);
+  return run_aux(c, a, kKernelEnergy, 0);
+}
+
+int mepgl_gradient(mepgl_ctx* c, int si, double* a_g, double* p1_g, double* p2_g) {
+  if (!valid_string(c, si) || !a_g || !p1_g) return fail(c, MEPGL_ERR_ARG, "gradient: bad arguments");
+  hipSetDevice(c->device);
+  int rc = flush(c);
+  if (rc) return rc;
+  bool changed = false;
+  for (auto& s : c->strings) {
+    if (s.dev.gs) continue;
+    const size_t fs = (size_t)s.dev.F * s.dev.image_stride * s.elem;
+    if ((rc = dev_alloc(c, &s, &s.dev.gs, fs))) return rc;
+    HIP_TRY(c, hipMemsetAsync(s.dev.gs, 0, fs, c->stream));
+    changed = true;
+  }
+  if (changed && (rc = upload_descriptors(c))) return rc;
+  StepArgs a = step_args(c);
+  a.xin = a.xout = c->xp;
+  a.din = c->dp;
+  a.scratch = 1;
+  if ((rc = run_step(c, a, kKernelGrad))) return rc;
+  return copy_out_fields(c, si, c->strings[si].dev.gs, a_g, p1_g, p2_g);
+}
+
+int mepgl_nlcg_steps(mepgl_ctx* c, int iter_n, int nlcg, int maxwell) {
+  if (!c || iter_n < 0) return MEPGL_ERR_ARG;
+  hipSetDevice(c->device);
+  for (int k = 0; k < iter_n; ++k) {
+    int rc = nlcg_step(c, k, nlcg ? 1 : 0, maxwell ? 1 : 0);
+    if (rc) return rc;
+  }
+  return check_finite(c);
+}
+
+int mepgl_compute_jb(mepgl_ctx* c, int si, double* b, double* j1, double* j2) {
+  if (!valid_string(c, si) || !b || !j1) return fail(c, MEPGL_ERR_ARG, "compute_jb: bad arguments");
+  hipSetDevice(c->device);
+  int rc = flush(c);
+  if (rc) return rc;
+  bool changed = false;
+  for (auto& s : c->strings) {
+    if (s.jb) continue;
+    void* p;
+    if ((rc = dev_alloc(c, &s, &p, (size_t)s.dev.F * (1 + 2 * s.dev.nc) * s.dev.plane_stride *
+                                       sizeof(double))))
+      return rc;
+    s.jb = (double*)p;
+    changed = true;
+  }
+  if (changed && (rc = upload_descriptors(c))) return rc;
+  AuxArgs a = aux_args(c);
+  if ((rc = run_aux(c, a, kKernelJB, 0))) return rc;
+  StringHost& s = c->strings[si];
+  const DevString& d = s.dev;
+  const int np = 1 + 2 * d.nc;
+  std::vector<double> h((size_t)d.F * np * d.plane_stride);
+  HIP_TRY(c, hipMemcpyAsync(h.data(), s.jb, h.size() * sizeof(double), hipMemcpyDeviceToHost,
+                            c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  const int N = d.N, P = d.pitch;
+  for (int n = 0; n < d.F; ++n) {
+    const double* base = h.data() + (size_t)n * np * d.plane_stride;
+    for (int j = 0; j < N; ++j)
+      for (int i = 0; i < N; ++i) {
+        const size_t o = (size_t)j * P + i, q = (size_t)n * N * N + (size_t)j * N + i;
+        b[q] = base[o];
+        j1[2 * q] = base[d.plane_stride + o];
+        j1[2 * q + 1] = base[2 * d.plane_stride + o];
+        if (j2) {
+          j2[2 * q] = d.nc == 2 ? base[3 * d.plane_stride + o] : 0.0;
+          j2[2 * q + 1] = d.nc == 2 ? base[4 * d.plane_stride + o] : 0.0;
+        }
+      }
+  }
+  return MEPGL_OK;
+}
+
+int mepgl_arc_length(mepgl_ctx* c) {
+  if (!c) return MEPGL_ERR_ARG;
+  hipSetDevice(c->device);
+  return arc_length_all(c);
+}
+
+int mepgl_redistribute(mepgl_ctx* c, int cubic) {
+  if (!c) return MEPGL_ERR_ARG;
+  hipSetDevice(c->device);
+  int rc;
+  if (cubic && (rc = ensure_cubic_rhs(c))) return rc;
+  return redistribute_all(c, cubic);
+}
+
+int mepgl_reset_interior_beta(mepgl_ctx* c) {
+  if (!c) return MEPGL_ERR_ARG;
+  hipSetDevice(c->device);
+  AuxArgs a = aux_args(c);
+  return run_aux(c, a, kKernelResetNorms, 1);
+}
+
+int mepgl_resample(mepgl_ctx* c, int si, int F_new) {
+  if (!valid_string(c, si) || F_new < 2) return fail(c, MEPGL_ERR_ARG, "resample: bad arguments");
+  hipSetDevice(c->device);
+  int rc = arc_length_all(c);
+  if (rc) return rc;
+  StringHost& s = c->strings[si];
+  const int F = s.dev.F;
+  std::vector<ImageScalars> h(c->total_images);
+  HIP_TRY(c, hipMemcpyAsync(h.data(), c->d_sc, sizeof(ImageScalars) * h.size(),
+                            hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  std::vector<double> S(F), sp(F), sj(F);
+  for (int n = 0; n < F; ++n) {
+    S[n] = h[s.dev.img0 + n].s_jb;
+    sp[n] = h[s.dev.img0 + n].s;
+  }
+  // generateNewFramesLinear's segment search (interpolation.cu:200-219).
+  std::vector<int> pk(F_new);
+  std::vector<double> pw(F_new);
+  int k = 1;
+  double seg_start = 0, seg_end = S[1];
+  for (int n = 0; n < F_new; ++n) {
+    const double s_new = (double)n / ((double)F_new - 1);
+    while (seg_end < s_new && k + 1 < F) {
+      ++k;
+      seg_start = S[k - 1];
+      seg_end = S[k];
+    }
+    pk[n] = k;
+    pw[n] = (s_new - seg_start) / (S[k] - S[k - 1]);
+  }
+  // Old fields to host, resample there in double with the kernel's formula,
+  // then rebuild the string's buffers for F_new images.
+  std::vector<double> fa((size_t)F * s.dev.N * s.dev.N * 2), f1(fa.size()), f2(fa.size());
+  if ((rc = copy_out_fields(c, si, s.dev.x[c->xp], fa.data(), f1.data(), f2.data()))) return rc;
+  const size_t img = (size_t)s.dev.N * s.dev.N * 2;
+  std::vector<double> na(img * F_new), n1(img * F_new), n2(img * F_new);
+  for (int n = 0; n < F_new; ++n) {
+    const double w = pw[n];
+    for (size_t e = 0; e < img; ++e) {
+      const size_t o1 = (size_t)pk[n] * img + e, o0 = o1 - img;
+      na[n * img + e] = std::fma(w, fa[o1], (1 - w) * fa[o0]);
+      n1[n * img + e] = std::fma(w, f1[o1], (1 - w) * f1[o0]);
+      n2[n * img + e] = std::fma(w, f2[o1], (1 - w) * f2[o0]);
+    }
+  }
+  // Free per-image buffers of this string (keep nn and parameter planes).
+  std::vector<Alloc> keep;
+  for (auto& a : s.owned) {
+    const void* p = a.p;
+    bool shared = p == (const void*)s.dev.nn;
+    const DevParam* prm[] = {&s.dev.a[0], &s.dev.b[0], &s.dev.mxx[0], &s.dev.myy[0], &s.dev.a[1],
+                             &s.dev.b[1], &s.dev.mxx[1], &s.dev.myy[1], &s.dev.h};
+    for (auto* q : prm) shared = shared || p == (const void*)q->p;
+    if (shared) {
+      keep.push_back(a);
+    } else {
+      hipFree(a.p);
+      c->bytes -= (long long)a.bytes;
+    }
+  }
+  s.owned = keep;
+  if ((rc = alloc_images(c, s, F_new))) return rc;
+  s.desc.F = F_new;
+  destroy_graphs(c);
+  if ((rc = alloc_scalars(c)) || (rc = upload_descriptors(c)) || (rc = build_items(c))) return rc;
+  const bool saved_pending = c->pending;
+  c->pending = false;
+  if ((rc = mepgl_upload(c, si, na.data(), n1.data(), n2.data()))) return rc;
+  c->pending = saved_pending;
+  if ((rc = reset_solver(c))) return rc;
+  return arc_length_all(c);
+}
+
+int mepgl_loop(mepgl_ctx* c, int iterations, int relax, char mode, int nlcg) {
+  if (!c || iterations < 0 || relax < 0) return MEPGL_ERR_ARG;
+  if (mode != 'S' && mode != 'M' && mode != 'L' && mode != 'C')
+    return fail(c, MEPGL_ERR_ARG, std::string("unknown mode ") + mode);
+  hipSetDevice(c->device);
+  int rc;
+  if (mode == 'C' && (rc = ensure_cubic_rhs(c))) return rc;
+  const bool use_graph = !(c->flags & MEPGL_NO_GRAPH) && !c->timing.on;
+  for (int it = 0; it < iterations; ++it) {
+    if (c->stop.load()) break;
+    if (!use_graph) {
+      if ((rc = outer_iteration(c, relax, mode, nlcg ? 1 : 0))) return rc;
+      continue;
+    }
+    // Replay a captured outer iteration.  The capture depends on the parity
+    // state it starts from, which repeats after one or two iterations.
+    auto key = std::make_tuple(c->xp, c->dp, (int)c->pending, relax, nlcg ? 1 : 0, mode);
+    auto itg = c->graphs.find(key);
+    if (itg == c->graphs.end()) {
+      const int xp0 = c->xp, dp0 = c->dp;
+      const bool pend0 = c->pending;
+      HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      rc = outer_iteration(c, relax, mode, nlcg ? 1 : 0);
+      hipGraph_t g = nullptr;
+      hipError_t e = hipStreamEndCapture(c->stream, &g);
+      c->xp = xp0;
+      c->dp = dp0;
+      c->pending = pend0;
+      if (rc) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+      }
+      if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, "graph capture failed");
+      hipGraphExec_t ex;
+      e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, "graph instantiate failed");
+      itg = c->graphs.emplace(key, ex).first;
+    }
+    HIP_TRY(c, hipGraphLaunch(itg->second, c->stream));
+    // Advance the parity state exactly as the captured sequence does.
+    c->dry = true;
+    rc = outer_iteration(c, relax, mode, nlcg ? 1 : 0);
+    c->dry = false;
+    if (rc) return rc;
+  }
+  return check_finite(c);
+}
+
+int mepgl_get_scalars(mepgl_ctx* c, int si, int which, double* out) {
+  if (!valid_string(c, si) || !out) return fail(c, MEPGL_ERR_ARG, "get_scalars: bad arguments");
+  hipSetDevice(c->device);
+  const DevString& d = c->strings[si].dev;
+  std::vector<ImageScalars> h(d.F);
+  HIP_TRY(c, hipMemcpyAsync(h.data(), c->d_sc + d.img0, sizeof(ImageScalars) * d.F,
+                            hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> e(d.F);
+  if (which == MEPGL_SCALAR_ENERGY)
+    HIP_TRY(c, hipMemcpyAsync(e.data(), c->d_out + d.img0, sizeof(double) * d.F,
+                              hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (int n = 0; n < d.F; ++n) {
+    const ImageScalars& x = h[n];
+    switch (which) {
+      case MEPGL_SCALAR_FENERGY: out[n] = x.fenergy; break;
+      case MEPGL_SCALAR_G_NORM2: out[n] = x.g_norm2[0]; break;
+      case MEPGL_SCALAR_G_NORM2_OLD: out[n] = x.g_norm2[1]; break;
+      case MEPGL_SCALAR_ALPHA: out[n] = x.alpha; break;
+      case MEPGL_SCALAR_S: out[n] = x.s; break;
+      case MEPGL_SCALAR_S_JB: out[n] = x.s_jb; break;
+      case MEPGL_SCALAR_BETA: out[n] = x.beta; break;
+      case MEPGL_SCALAR_G_DELTA_G: out[n] = x.g_delta_g; break;
+      case MEPGL_SCALAR_LS_INDEX: out[n] = x.ls_index; break;
+      case MEPGL_SCALAR_ENERGY: out[n] = e[n]; break;
+      case MEPGL_SCALAR_POLY:
+        for (int q = 0; q < 4; ++q) out[4 * n + q] = x.poly[q];
+        break;
+      default:
+        return fail(c, MEPGL_ERR_ARG, "get_scalars: unknown scalar");
+    }
+  }
+  return MEPGL_OK;
+}
+
+int mepgl_download_state(mepgl_ctx* c, int si, int which, double* a, double* p1, double* p2) {
+  if (!valid_string(c, si) || !a || !p1 || which < 0 || which > 2)
+    return fail(c, MEPGL_ERR_ARG, "download_state: bad arguments");
+  hipSetDevice(c->device);
+  const DevString& d = c->strings[si].dev;
+  const void* src = which == 0 ? d.g[0] : which == 1 ? d.g[1] : d.d[c->dp];
+  return copy_out_fields(c, si, src, a, p1, p2);
+}
+
+int mepgl_set_control(mepgl_ctx* c, int key, int value) {
+  if (!c) return MEPGL_ERR_ARG;
+  switch (key) {
+    case MEPGL_CONTROL_STOP: c->stop.store(value); break;
+    case MEPGL_CONTROL_RELAXATION_STEPS: c->relax_steps.store(value); break;
+    case MEPGL_CONTROL_NLCG: c->nlcg.store(value); break;
+    default: return MEPGL_ERR_ARG;
+  }
+  return MEPGL_OK;
+}
+
+int mepgl_get_control(mepgl_ctx* c, int key, int* value) {
+  if (!c || !value) return MEPGL_ERR_ARG;
+  switch (key) {
+    case MEPGL_CONTROL_STOP: *value = c->stop.load(); break;
+    case MEPGL_CONTROL_RELAXATION_STEPS: *value = c->relax_steps.load(); break;
+    case MEPGL_CONTROL_NLCG: *value = c->nlcg.load(); break;
+    default: return MEPGL_ERR_ARG;
+  }
+  return MEPGL_OK;
+}
+
+int mepgl_profile(mepgl_ctx* c, int enable) {
+  if (!c) return MEPGL_ERR_ARG;
+  hipSetDevice(c->device);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  c->timing.on = enable != 0;
+  c->timing.used = 0;
+  c->timing.marks.clear();
+  for (int k = 0; k < kKernelCount; ++k) {
+    c->timing.total_ms[k] = 0;
+    c->timing.count[k] = 0;
+  }
+  return MEPGL_OK;
+}
+
+int mepgl_profile_read(mepgl_ctx* c, int kernel, double* avg_ms, int* count) {
+  if (!c || kernel < 0 || kernel >= kKernelCount || !avg_ms || !count) return MEPGL_ERR_ARG;
+  hipSetDevice(c->device);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  Timing& t = c->timing;
+  for (auto& m : t.marks) {
+    float ms = 0;
+    HIP_TRY(c, hipEventElapsedTime(&ms, t.pool[m.second.first], t.pool[m.second.second]));
+    t.total_ms[m.first] += ms;
+    t.count[m.first] += 1;
+  }
+  t.marks.clear();
+  t.used = 0;
+  *count = t.count[kernel];
+  *avg_ms = t.count[kernel] ? t.total_ms[kernel] / t.count[kernel] : 0.0;
+  return MEPGL_OK;
+}
+
+int mepgl_synchronize(mepgl_ctx* c) {
+  if (!c) return MEPGL_ERR_ARG;
+  hipSetDevice(c->device);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return MEPGL_OK;
+}
+
+int mepgl_info(mepgl_ctx* c, int si, int* tile_rows, int* work_items, long long* bytes) {
+  if (!valid_string(c, si)) return MEPGL_ERR_ARG;
+  if (tile_rows) *tile_rows = 4 * c->strings[si].dev.rows_per_thread;
+  if (work_items) *work_items = c->nitems;
+  if (bytes) *bytes = c->bytes;
+  return MEPGL_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,62 @@
+// Launch interface between the host context and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "common.hpp"
+
+namespace mepgl {
+
+enum KernelId {
+  kKernelGrad = 0,
+  kKernelPoly,
+  kKernelUpdate,
+  kKernelEnergy,
+  kKernelJB,
+  kKernelArclen,
+  kKernelArcFinalize,
+  kKernelLinear,
+  kKernelCubic,
+  kKernelCopyA,
+  kKernelResetNorms,
+  kKernelCount
+};
+
+// Arguments of one NLCG-step launch (G, P or the in-place update U).
+struct StepArgs {
+  const DevString* strings;
+  const WorkItem* items;
+  int nitems;
+  ImageScalars* sc;
+  const LineSearchTable* ls;
+  int xin, xout;    // field buffers read / written
+  int din, dout;    // direction buffers read / written
+  int gcur, gold;   // gradient buffers (current, previous)
+  int scur, sold;   // g_norm2 slots (current, previous)
+  int upd;          // G applies the pending update x += alpha d
+  int maxwell;      // Maxwell mode: psi gradient not recomputed
+  int nlcg;         // Polak-Ribiere beta (1) or steepest descent (0)
+  int scratch;      // G writes the diagnostic gradient buffer only
+};
+
+// Arguments of the auxiliary launches (energy, JB, arclength,
+// redistribution, scalar maintenance).
+struct AuxArgs {
+  const DevString* strings;
+  const WorkItem* items;
+  int nitems;
+  int nstrings;
+  ImageScalars* sc;
+  int xin, xout;
+  double* out;          // per global image (energy)
+  double* const* jb;    // per string JB planes (device array of pointers)
+};
+
+hipError_t launch_step(const StepArgs& a, int which, bool fp32, hipStream_t st);
+hipError_t launch_aux(const AuxArgs& a, int which, int arg, bool fp32, hipStream_t st);
+hipError_t launch_nn_map(const int* sc_dom, const int* comp_dom, int N, int pitch, uint16_t* nn,
+                         hipStream_t st);
+hipError_t launch_fill(void* p, long long n, bool fp32, double v, hipStream_t st);
+hipError_t launch_fill_u32(unsigned* p, long long n, hipStream_t st);
+
+}  // namespace mepgl

@@ -1,0 +1,884 @@
+/*
+ * CPU oracle for the gauged-string-method relaxation loop.
+ *
+ * TEST INFRASTRUCTURE ONLY: a plain-C FP64 restatement of the reference's
+ * arithmetic, used by tests/ as the checker and by bench.py as the timed CPU
+ * baseline.  The product path (mepgl_amd/) never links or calls it.
+ *
+ * Every function cites the reference lines it restates.  Expressions keep the
+ * reference's operand order and parenthesisation so that rounding follows the
+ * same sequence of operations; the file is compiled with -ffp-contract=off
+ * and uses fma() exactly where the reference's compiler fuses an obvious
+ * multiply-add that matters for bitwise agreement (field and direction
+ * updates).  Per-image sums are sequential in cell order (the reference's
+ * two-pass block sums in parallel_sum.cu:11-67 only reorder them).
+ *
+ * Pinning (tests/test_oracle_*.py):
+ *   or_build_nn_map        brute force over all 3x3 neighbourhoods
+ *   or_free_energy*        closed forms (normal state, Meissner state,
+ *                          uniform field, constant-phase kinetic term),
+ *                          per-term toggles, hand-computed 2x2 grids
+ *   or_gradient            central finite differences of or_free_energy at
+ *                          interior sites; boundary sites: parity unpinned
+ *                          beyond this restatement (reference weights, Q4)
+ *   or_compute_jb          closed forms (B of a linear gauge, j of a uniform
+ *                          real order parameter, phase invariance) and the
+ *                          identity j = -dF_kin/dA at interior sites
+ *   or_poly_expansion      exact quartic (Vandermonde against the energy) in
+ *                          corrected mode, and in quirk mode with d_A = 0;
+ *                          the Q1/Q2 terms with d_A != 0: parity unpinned
+ *   or_line_search         the literal loop against an independent scan,
+ *                          closed-form special cases, invariants
+ *   or_arc_length(_jb)     affine strings give s_n = n/(F-1)
+ *   or_redistribute_*      exactness on linear / cubic data, node
+ *                          interpolation, banded solve of the spline system
+ */
+#include "mepgl_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+static inline double sq(double x) { return x * x; }
+static inline double cub(double x) { return x * x * x; }
+static inline int bit(int m, int k) { return (m >> k) & 1; }
+
+#define AX(k) a[2 * (size_t)(k)]
+#define AY(k) a[2 * (size_t)(k) + 1]
+
+/* ------------------------------------------------------------------------ */
+/* Neighbour map: nlcg.cu:48-103.  Bit k is set when neighbour k lies in the
+ * domain:  3 2 1 / 4 X 0 / 5 6 7  (x to the right, y up).                  */
+void or_build_nn_map(int N, const int *domain, int *nn_map)
+{
+    for (int idx = 0; idx < N * N; idx++) {
+        int nn = 0;
+        if (domain[idx] != 0) {
+            int i = idx % N, j = idx / N;
+            if (i != N - 1 && domain[idx + 1] != 0) nn += 1 << 0;
+            if (i != N - 1 && j != N - 1 && domain[idx + 1 + N] != 0) nn += 1 << 1;
+            if (j != N - 1 && domain[idx + N] != 0) nn += 1 << 2;
+            if (j != N - 1 && i != 0 && domain[idx - 1 + N] != 0) nn += 1 << 3;
+            if (i != 0 && domain[idx - 1] != 0) nn += 1 << 4;
+            if (i != 0 && j != 0 && domain[idx - 1 - N] != 0) nn += 1 << 5;
+            if (j != 0 && domain[idx - N] != 0) nn += 1 << 6;
+            if (j != 0 && i != N - 1 && domain[idx - N + 1] != 0) nn += 1 << 7;
+        }
+        nn_map[idx] = nn;
+    }
+}
+
+static inline float area_weight(int sc)
+{
+    /* nlcg.cu:140-143 */
+    return (bit(sc, 0) * bit(sc, 1) * bit(sc, 2) + bit(sc, 2) * bit(sc, 3) * bit(sc, 4) +
+            bit(sc, 4) * bit(sc, 5) * bit(sc, 6) + bit(sc, 6) * bit(sc, 7) * bit(sc, 0)) / 4.0f;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Energy density of one site: nlcg.cu:105-214 (restated per component).   */
+static double kinetic_x(const or_system *s, const double *a, const double *p, double q,
+                        double m, size_t idx)
+{
+    const double dx = s->dx;
+#define PU(k) p[2 * (size_t)(k)]
+#define PV(k) p[2 * (size_t)(k) + 1]
+    return 0.25 / m * (sq(-(PU(idx + 1) - PU(idx)) / dx - q * AX(idx) * PV(idx)) +
+                       sq(-(PU(idx + 1) - PU(idx)) / dx - q * AX(idx + 1) * PV(idx + 1)) +
+                       sq(+(PV(idx + 1) - PV(idx)) / dx - q * AX(idx) * PU(idx)) +
+                       sq(+(PV(idx + 1) - PV(idx)) / dx - q * AX(idx + 1) * PU(idx + 1)));
+}
+
+static double kinetic_y(const or_system *s, const double *a, const double *p, double q,
+                        double m, size_t idx)
+{
+    const double dx = s->dx;
+    const size_t N = (size_t)s->N;
+    return 0.25 / m * (sq(-(PU(idx + N) - PU(idx)) / dx - q * AY(idx) * PV(idx)) +
+                       sq(-(PU(idx + N) - PU(idx)) / dx - q * AY(idx + N) * PV(idx + N)) +
+                       sq(+(PV(idx + N) - PV(idx)) / dx - q * AY(idx) * PU(idx)) +
+                       sq(+(PV(idx + N) - PV(idx)) / dx - q * AY(idx + N) * PU(idx + N)));
+}
+
+static double density_site(const or_system *s, const double *a, const double *p1,
+                           const double *p2, int idx0, size_t idx)
+{
+    const size_t N = (size_t)s->N;
+    const double dx = s->dx;
+    const int sc = s->sc_nn[idx0], cm = s->comp_nn[idx0];
+    const double u1 = p1[2 * idx], v1 = p1[2 * idx + 1];
+
+    double nl_term = s->b_1[idx0] / 2.0 * sq(s->a_1[idx0] / s->b_1[idx0] + sq(u1) + sq(v1));
+    if (s->multicomponent) {
+        const double u2 = p2[2 * idx], v2 = p2[2 * idx + 1];
+        nl_term += s->b_2[idx0] / 2.0 * sq(s->a_2[idx0] / s->b_2[idx0] + sq(u2) + sq(v2));
+        nl_term += s->eta * (u1 * u2 + v1 * v2);
+        nl_term += s->gamma / 2.0 * (sq(u1) + sq(v1)) * (sq(u2) + sq(v2));
+        nl_term += s->delta / 2.0 * (+sq(u1 * u2 + v1 * v2) - sq(u1 * v2 - v1 * u2));
+    }
+    nl_term *= area_weight(sc);
+
+    double kx = 0, ky = 0;
+    if (bit(sc, 0)) {
+        kx += kinetic_x(s, a, p1, s->q_1, s->m_xx_1[idx0], idx);
+        if (s->multicomponent) kx += kinetic_x(s, a, p2, s->q_2, s->m_xx_2[idx0], idx);
+    }
+    if (bit(sc, 2)) {
+        ky += kinetic_y(s, a, p1, s->q_1, s->m_yy_1[idx0], idx);
+        if (s->multicomponent) ky += kinetic_y(s, a, p2, s->q_2, s->m_yy_2[idx0], idx);
+    }
+    kx *= (bit(sc, 1) * bit(sc, 2) + bit(sc, 6) * bit(sc, 7)) / 2.0f;
+    ky *= (bit(sc, 0) * bit(sc, 1) + bit(sc, 3) * bit(sc, 4)) / 2.0f;
+
+    double em = 0;
+    const double *h = s->h;
+    if (bit(cm, 0) && bit(cm, 1) && bit(cm, 2))
+        em += 0.25 * 0.5 * sq((AY(idx + 1) - AY(idx)) / dx - (AX(idx + N) - AX(idx)) / dx - h[idx0]);
+    if (bit(cm, 2) && bit(cm, 3) && bit(cm, 4))
+        em += 0.25 * 0.5 * sq((AY(idx) - AY(idx - 1)) / dx - (AX(idx + N) - AX(idx)) / dx - h[idx0]);
+    if (bit(cm, 4) && bit(cm, 5) && bit(cm, 6))
+        em += 0.25 * 0.5 * sq((AY(idx) - AY(idx - 1)) / dx - (AX(idx) - AX(idx - N)) / dx - h[idx0]);
+    if (bit(cm, 6) && bit(cm, 7) && bit(cm, 0))
+        em += 0.25 * 0.5 * sq((AY(idx + 1) - AY(idx)) / dx - (AX(idx) - AX(idx - N)) / dx - h[idx0]);
+
+    /* Q3: the reference accumulates into a fresh allocation; zero here. */
+    double dens = 0;
+    if (s->terms & OR_TERM_KIN) dens += (kx + ky) * dx * dx;
+    if (s->terms & OR_TERM_NL) dens += nl_term * dx * dx;
+    if ((s->terms & OR_TERM_EM) && !s->no_self_field) dens += em * dx * dx;
+    return dens;
+}
+
+void or_free_energy_density(const or_system *s, int F, const double *a, const double *psi_1,
+                            const double *psi_2, double *density)
+{
+    const size_t NN = (size_t)s->N * s->N;
+#pragma omp parallel for schedule(static)
+    for (long t = 0; t < (long)(NN * F); t++) {
+        int idx0 = (int)(t % NN);
+        density[t] = density_site(s, a, psi_1, psi_2, idx0, (size_t)t);
+    }
+}
+
+/* computeFreeEnergy, nlcg.cu:217-240: density then per-image sum. */
+void or_free_energy(const or_system *s, int F, const double *a, const double *psi_1,
+                    const double *psi_2, double *fenergy)
+{
+    const size_t NN = (size_t)s->N * s->N;
+#pragma omp parallel for schedule(static)
+    for (int n = 0; n < F; n++) {
+        double sum = 0;
+        for (size_t idx0 = 0; idx0 < NN; idx0++)
+            sum += density_site(s, a, psi_1, psi_2, (int)idx0, idx0 + NN * n);
+        fenergy[n] = sum;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Gradient with respect to psi: kinetic bonds of one component,
+ * nlcg.cu:529-615 (component 1) and 670-736 (component 2).                 */
+static void kin_grad_psi(const or_system *s, const double *a, const double *p, double q,
+                         double m_xx, double m_yy, int sc, size_t idx, double *gx, double *gy)
+{
+    const size_t N = (size_t)s->N;
+    const double dx = s->dx;
+    if (bit(sc, 0)) {
+        *gx += 0.5 / m_xx * (+1.0 / dx * (-(PU(idx + 1) - PU(idx)) / dx - q * AX(idx) * PV(idx))
+                             + 1.0 / dx * (-(PU(idx + 1) - PU(idx)) / dx - q * AX(idx + 1) * PV(idx + 1))
+                             - q * AX(idx) * (+(PV(idx + 1) - PV(idx)) / dx - q * AX(idx) * PU(idx)))
+               * (bit(sc, 1) * bit(sc, 2) + bit(sc, 6) * bit(sc, 7)) / 2.0f;
+        *gy += 0.5 / m_xx * (-1.0 / dx * (+(PV(idx + 1) - PV(idx)) / dx - q * AX(idx) * PU(idx))
+                             - 1.0 / dx * (+(PV(idx + 1) - PV(idx)) / dx - q * AX(idx + 1) * PU(idx + 1))
+                             - q * AX(idx) * (-(PU(idx + 1) - PU(idx)) / dx - q * AX(idx) * PV(idx)))
+               * (bit(sc, 1) * bit(sc, 2) + bit(sc, 6) * bit(sc, 7)) / 2.0f;
+    }
+    if (bit(sc, 4)) {
+        *gx += 0.5 / m_xx * (-1.0 / dx * (-(PU(idx) - PU(idx - 1)) / dx - q * AX(idx) * PV(idx))
+                             - 1.0 / dx * (-(PU(idx) - PU(idx - 1)) / dx - q * AX(idx - 1) * PV(idx - 1))
+                             - q * AX(idx) * (+(PV(idx) - PV(idx - 1)) / dx - q * AX(idx) * PU(idx)))
+               * (bit(sc, 2) * bit(sc, 3) + bit(sc, 5) * bit(sc, 6)) / 2.0f;
+        *gy += 0.5 / m_xx * (+1.0 / dx * (+(PV(idx) - PV(idx - 1)) / dx - q * AX(idx) * PU(idx))
+                             + 1.0 / dx * (+(PV(idx) - PV(idx - 1)) / dx - q * AX(idx - 1) * PU(idx - 1))
+                             - q * AX(idx) * (-(PU(idx) - PU(idx - 1)) / dx - q * AX(idx) * PV(idx)))
+               * (bit(sc, 2) * bit(sc, 3) + bit(sc, 5) * bit(sc, 6)) / 2.0f;
+    }
+    if (bit(sc, 2)) {
+        *gx += 0.5 / m_yy * (+1.0 / dx * (-(PU(idx + N) - PU(idx)) / dx - q * AY(idx) * PV(idx))
+                             + 1.0 / dx * (-(PU(idx + N) - PU(idx)) / dx - q * AY(idx + N) * PV(idx + N))
+                             - q * AY(idx) * (+(PV(idx + N) - PV(idx)) / dx - q * AY(idx) * PU(idx)))
+               * (bit(sc, 0) * bit(sc, 1) + bit(sc, 3) * bit(sc, 4)) / 2.0f;
+        *gy += 0.5 / m_yy * (-1.0 / dx * (+(PV(idx + N) - PV(idx)) / dx - q * AY(idx) * PU(idx))
+                             - 1.0 / dx * (+(PV(idx + N) - PV(idx)) / dx - q * AY(idx + N) * PU(idx + N))
+                             - q * AY(idx) * (-(PU(idx + N) - PU(idx)) / dx - q * AY(idx) * PV(idx)))
+               * (bit(sc, 0) * bit(sc, 1) + bit(sc, 3) * bit(sc, 4)) / 2.0f;
+    }
+    if (bit(sc, 6)) {
+        *gx += 0.5 / m_yy * (-1.0 / dx * (-(PU(idx) - PU(idx - N)) / dx - q * AY(idx) * PV(idx))
+                             - 1.0 / dx * (-(PU(idx) - PU(idx - N)) / dx - q * AY(idx - N) * PV(idx - N))
+                             - q * AY(idx) * (+(PV(idx) - PV(idx - N)) / dx - q * AY(idx) * PU(idx)))
+               * (bit(sc, 4) * bit(sc, 5) + bit(sc, 7) * bit(sc, 0)) / 2.0f;
+        *gy += 0.5 / m_yy * (+1.0 / dx * (+(PV(idx) - PV(idx - N)) / dx - q * AY(idx) * PU(idx))
+                             + 1.0 / dx * (+(PV(idx) - PV(idx - N)) / dx - q * AY(idx - N) * PU(idx - N))
+                             - q * AY(idx) * (-(PU(idx) - PU(idx - N)) / dx - q * AY(idx) * PV(idx)))
+               * (bit(sc, 4) * bit(sc, 5) + bit(sc, 7) * bit(sc, 0)) / 2.0f;
+    }
+}
+
+/* gradientPsi, nlcg.cu:504-744. */
+static void grad_psi_site(const or_system *s, const double *a, const double *p1,
+                          const double *p2, double *g1, double *g2, int idx0, size_t idx)
+{
+    const double dx = s->dx;
+    const int sc = s->sc_nn[idx0];
+    const double af = area_weight(sc);
+    const double u1 = p1[2 * idx], v1 = p1[2 * idx + 1];
+    double g1x = (2 * s->a_1[idx0] * u1 + s->b_1[idx0] * 2 * u1 * (sq(u1) + sq(v1))) * af;
+    double g1y = (2 * s->a_1[idx0] * v1 + s->b_1[idx0] * 2 * v1 * (sq(u1) + sq(v1))) * af;
+    kin_grad_psi(s, a, p1, s->q_1, s->m_xx_1[idx0], s->m_yy_1[idx0], sc, idx, &g1x, &g1y);
+
+    if (s->multicomponent) {
+        const double u2 = p2[2 * idx], v2 = p2[2 * idx + 1];
+        double g2x = (2 * s->a_2[idx0] * u2 + s->b_2[idx0] * 2 * u2 * (sq(u2) + sq(v2))) * af;
+        double g2y = (2 * s->a_2[idx0] * v2 + s->b_2[idx0] * 2 * v2 * (sq(u2) + sq(v2))) * af;
+
+        g1x += s->eta * u2 * af;
+        g1y += s->eta * v2 * af;
+        g2x += s->eta * u1 * af;
+        g2y += s->eta * v1 * af;
+
+        g1x += s->gamma * u1 * (sq(u2) + sq(v2)) * af;
+        g1y += s->gamma * v1 * (sq(u2) + sq(v2)) * af;
+        g2x += s->gamma * u2 * (sq(u1) + sq(v1)) * af;
+        g2y += s->gamma * v2 * (sq(u1) + sq(v1)) * af;
+
+        g1x += s->delta * (+u2 * (u1 * u2 + v1 * v2) - v2 * (u1 * v2 - v1 * u2)) * af;
+        g1y += s->delta * (+v2 * (u1 * u2 + v1 * v2) + u2 * (u1 * v2 - v1 * u2)) * af;
+        g2x += s->delta * (+u1 * (u1 * u2 + v1 * v2) + v1 * (u1 * v2 - v1 * u2)) * af;
+        g2y += s->delta * (+v1 * (u1 * u2 + v1 * v2) - u1 * (u1 * v2 - v1 * u2)) * af;
+
+        kin_grad_psi(s, a, p2, s->q_2, s->m_xx_2[idx0], s->m_yy_2[idx0], sc, idx, &g2x, &g2y);
+        g2[2 * idx] = g2x;
+        g2[2 * idx + 1] = g2y;
+    }
+    g1[2 * idx] = g1x * (dx * dx);
+    g1[2 * idx + 1] = g1y * (dx * dx);
+    /* nlcg.cu:742-743 scale the second component's buffer in both builds. */
+    g2[2 * idx] *= dx * dx;
+    g2[2 * idx + 1] *= dx * dx;
+}
+
+/* Kinetic part of gradientA for one component, nlcg.cu:769-849. */
+static void kin_grad_a(const or_system *s, const double *a, const double *p, double q,
+                       double m_xx, double m_yy, int sc, size_t idx, double *gx, double *gy)
+{
+    const size_t N = (size_t)s->N;
+    const double dx = s->dx;
+    if (bit(sc, 0))
+        *gx += -0.5 * q / m_xx * (+PV(idx) * (-(PU(idx + 1) - PU(idx)) / dx - q * AX(idx) * PV(idx))
+                                 + PU(idx) * (+(PV(idx + 1) - PV(idx)) / dx - q * AX(idx) * PU(idx)))
+               * (bit(sc, 1) * bit(sc, 2) + bit(sc, 6) * bit(sc, 7)) / 2.0f;
+    if (bit(sc, 4))
+        *gx += -0.5 * q / m_xx * (+PV(idx) * (-(PU(idx) - PU(idx - 1)) / dx - q * AX(idx) * PV(idx))
+                                 + PU(idx) * (+(PV(idx) - PV(idx - 1)) / dx - q * AX(idx) * PU(idx)))
+               * (bit(sc, 2) * bit(sc, 3) + bit(sc, 5) * bit(sc, 6)) / 2.0f;
+    if (bit(sc, 2))
+        *gy += -0.5 * q / m_yy * (+PV(idx) * (-(PU(idx + N) - PU(idx)) / dx - q * AY(idx) * PV(idx))
+                                 + PU(idx) * (+(PV(idx + N) - PV(idx)) / dx - q * AY(idx) * PU(idx)))
+               * (bit(sc, 0) * bit(sc, 1) + bit(sc, 3) * bit(sc, 4)) / 2.0f;
+    if (bit(sc, 6))
+        *gy += -0.5 * q / m_yy * (+PV(idx) * (-(PU(idx) - PU(idx - N)) / dx - q * AY(idx) * PV(idx))
+                                 + PU(idx) * (+(PV(idx) - PV(idx - N)) / dx - q * AY(idx) * PU(idx)))
+               * (bit(sc, 4) * bit(sc, 5) + bit(sc, 7) * bit(sc, 0)) / 2.0f;
+}
+
+/* gradientA, nlcg.cu:746-888. */
+static void grad_a_site(const or_system *s, const double *a, const double *p1,
+                        const double *p2, double *ga, int idx0, size_t idx)
+{
+    const size_t N = (size_t)s->N;
+    const double dx = s->dx;
+    const int sc = s->sc_nn[idx0], cm = s->comp_nn[idx0];
+    const double *h = s->h;
+    double gx = 0, gy = 0;
+    kin_grad_a(s, a, p1, s->q_1, s->m_xx_1[idx0], s->m_yy_1[idx0], sc, idx, &gx, &gy);
+    if (s->multicomponent)
+        kin_grad_a(s, a, p2, s->q_2, s->m_xx_2[idx0], s->m_yy_2[idx0], sc, idx, &gx, &gy);
+
+    if (bit(cm, 0) && bit(cm, 1) && bit(cm, 2)) {
+        gx += +0.25 / dx * ((AY(idx + 1) - AY(idx)) / dx - (AX(idx + N) - AX(idx)) / dx - h[idx0]);
+        gy += -0.25 / dx * ((AY(idx + 1) - AY(idx)) / dx - (AX(idx + N) - AX(idx)) / dx - h[idx0]);
+        gy += -0.25 / dx * ((AY(idx + 1) - AY(idx)) / dx - (AX(idx + N + 1) - AX(idx + 1)) / dx - h[idx0 + 1]);
+        gx += +0.25 / dx * ((AY(idx + 1 + N) - AY(idx + N)) / dx - (AX(idx + N) - AX(idx)) / dx - h[idx0 + N]);
+    }
+    if (bit(cm, 2) && bit(cm, 3) && bit(cm, 4)) {
+        gx += +0.25 / dx * ((AY(idx) - AY(idx - 1)) / dx - (AX(idx + N) - AX(idx)) / dx - h[idx0]);
+        gy += +0.25 / dx * ((AY(idx) - AY(idx - 1)) / dx - (AX(idx + N) - AX(idx)) / dx - h[idx0]);
+        gx += +0.25 / dx * ((AY(idx + N) - AY(idx + N - 1)) / dx - (AX(idx + N) - AX(idx)) / dx - h[idx0 + N]);
+        gy += +0.25 / dx * ((AY(idx) - AY(idx - 1)) / dx - (AX(idx + N - 1) - AX(idx - 1)) / dx - h[idx0 - 1]);
+    }
+    if (bit(cm, 4) && bit(cm, 5) && bit(cm, 6)) {
+        gx += -0.25 / dx * ((AY(idx) - AY(idx - 1)) / dx - (AX(idx) - AX(idx - N)) / dx - h[idx0]);
+        gy += +0.25 / dx * ((AY(idx) - AY(idx - 1)) / dx - (AX(idx) - AX(idx - N)) / dx - h[idx0]);
+        gy += +0.25 / dx * ((AY(idx) - AY(idx - 1)) / dx - (AX(idx - 1) - AX(idx - 1 - N)) / dx - h[idx0 - 1]);
+        gx += -0.25 / dx * ((AY(idx - N) - AY(idx - N - 1)) / dx - (AX(idx) - AX(idx - N)) / dx - h[idx0 - N]);
+    }
+    if (bit(cm, 6) && bit(cm, 7) && bit(cm, 0)) {
+        gx += -0.25 / dx * ((AY(idx + 1) - AY(idx)) / dx - (AX(idx) - AX(idx - N)) / dx - h[idx0]);
+        gy += -0.25 / dx * ((AY(idx + 1) - AY(idx)) / dx - (AX(idx) - AX(idx - N)) / dx - h[idx0]);
+        gx += -0.25 / dx * ((AY(idx - N + 1) - AY(idx - N)) / dx - (AX(idx) - AX(idx - N)) / dx - h[idx0 - N]);
+        gy += -0.25 / dx * ((AY(idx + 1) - AY(idx)) / dx - (AX(idx + 1) - AX(idx + 1 - N)) / dx - h[idx0 + 1]);
+    }
+    ga[2 * idx] = gx * (dx * dx);
+    ga[2 * idx + 1] = gy * (dx * dx);
+}
+
+/* The two gradient launches of nlcgSteps, nlcg.cu:1521-1537.  In Maxwell
+ * mode gradientPsi is skipped and the psi buffers keep their contents; with
+ * NO_SELF_FIELD gradientA is skipped and the A buffer keeps its contents. */
+void or_gradient(const or_system *s, int F, const double *a, const double *psi_1,
+                 const double *psi_2, double *a_g, double *psi_1_g, double *psi_2_g,
+                 int maxwell_solver)
+{
+    const size_t NN = (size_t)s->N * s->N;
+#pragma omp parallel for schedule(static)
+    for (long t = 0; t < (long)(NN * F); t++) {
+        int idx0 = (int)(t % NN);
+        if (!maxwell_solver) grad_psi_site(s, a, psi_1, psi_2, psi_1_g, psi_2_g, idx0, (size_t)t);
+        if (!s->no_self_field) grad_a_site(s, a, psi_1, psi_2, a_g, idx0, (size_t)t);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* computeJBKernel, nlcg.cu:243-364. */
+static void jb_current(const or_system *s, const double *a, const double *p, double q,
+                       double m_xx, double m_yy, int sc, size_t idx, double *j)
+{
+    const size_t N = (size_t)s->N;
+    const double dx = s->dx;
+    double jx = 0, jy = 0;
+    if (bit(sc, 0))
+        jx += +0.5 * q / m_xx * (+PV(idx) * (-(PU(idx + 1) - PU(idx)) / dx - q * AX(idx) * PV(idx))
+                                + PU(idx) * (+(PV(idx + 1) - PV(idx)) / dx - q * AX(idx) * PU(idx)));
+    if (bit(sc, 4))
+        jx += +0.5 * q / m_xx * (+PV(idx) * (-(PU(idx) - PU(idx - 1)) / dx - q * AX(idx) * PV(idx))
+                                + PU(idx) * (+(PV(idx) - PV(idx - 1)) / dx - q * AX(idx) * PU(idx)));
+    if (bit(sc, 2))
+        jy += +0.5 * q / m_yy * (+PV(idx) * (-(PU(idx + N) - PU(idx)) / dx - q * AY(idx) * PV(idx))
+                                + PU(idx) * (+(PV(idx + N) - PV(idx)) / dx - q * AY(idx) * PU(idx)));
+    if (bit(sc, 6))
+        jy += +0.5 * q / m_yy * (+PV(idx) * (-(PU(idx) - PU(idx - N)) / dx - q * AY(idx) * PV(idx))
+                                + PU(idx) * (+(PV(idx) - PV(idx - N)) / dx - q * AY(idx) * PU(idx)));
+    j[2 * idx] = jx;
+    j[2 * idx + 1] = jy;
+}
+
+void or_compute_jb(const or_system *s, int F, const double *a, const double *psi_1,
+                   const double *psi_2, double *b, double *j_1, double *j_2)
+{
+    const size_t NN = (size_t)s->N * s->N;
+    const size_t N = (size_t)s->N;
+    const double dx = s->dx;
+#pragma omp parallel for schedule(static)
+    for (long t = 0; t < (long)(NN * F); t++) {
+        const int idx0 = (int)(t % NN);
+        const size_t idx = (size_t)t;
+        const int sc = s->sc_nn[idx0], cm = s->comp_nn[idx0];
+        double bb = 0;
+        if (bit(cm, 0) && bit(cm, 1) && bit(cm, 2))
+            bb += 0.25 * ((AY(idx + 1) - AY(idx)) / dx - (AX(idx + N) - AX(idx)) / dx);
+        if (bit(cm, 2) && bit(cm, 3) && bit(cm, 4))
+            bb += 0.25 * ((AY(idx) - AY(idx - 1)) / dx - (AX(idx + N) - AX(idx)) / dx);
+        if (bit(cm, 4) && bit(cm, 5) && bit(cm, 6))
+            bb += 0.25 * ((AY(idx) - AY(idx - 1)) / dx - (AX(idx) - AX(idx - N)) / dx);
+        if (bit(cm, 6) && bit(cm, 7) && bit(cm, 0))
+            bb += 0.25 * ((AY(idx + 1) - AY(idx)) / dx - (AX(idx) - AX(idx - N)) / dx);
+        if (bb != 0) bb /= area_weight(cm);
+        b[idx] = bb;
+        jb_current(s, a, psi_1, s->q_1, s->m_xx_1[idx0], s->m_yy_1[idx0], sc, idx, j_1);
+        if (s->multicomponent)
+            jb_current(s, a, psi_2, s->q_2, s->m_xx_2[idx0], s->m_yy_2[idx0], sc, idx, j_2);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* polynomialExpansion, nlcg.cu:997-1370: per-site coefficients of
+ * F(x + alpha d) = A + B alpha + C alpha^2 + D alpha^3 + E alpha^4.
+ * c1 multiplies d_A * psi in the first-order x-bond term (reference: +1,
+ * nlcg.cu:1162-1165, 1269-1272; exact expansion: -q).  c2 multiplies the
+ * y-bond cubic coefficient (reference: 0.5, nlcg.cu:1251, 1358; exact: 1). */
+typedef struct { double A, B, C, D, E; } quartic;
+
+static void poly_kin_bond(const or_system *s, const double *a, const double *da,
+                          const double *p, const double *pd, double q, double m, double coeff,
+                          size_t idx, size_t step, int comp_a, double c1, double c2, quartic *r)
+{
+    const double dx = s->dx;
+    const size_t i0 = idx, i1 = idx + step;
+#define A_(k) a[2 * (k) + comp_a]
+#define DA_(k) da[2 * (k) + comp_a]
+#define DU(k) pd[2 * (k)]
+#define DV(k) pd[2 * (k) + 1]
+    double k0[4], k1[4], k2[4];
+    k0[0] = (-(PU(i1) - PU(i0)) / dx - q * A_(i0) * PV(i0));
+    k0[1] = (-(PU(i1) - PU(i0)) / dx - q * A_(i1) * PV(i1));
+    k0[2] = (+(PV(i1) - PV(i0)) / dx - q * A_(i0) * PU(i0));
+    k0[3] = (+(PV(i1) - PV(i0)) / dx - q * A_(i1) * PU(i1));
+
+    k1[0] = (-(DU(i1) - DU(i0)) / dx - q * A_(i0) * DV(i0) + c1 * DA_(i0) * PV(i0));
+    k1[1] = (-(DU(i1) - DU(i0)) / dx - q * A_(i1) * DV(i1) + c1 * DA_(i1) * PV(i1));
+    k1[2] = (+(DV(i1) - DV(i0)) / dx - q * A_(i0) * DU(i0) + c1 * DA_(i0) * PU(i0));
+    k1[3] = (+(DV(i1) - DV(i0)) / dx - q * A_(i1) * DU(i1) + c1 * DA_(i1) * PU(i1));
+
+    k2[0] = -q * DA_(i0) * DV(i0);
+    k2[1] = -q * DA_(i1) * DV(i1);
+    k2[2] = -q * DA_(i0) * DU(i0);
+    k2[3] = -q * DA_(i1) * DU(i1);
+
+    r->A += (+0.5 * sq(k0[0]) + 0.5 * sq(k0[1]) + 0.5 * sq(k0[2]) + 0.5 * sq(k0[3]))
+            / (2 * m) * coeff * dx * dx;
+    r->B += (+(k0[0]) * (k1[0]) + (k0[1]) * (k1[1]) + (k0[2]) * (k1[2]) + (k0[3]) * (k1[3]))
+            / (2 * m) * coeff * dx * dx;
+    r->C += (+0.5 * sq(k1[0]) + (k0[0]) * (k2[0]) + 0.5 * sq(k1[1]) + (k0[1]) * (k2[1])
+             + 0.5 * sq(k1[2]) + (k0[2]) * (k2[2]) + 0.5 * sq(k1[3]) + (k0[3]) * (k2[3]))
+            / (2 * m) * coeff * dx * dx;
+    r->D += (+(k1[0]) * (k2[0]) + (k1[1]) * (k2[1]) + (k1[2]) * (k2[2]) + (k1[3]) * (k2[3]))
+            / (2 * m) * coeff * c2 * dx * dx;
+    r->E += (+0.5 * sq(k2[0]) + 0.5 * sq(k2[1]) + 0.5 * sq(k2[2]) + 0.5 * sq(k2[3]))
+            / (2 * m) * coeff * dx * dx;
+}
+
+static quartic poly_site(const or_system *s, const double *a, const double *p1,
+                         const double *p2, const double *da, const double *d1,
+                         const double *d2, int idx0, size_t idx)
+{
+    const size_t N = (size_t)s->N;
+    const double dx = s->dx;
+    const int sc = s->sc_nn[idx0], cm = s->comp_nn[idx0];
+    const double c1_1 = s->poly_quirks ? 1.0 : -s->q_1;
+    const double c1_2 = s->poly_quirks ? 1.0 : -s->q_2;
+    const double c2 = s->poly_quirks ? 0.5 : 1.0;
+    quartic r = {0, 0, 0, 0, 0};
+
+    if (s->terms & OR_TERM_NL) {
+        const double af = area_weight(sc);
+        double nl[3];
+        const double u1 = p1[2 * idx], v1 = p1[2 * idx + 1];
+        const double ud1 = d1[2 * idx], vd1 = d1[2 * idx + 1];
+        const double b1 = s->b_1[idx0];
+        nl[0] = (s->a_1[idx0] / b1 + sq(u1) + sq(v1));
+        nl[1] = +2 * (u1 * ud1 + v1 * vd1);
+        nl[2] = +(ud1 * ud1 + vd1 * vd1);
+        r.A += b1 * (0.5 * sq(nl[0])) * af * dx * dx;
+        r.B += b1 * (nl[0] * nl[1]) * af * dx * dx;
+        r.C += b1 * (0.5 * sq(nl[1]) + nl[0] * nl[2]) * af * dx * dx;
+        r.D += b1 * (nl[1] * nl[2]) * af * dx * dx;
+        r.E += b1 * (0.5 * sq(nl[2])) * af * dx * dx;
+        if (s->multicomponent) {
+            const double u2 = p2[2 * idx], v2 = p2[2 * idx + 1];
+            const double ud2 = d2[2 * idx], vd2 = d2[2 * idx + 1];
+            const double b2 = s->b_2[idx0];
+            nl[0] = (s->a_2[idx0] / b2 + sq(u2) + sq(v2));
+            nl[1] = +2 * (u2 * ud2 + v2 * vd2);
+            nl[2] = +(ud2 * ud2 + vd2 * vd2);
+            r.A += b2 * (0.5 * sq(nl[0])) * af * dx * dx;
+            r.B += b2 * (nl[0] * nl[1]) * af * dx * dx;
+            r.C += b2 * (0.5 * sq(nl[1]) + nl[0] * nl[2]) * af * dx * dx;
+            r.D += b2 * (nl[1] * nl[2]) * af * dx * dx;
+            r.E += b2 * (0.5 * sq(nl[2])) * af * dx * dx;
+
+            /* bilinear Josephson, nlcg.cu:1056-1062 */
+            r.A += s->eta * (u1 * u2 + v1 * v2) * af * dx * dx;
+            r.B += s->eta * (ud1 * u2 + vd1 * v2 + u1 * ud2 + v1 * vd2) * af * dx * dx;
+            r.C += s->eta * (ud1 * ud2 + vd1 * vd2) * af * dx * dx;
+
+            /* density coupling, nlcg.cu:1064-1082 */
+            const double n1 = (sq(u1) + sq(v1)), n2 = (sq(u2) + sq(v2));
+            const double pd1 = (u1 * ud1 + v1 * vd1), pd2 = (u2 * ud2 + v2 * vd2);
+            const double dd1 = (sq(ud1) + sq(vd1)), dd2 = (sq(ud2) + sq(vd2));
+            r.A += s->gamma / 2.0 * n1 * n2 * af * dx * dx;
+            r.B += s->gamma / 2.0 * (2.0 * pd1 * n2 + 2.0 * pd2 * n1) * af * dx * dx;
+            r.C += s->gamma / 2.0 * (4.0 * pd1 * pd2 + n1 * dd2 + n2 * dd1) * af * dx * dx;
+            r.D += s->gamma / 2.0 * (2.0 * pd1 * dd2 + 2.0 * pd2 * dd1) * af * dx * dx;
+            r.E += s->gamma / 2.0 * (dd1 * dd2) * af * dx * dx;
+
+            /* biquadratic Josephson, nlcg.cu:1084-1112 */
+            const double P = (u1 * u2 + v1 * v2), Q = (u1 * v2 - v1 * u2);
+            const double P1 = (+u1 * ud2 + ud1 * u2 + v1 * vd2 + vd1 * v2);
+            const double Q1 = (+u1 * vd2 + ud1 * v2 - v1 * ud2 - vd1 * u2);
+            const double P2 = (ud1 * ud2 + vd1 * vd2), Q2 = (ud1 * vd2 - vd1 * ud2);
+            r.A += s->delta / 2.0 * (+sq(P) - sq(Q)) * af * dx * dx;
+            r.B += s->delta / 2.0 * (+2.0 * P1 * P - 2.0 * Q1 * Q) * af * dx * dx;
+            r.C += s->delta / 2.0 * (+sq(P1) - sq(Q1) + 2.0 * P * P2 - 2.0 * Q * Q2) * af * dx * dx;
+            r.D += s->delta / 2.0 * (+2.0 * P1 * P2 - 2.0 * Q1 * Q2) * af * dx * dx;
+            r.E += s->delta / 2.0 * (+sq(P2) - sq(Q2)) * af * dx * dx;
+        }
+    }
+
+    if ((s->terms & OR_TERM_EM) && !s->no_self_field) {
+        /* nlcg.cu:1116-1157 */
+        const double *h = s->h;
+#define DAX(k) da[2 * (size_t)(k)]
+#define DAY(k) da[2 * (size_t)(k) + 1]
+        double e0, e1;
+        if (bit(cm, 0) && bit(cm, 1) && bit(cm, 2)) {
+            e0 = ((AY(idx + 1) - AY(idx)) / dx - (AX(idx + N) - AX(idx)) / dx - h[idx0]);
+            e1 = ((DAY(idx + 1) - DAY(idx)) / dx - (DAX(idx + N) - DAX(idx)) / dx);
+            r.A += (0.5 * sq(e0)) * 0.25 * dx * dx;
+            r.B += (e0 * e1) * 0.25 * dx * dx;
+            r.C += (0.5 * sq(e1)) * 0.25 * dx * dx;
+        }
+        if (bit(cm, 2) && bit(cm, 3) && bit(cm, 4)) {
+            e0 = ((AY(idx) - AY(idx - 1)) / dx - (AX(idx + N) - AX(idx)) / dx - h[idx0]);
+            e1 = ((DAY(idx) - DAY(idx - 1)) / dx - (DAX(idx + N) - DAX(idx)) / dx);
+            r.A += (0.5 * sq(e0)) * 0.25 * dx * dx;
+            r.B += (e0 * e1) * 0.25 * dx * dx;
+            r.C += (0.5 * sq(e1)) * 0.25 * dx * dx;
+        }
+        if (bit(cm, 4) && bit(cm, 5) && bit(cm, 6)) {
+            e0 = ((AY(idx) - AY(idx - 1)) / dx - (AX(idx) - AX(idx - N)) / dx - h[idx0]);
+            e1 = ((DAY(idx) - DAY(idx - 1)) / dx - (DAX(idx) - DAX(idx - N)) / dx);
+            r.A += (0.5 * sq(e0)) * 0.25 * dx * dx;
+            r.B += (e0 * e1) * 0.25 * dx * dx;
+            r.C += (0.5 * sq(e1)) * 0.25 * dx * dx;
+        }
+        if (bit(cm, 6) && bit(cm, 7) && bit(cm, 0)) {
+            e0 = ((AY(idx + 1) - AY(idx)) / dx - (AX(idx) - AX(idx - N)) / dx - h[idx0]);
+            e1 = ((DAY(idx + 1) - DAY(idx)) / dx - (DAX(idx) - DAX(idx - N)) / dx);
+            r.A += (0.5 * sq(e0)) * 0.25 * dx * dx;
+            r.B += (e0 * e1) * 0.25 * dx * dx;
+            r.C += (0.5 * sq(e1)) * 0.25 * dx * dx;
+        }
+    }
+
+    if (s->terms & OR_TERM_KIN) {
+        /* nlcg.cu:1160-1366 */
+        const double coeff_x = (bit(sc, 1) * bit(sc, 2) + bit(sc, 6) * bit(sc, 7)) / 2.0f;
+        const double coeff_y = (bit(sc, 0) * bit(sc, 1) + bit(sc, 3) * bit(sc, 4)) / 2.0f;
+        if (bit(sc, 0))
+            poly_kin_bond(s, a, da, p1, d1, s->q_1, s->m_xx_1[idx0], coeff_x, idx, 1, 0, c1_1, 1.0, &r);
+        if (bit(sc, 2))
+            poly_kin_bond(s, a, da, p1, d1, s->q_1, s->m_yy_1[idx0], coeff_y, idx, N, 1, -s->q_1, c2, &r);
+        if (s->multicomponent) {
+            if (bit(sc, 0))
+                poly_kin_bond(s, a, da, p2, d2, s->q_2, s->m_xx_2[idx0], coeff_x, idx, 1, 0, c1_2, 1.0, &r);
+            if (bit(sc, 2))
+                poly_kin_bond(s, a, da, p2, d2, s->q_2, s->m_yy_2[idx0], coeff_y, idx, N, 1, -s->q_2, c2, &r);
+        }
+    }
+    return r;
+}
+
+void or_poly_expansion(const or_system *s, int F, const double *a, const double *psi_1,
+                       const double *psi_2, const double *a_d, const double *psi_1_d,
+                       const double *psi_2_d, double *fenergy_density, double *poly_density)
+{
+    const size_t NN = (size_t)s->N * s->N;
+#pragma omp parallel for schedule(static)
+    for (long t = 0; t < (long)(NN * F); t++) {
+        quartic r = poly_site(s, a, psi_1, psi_2, a_d, psi_1_d, psi_2_d, (int)(t % NN), (size_t)t);
+        fenergy_density[t] = r.A;
+        poly_density[4 * t + 0] = r.B;
+        poly_density[4 * t + 1] = r.C;
+        poly_density[4 * t + 2] = r.D;
+        poly_density[4 * t + 3] = r.E;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* lineSearch, nlcg.cu:1398-1445 (FP64 build: the multiplier is the float
+ * literal 1.025f widened to double). */
+double or_line_search(double A, double B, double C, double D, double E)
+{
+    double x = 1e-10;
+    const double multiplier = 1.025f;
+    const int max_iter = 2000;
+    double f_old = E * pow(x, 4) + D * x * x * x + C * x * x + B * x + A;
+    double f_new = 0;
+    for (int p = 0; p < max_iter; p++) {
+        x *= multiplier;
+        f_new = E * pow(x, 4) + D * x * x * x + C * x * x + B * x + A;
+        if (f_new > f_old)
+            break;
+        else
+            f_old = f_new;
+    }
+    return x / multiplier;
+}
+
+/* ------------------------------------------------------------------------ */
+/* nlcgSteps, nlcg.cu:1462-1586, with the reference's pointer-swap pairing:
+ * the swap of step k acts on local copies (Q5), so at step k the current
+ * gradient is buffer 1 when k is even and buffer 0 when k is odd. */
+void or_nlcg_steps(const or_system *s, or_state *st, int iter_n, int nlcg, int maxwell_solver)
+{
+    const int F = st->F;
+    const size_t NN = (size_t)s->N * s->N;
+    const size_t tot = NN * F;
+    double *gn_cur = st->g_norm2[0], *gn_old = st->g_norm2[1];
+    double *ga = st->g_a[0], *gp1 = st->g_p1[0], *gp2 = st->g_p2[0];
+    double *ga_o = st->g_a[1], *gp1_o = st->g_p1[1], *gp2_o = st->g_p2[1];
+    double *dens = (double *)malloc(sizeof(double) * tot);
+    double *poly = (double *)malloc(sizeof(double) * 4 * tot);
+
+    for (int k = 0; k < iter_n; k++) {
+        double *t;
+        t = gn_old; gn_old = gn_cur; gn_cur = t;
+        t = ga_o; ga_o = ga; ga = t;
+        t = gp1_o; gp1_o = gp1; gp1 = t;
+        t = gp2_o; gp2_o = gp2; gp2 = t;
+
+        or_gradient(s, F, st->a, st->psi_1, st->psi_2, ga, gp1, gp2, maxwell_solver);
+
+        /* gradientDensity + sumArrayF x2, nlcg.cu:919-949, 1543-1548 */
+#pragma omp parallel for schedule(static)
+        for (int n = 0; n < F; n++) {
+            double s1 = 0, s2 = 0;
+            for (size_t c = 0; c < NN; c++) {
+                const size_t i = 2 * (c + NN * n);
+                s1 += sq(ga[i]) + sq(ga[i + 1]) + sq(gp1[i]) + sq(gp1[i + 1]) + sq(gp2[i]) +
+                      sq(gp2[i + 1]);
+                s2 += ga[i] * (ga[i] - ga_o[i]) + ga[i + 1] * (ga[i + 1] - ga_o[i + 1]) +
+                      gp1[i] * (gp1[i] - gp1_o[i]) + gp1[i + 1] * (gp1[i + 1] - gp1_o[i + 1]) +
+                      gp2[i] * (gp2[i] - gp2_o[i]) + gp2[i + 1] * (gp2[i + 1] - gp2_o[i + 1]);
+            }
+            gn_cur[n] = s1;
+            st->g_delta_g[n] = s2;
+        }
+
+        /* computeStepDirection, nlcg.cu:955-994 */
+#pragma omp parallel for schedule(static)
+        for (int n = 0; n < F; n++) {
+            double beta = 0;
+            if (nlcg) {
+                beta = st->g_delta_g[n] / gn_old[n];
+                beta = fmax(beta, 0.0f);
+            }
+            for (size_t c = 0; c < NN; c++) {
+                const size_t i = 2 * (c + NN * n);
+                for (int r = 0; r < 2; r++) {
+                    st->d_a[i + r] = fma(beta, st->d_a[i + r], -ga[i + r]);
+                    st->d_p1[i + r] = fma(beta, st->d_p1[i + r], -gp1[i + r]);
+                    st->d_p2[i + r] = fma(beta, st->d_p2[i + r], -gp2[i + r]);
+                }
+            }
+        }
+
+        /* polynomialExpansion + sums + lineSearch, nlcg.cu:1555-1577 */
+        or_poly_expansion(s, F, st->a, st->psi_1, st->psi_2, st->d_a, st->d_p1, st->d_p2, dens,
+                          poly);
+#pragma omp parallel for schedule(static)
+        for (int n = 0; n < F; n++) {
+            double A = 0, B = 0, C = 0, D = 0, E = 0;
+            for (size_t c = 0; c < NN; c++) {
+                const size_t i = c + NN * n;
+                A += dens[i];
+                B += poly[4 * i];
+                C += poly[4 * i + 1];
+                D += poly[4 * i + 2];
+                E += poly[4 * i + 3];
+            }
+            st->fenergy[n] = A;
+            st->poly[4 * n] = B;
+            st->poly[4 * n + 1] = C;
+            st->poly[4 * n + 2] = D;
+            st->poly[4 * n + 3] = E;
+            st->alpha[n] = or_line_search(A, B, C, D, E);
+        }
+
+        /* updateFields, nlcg.cu:1372-1396 */
+#pragma omp parallel for schedule(static)
+        for (long c = 0; c < (long)tot; c++) {
+            const double al = st->alpha[c / NN];
+            for (int r = 0; r < 2; r++) {
+                const size_t i = 2 * (size_t)c + r;
+                st->a[i] = fma(al, st->d_a[i], st->a[i]);
+                st->psi_1[i] = fma(al, st->d_p1[i], st->psi_1[i]);
+                if (s->multicomponent) st->psi_2[i] = fma(al, st->d_p2[i], st->psi_2[i]);
+            }
+        }
+    }
+    free(dens);
+    free(poly);
+}
+
+/* ------------------------------------------------------------------------ */
+/* computeArcLength, interpolation.cu:126-140: ds2 -> normalised arclength. */
+static void arc_from_ds2(int F, double *s)
+{
+    s[0] = 0;
+    for (int n = 1; n < F; n++) s[n] = s[n - 1] + sqrt(s[n]);
+    for (int n = 0; n < F; n++) s[n] /= s[F - 1];
+}
+
+/* arcLength, interpolation.cu:39-65, 143-155 (plain metric s). */
+void or_arc_length(int N, int F, int multicomponent, const double *a, const double *psi_1,
+                   const double *psi_2, double *s)
+{
+    const size_t NN = (size_t)N * N;
+    s[0] = 0;
+#pragma omp parallel for schedule(static)
+    for (int n = 1; n < F; n++) {
+        double acc = 0;
+        for (size_t c = 0; c < NN; c++) {
+            const size_t i = 2 * (c + NN * n), j = i - 2 * NN;
+            double v = sq((a[i] - a[j])) + sq((a[i + 1] - a[j + 1])) + sq((psi_1[i] - psi_1[j])) +
+                       sq((psi_1[i + 1] - psi_1[j + 1]));
+            if (multicomponent) v += sq((psi_2[i] - psi_2[j])) + sq((psi_2[i + 1] - psi_2[j + 1]));
+            acc += v;
+        }
+        s[n] = acc;
+    }
+    arc_from_ds2(F, s);
+}
+
+/* arcLengthJB, interpolation.cu:68-92, 157-169 (gauge-invariant s_jb). */
+void or_arc_length_jb(int N, int F, int multicomponent, const double *b, const double *j_1,
+                      const double *j_2, double *s_jb)
+{
+    const size_t NN = (size_t)N * N;
+    s_jb[0] = 0;
+#pragma omp parallel for schedule(static)
+    for (int n = 1; n < F; n++) {
+        double acc = 0;
+        for (size_t c = 0; c < NN; c++) {
+            const size_t k = c + NN * n, kp = k - NN;
+            double v = sq((b[k] - b[kp])) + sq((j_1[2 * k] - j_1[2 * kp])) +
+                       sq((j_1[2 * k + 1] - j_1[2 * kp + 1]));
+            if (multicomponent)
+                v += sq((j_2[2 * k] - j_2[2 * kp])) + sq((j_2[2 * k + 1] - j_2[2 * kp + 1]));
+            acc += v;
+        }
+        s_jb[n] = acc;
+    }
+    arc_from_ds2(F, s_jb);
+}
+
+/* generateNewFramesLinear, interpolation.cu:190-227, for one real2 field.
+ * The segment search never steps past the last node (the reference relies
+ * on s[F-1] == 1 for that). */
+void or_redistribute_linear(int N, int F, const double *s, const double *f, int F_new,
+                            double *f_new)
+{
+    const size_t NN = (size_t)N * N;
+#pragma omp parallel for schedule(static)
+    for (long idx = 0; idx < (long)NN; idx++) {
+        int k = 1;
+        double segment_start = 0;
+        double segment_end = s[1];
+        for (int n = 0; n < F_new; n++) {
+            const double s_new = (double)n / ((double)F_new - 1);
+            while (segment_end < s_new && k + 1 < F) {
+                k++;
+                segment_start = s[k - 1];
+                segment_end = s[k];
+            }
+            if (k >= F) k = F - 1;
+            const double w = (s_new - segment_start) / (s[k] - s[k - 1]);
+            for (int r = 0; r < 2; r++)
+                f_new[2 * (idx + n * NN) + r] =
+                    fma(w, f[2 * (idx + k * NN) + r], (1 - w) * f[2 * (idx + (k - 1) * NN) + r]);
+        }
+    }
+}
+
+/* Textbook Thomas algorithm (no pivoting) for one tridiagonal system,
+ * in place on rhs.  Stands in for the closed cusparseSgtsv2 call of
+ * interpolation.cu:443-454. */
+void or_thomas(int n, const double *dl, const double *d, const double *du, double *rhs)
+{
+    double *cp = (double *)malloc(sizeof(double) * n);
+    cp[0] = du[0] / d[0];
+    rhs[0] = rhs[0] / d[0];
+    for (int i = 1; i < n; i++) {
+        const double m = d[i] - dl[i] * cp[i - 1];
+        cp[i] = du[i] / m;
+        rhs[i] = (rhs[i] - dl[i] * rhs[i - 1]) / m;
+    }
+    for (int i = n - 2; i >= 0; i--) rhs[i] = rhs[i] - cp[i] * rhs[i + 1];
+    free(cp);
+}
+
+/* redistributeCubic, interpolation.cu:252-481, in FP64, for one real2
+ * field.  textbook = 0 keeps the reference's (w-1)^3-(w-1) weight on
+ * sigma_{k-1} (interpolation.cu:394-395); textbook = 1 uses
+ * (1-w)^3-(1-w).  Returns -1 when F < 4 (the end conditions need four
+ * nodes). */
+int or_redistribute_cubic(int N, int F, const double *s, const double *f, int F_new,
+                          double *f_new, int textbook)
+{
+    if (F < 4) return -1;
+    const size_t NN = (size_t)N * N;
+    double *h = (double *)malloc(sizeof(double) * F);
+    double *dl = (double *)malloc(sizeof(double) * F);
+    double *d = (double *)malloc(sizeof(double) * F);
+    double *du = (double *)malloc(sizeof(double) * F);
+    /* buildHMatrix, interpolation.cu:252-290 */
+    h[0] = 0;
+    for (int n = 1; n < F; n++) h[n] = s[n] - s[n - 1];
+    du[0] = +h[1];
+    d[0] = -h[1];
+    dl[0] = 0;
+    du[F - 1] = 0;
+    d[F - 1] = -h[F - 1];
+    dl[F - 1] = +h[F - 1];
+    for (int n = 1; n < F - 1; n++) {
+        dl[n] = h[n];
+        d[n] = 2 * (h[n] + h[n + 1]);
+        du[n] = h[n + 1];
+    }
+#pragma omp parallel
+    {
+        double *D = (double *)malloc(sizeof(double) * F);
+        double *sig[2];
+        sig[0] = (double *)malloc(sizeof(double) * F);
+        sig[1] = (double *)malloc(sizeof(double) * F);
+#pragma omp for schedule(static)
+        for (long idx = 0; idx < (long)NN; idx++) {
+            for (int r = 0; r < 2; r++) {
+                /* computeDeltaX/Y, interpolation.cu:292-357 */
+                for (int n = 1; n < F; n++)
+                    D[n] = (f[2 * (idx + n * NN) + r] - f[2 * (idx + (n - 1) * NN) + r]) / h[n];
+                const double d3s = ((D[3] - D[2]) / (h[3] + h[2]) - (D[2] - D[1]) / (h[2] + h[1])) /
+                                   (h[3] + h[2] + h[1]);
+                const double d3e = ((D[F - 1] - D[F - 2]) / (h[F - 1] + h[F - 2]) -
+                                    (D[F - 2] - D[F - 3]) / (h[F - 2] + h[F - 3])) /
+                                   (h[F - 1] + h[F - 2] + h[F - 3]);
+                for (int n = 1; n < F - 1; n++) D[n] = D[n + 1] - D[n];
+                D[0] = sq(h[1]) * d3s;
+                D[F - 1] = -sq(h[F - 1]) * d3e;
+                or_thomas(F, dl, d, du, D);
+                memcpy(sig[r], D, sizeof(double) * F);
+            }
+            /* generateNewFramesCubic, interpolation.cu:361-402; the segment
+             * search stops at the last node (Q11). */
+            int k = 1;
+            double segment_start = 0;
+            double segment_end = h[1];
+            for (int n = 0; n < F_new; n++) {
+                const double s_new = (double)n / ((double)F_new - 1);
+                while (segment_end < s_new && k + 1 < F) {
+                    k++;
+                    segment_start += h[k - 1];
+                    segment_end += h[k];
+                }
+                const double w = (s_new - segment_start) / h[k];
+                const double wm = textbook ? (cub(1 - w) - (1 - w)) : (cub(w - 1) - (w - 1));
+                for (int r = 0; r < 2; r++) {
+                    const double fk = f[2 * (idx + k * NN) + r];
+                    const double fk1 = f[2 * (idx + (k - 1) * NN) + r];
+                    f_new[2 * (idx + n * NN) + r] =
+                        w * fk + (1 - w) * fk1 +
+                        sq(h[k]) * ((cub(w) - w) * sig[r][k] + wm * sig[r][k - 1]);
+                }
+            }
+        }
+        free(D);
+        free(sig[0]);
+        free(sig[1]);
+    }
+    free(h);
+    free(dl);
+    free(d);
+    free(du);
+    return 0;
+}

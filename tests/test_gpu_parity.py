@@ -1,0 +1,218 @@
+"""Parity of the HIP path (through the C ABI) with the CPU oracle.
+
+Tolerances (FP64 path, identical inputs):
+  per-image energy, A..E, s, s_jb          relative 1e-12
+  gradient, JB, direction                  per-image norm-wise relative 1e-12
+  line-search step alpha                   bit-identical when the GPU's
+                                           coefficients are fed to the oracle's
+                                           scan, and equal to the oracle's
+  fields after steps / reparametrisation   per-image norm-wise relative 1e-11
+Mixed path: energies within 1e-5 relative of the FP64 oracle on the same
+inputs (float storage of the fields), see test_gpu_mixed.py.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import normwise, rel_err, small_string
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "one_comp": dict(multicomponent=False),
+    "one_comp_masked_planes": dict(multicomponent=False, masked=True, planes=True),
+    "two_comp": dict(multicomponent=True),
+    "two_comp_masked": dict(multicomponent=True, masked=True),
+    "two_comp_thin_film": dict(multicomponent=True, masked=True, no_self_field=True),
+}
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def solver_for(strings, **kw):
+    from mepgl_amd.solver import Solver
+
+    return Solver(strings, **kw)
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_energy_gradient_jb(case, oracle):
+    s = small_string(N=24, F=4, seed=3, **CASES[case])
+    say(f"[gpu] energy/gradient/jb case={case} N={s.N} F={s.F}")
+    with solver_for([s]) as sv:
+        e_gpu = sv.computeFreeEnergy()[0]
+        ga, g1, g2 = sv.gradient(0)
+        b, j1, j2 = sv.computeJB(0)
+    e_ref = oracle.free_energy(s)
+    assert rel_err(e_gpu, e_ref) < 1e-12, (e_gpu, e_ref)
+    ra, r1, r2 = oracle.gradient(s)
+    if not s.no_self_field:
+        assert normwise(ga, ra) < 1e-12
+    assert normwise(g1, r1) < 1e-12
+    if s.multicomponent:
+        assert normwise(g2, r2) < 1e-12
+    rb, rj1, rj2 = oracle.compute_jb(s)
+    assert normwise(b, rb) < 1e-12
+    assert normwise(j1, rj1) < 1e-12
+    if s.multicomponent:
+        assert normwise(j2, rj2) < 1e-12
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("exact", [False, True])
+def test_one_nlcg_step(case, exact, oracle):
+    s = small_string(N=20, F=4, seed=5, **CASES[case])
+    say(f"[gpu] one NLCG step case={case} exact={exact}")
+    ref = oracle.OracleString(s, poly_quirks=not exact)
+    ref.nlcg_steps(1)
+    with solver_for([s], poly_exact=exact) as sv:
+        sv.nlcgSteps(1)
+        fen = sv.scalars(0, "fenergy")
+        poly = sv.scalars(0, "poly")
+        alpha = sv.scalars(0, "alpha")
+        gn = sv.scalars(0, "g_norm2_old")  # step 0 writes the caller's g_norm2_old
+        a, p1, p2 = sv.download(0)
+        da, d1, d2 = sv.download_state(0, "d")
+    assert rel_err(fen, ref.fenergy) < 1e-12
+    for q in range(4):
+        assert rel_err(poly[:, q], ref.poly[:, q]) < 1e-11, q
+    assert rel_err(gn, ref.g_norm2[1]) < 1e-12
+    # The scan on the GPU's own coefficients is the oracle's scan bit for bit.
+    for n in range(s.F):
+        assert alpha[n] == oracle.line_search(fen[n], *poly[n]), n
+    np.testing.assert_array_equal(alpha, ref.alpha)
+    assert normwise(d1, ref.d_p1) < 1e-12
+    if not s.no_self_field:
+        assert normwise(da, ref.d_a) < 1e-12
+    assert normwise(p1, ref.psi_1) < 1e-12
+    assert normwise(a, ref.a) < 1e-12
+    if s.multicomponent:
+        assert normwise(p2, ref.psi_2) < 1e-12
+
+
+@pytest.mark.parametrize("case", ["one_comp", "two_comp_masked"])
+@pytest.mark.parametrize("steps", [6, 5])
+def test_nlcg_steps_call_boundaries(case, steps, oracle):
+    """Two calls in a row, with even and odd step counts (the reference's
+    pointer-swap pairing at call boundaries)."""
+    s = small_string(N=20, F=4, seed=7, **CASES[case])
+    say(f"[gpu] nlcg steps case={case} steps={steps}x2")
+    ref = oracle.OracleString(s)
+    ref.nlcg_steps(steps)
+    ref.nlcg_steps(steps)
+    with solver_for([s]) as sv:
+        sv.nlcgSteps(steps)
+        sv.nlcgSteps(steps)
+        a, p1, p2 = sv.download(0)
+        gn0 = sv.scalars(0, "g_norm2")
+        gn1 = sv.scalars(0, "g_norm2_old")
+        g = sv.download_state(0, "g")
+    assert normwise(p1, ref.psi_1) < 1e-11
+    assert normwise(a, ref.a) < 1e-11
+    assert rel_err(gn0, ref.g_norm2[0]) < 1e-10
+    assert rel_err(gn1, ref.g_norm2[1]) < 1e-10
+    assert normwise(g[1], ref.g_p1[0]) < 1e-10
+
+
+def test_maxwell_mode(oracle):
+    s = small_string(N=20, F=4, seed=8, multicomponent=False)
+    say("[gpu] maxwell mode")
+    ref = oracle.OracleString(s)
+    ref.nlcg_steps(4)
+    ref.nlcg_steps(4, maxwell_solver=True)
+    with solver_for([s]) as sv:
+        sv.nlcgSteps(4)
+        sv.nlcgSteps(4, maxwell_solver=True)
+        a, p1, _ = sv.download(0)
+    assert normwise(a, ref.a) < 1e-11
+    assert normwise(p1, ref.psi_1) < 1e-11
+
+
+@pytest.mark.parametrize("case", ["one_comp", "two_comp", "two_comp_thin_film"])
+def test_arclength_and_linear_redistribution(case, oracle):
+    s = small_string(N=20, F=6, seed=9, **CASES[case])
+    say(f"[gpu] arclength + linear case={case}")
+    ref = oracle.OracleString(s)
+    s_ref, sjb_ref = ref.arc_lengths()
+    ref.redistribute(cubic=False)
+    with solver_for([s]) as sv:
+        (s_gpu, sjb_gpu), = sv.arcLength()
+        sv.redistributeLinear()
+        a, p1, p2 = sv.download(0)
+    assert rel_err(s_gpu, s_ref) < 1e-12
+    assert rel_err(sjb_gpu, sjb_ref) < 1e-12
+    assert normwise(a, ref.a) < 1e-12
+    assert normwise(p1, ref.psi_1) < 1e-12
+    if s.multicomponent:
+        assert normwise(p2, ref.psi_2) < 1e-12
+
+
+@pytest.mark.parametrize("textbook", [False, True])
+def test_cubic_redistribution(textbook, oracle):
+    s = small_string(N=20, F=7, seed=10, multicomponent=True)
+    say(f"[gpu] cubic textbook={textbook}")
+    ref = oracle.OracleString(s, cubic_textbook=textbook)
+    ref.arc_lengths()
+    ref.redistribute(cubic=True)
+    with solver_for([s], cubic_textbook=textbook) as sv:
+        sv.arcLength()
+        sv.redistributeCubic()
+        a, p1, p2 = sv.download(0)
+    assert normwise(a, ref.a) < 1e-11
+    assert normwise(p1, ref.psi_1) < 1e-11
+    assert normwise(p2, ref.psi_2) < 1e-11
+
+
+@pytest.mark.parametrize("mode", ["L", "C", "S"])
+def test_loop_graph_and_eager(mode, oracle):
+    s = small_string(N=20, F=6, seed=11, multicomponent=False)
+    say(f"[gpu] loop mode={mode} graph and eager")
+    ref = oracle.OracleString(s)
+    ref.loop(3, 4, mode)
+    out = []
+    for use_graph in (True, False):
+        with solver_for([s], use_graph=use_graph) as sv:
+            sv.loop(3, 4, mode)
+            out.append(sv.download(0))
+            fen = sv.scalars(0, "fenergy")
+    for k in range(2):
+        np.testing.assert_array_equal(out[0][k], out[1][k])
+    assert normwise(out[0][1], ref.psi_1) < 1e-10
+    assert normwise(out[0][0], ref.a) < 1e-10
+    assert rel_err(fen, ref.fenergy) < 1e-12
+
+
+def test_batched_strings_match_single(oracle):
+    strings = [small_string(N=20, F=4, seed=12, multicomponent=False),
+               small_string(N=28, F=6, seed=13, multicomponent=True, masked=True),
+               small_string(N=70, F=3, seed=14, multicomponent=False)]
+    say("[gpu] batched 3 strings")
+    with solver_for(strings) as sv:
+        sv.loop(2, 4, "L")
+        got = [sv.download(i) for i in range(3)]
+        en = sv.computeFreeEnergy()
+    for i, s in enumerate(strings):
+        ref = oracle.OracleString(s)
+        ref.loop(2, 4, "L")
+        assert normwise(got[i][1], ref.psi_1) < 1e-10, i
+        e_ref = ref.free_energy()
+        assert rel_err(en[i], e_ref) < 1e-11, i
+
+
+def test_resample(oracle):
+    s = small_string(N=20, F=5, seed=15, multicomponent=True)
+    say("[gpu] resample to 8 images")
+    ref = oracle.OracleString(s)
+    ref.arc_lengths()
+    ref.redistribute(F_new=8)
+    with solver_for([s]) as sv:
+        sv.resample(0, 8)
+        a, p1, p2 = sv.download(0)
+    assert a.shape[0] == 8
+    assert normwise(p1, ref.psi_1) < 1e-12
+    assert normwise(p2, ref.psi_2) < 1e-12
+    assert normwise(a, ref.a) < 1e-12
