@@ -72,7 +72,7 @@ typedef struct {
   double q_1, q_2, eta, gamma, delta; /* q.npy (io.cu:60-67)              */
   const int32_t *comp_domain; /* N*N, computational domain (nonzero = in) */
   const int32_t *sc_domain;   /* N*N, superconducting domain              */
-  int rows_per_thread;   /* tile height / 4; 0 lets the library choose    */
+  int segment_rows;      /* rows one wavefront marches; 0 = library picks */
 } mepgl_string_desc;
 
 typedef struct mepgl_ctx mepgl_ctx;
@@ -175,8 +175,8 @@ int mepgl_profile_read(mepgl_ctx *ctx, int kernel, double *avg_ms, int *count);
 /* Waits for all work of the context. */
 int mepgl_synchronize(mepgl_ctx *ctx);
 
-/* Geometry and memory facts: work items per NLCG launch, bytes of device
- * memory held, and the tile height chosen for a string. */
+/* Geometry and memory facts: work items (wavefronts) per NLCG launch,
+ * bytes of device memory held, and the segment height of a string. */
 int mepgl_info(mepgl_ctx *ctx, int string, int *tile_rows, int *work_items, long long *bytes);
 
 #ifdef __cplusplus

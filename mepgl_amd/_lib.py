@@ -49,7 +49,7 @@ class StringDesc(C.Structure):
         ("q_1", C.c_double), ("q_2", C.c_double), ("eta", C.c_double), ("gamma", C.c_double),
         ("delta", C.c_double),
         ("comp_domain", _i32p), ("sc_domain", _i32p),
-        ("rows_per_thread", C.c_int),
+        ("segment_rows", C.c_int),
     ]
 
 

@@ -17,9 +17,8 @@
 namespace mepgl {
 
 constexpr int kWave = 64;          // CDNA wavefront width
-constexpr int kTileX = 64;         // tile width: one wavefront row
-constexpr int kWavesPerBlock = 4;  // 256-thread workgroups
-constexpr int kBlock = kWave * kWavesPerBlock;
+constexpr int kTileX = 64;         // strip width: one wavefront row segment
+constexpr int kBlock = kWave;      // one wavefront per workgroup / work item
 constexpr int kLsSteps = 2000;     // line-search scan length (nlcg.cu:1421)
 constexpr int kNumXcd = 8;         // MI355X accelerator complex dies
 
@@ -50,13 +49,17 @@ struct DevString {
   int nplanes;              // 2 + 2*nc
   int nsf;                  // NO_SELF_FIELD: A is fixed, no A gradient
   int img0;                 // global image id of image 0
-  int tiles_x, tiles_y;     // tiles per image
-  int rows_per_thread;      // tile height = rows_per_thread * kWavesPerBlock
+  int tiles_x, tiles_y;     // 64-column strips x row segments per image
+  int seg_rows;             // rows per segment (one wavefront marches them)
   int poly_exact;           // 0: reference line-search quartic (Q1, Q2)
+  int planes;               // some of a, b, m_xx, m_yy is a per-site plane
   long long plane_stride;   // elements per plane (rows * pitch)
   long long image_stride;   // elements per image (nplanes * plane_stride)
   double dx, inv_dx, dx2;
   double q[2], eta, gamma, delta;
+  // Per-component constants of uniform parameters: q, 2a, 2b, b, a/b,
+  // 0.5/m_xx, 0.5/m_yy (used when planes == 0).
+  double kc[2][7];
   DevParam a[2], b[2], mxx[2], myy[2], h;
   const uint16_t* nn;
   void* x[2];
@@ -66,8 +69,6 @@ struct DevString {
   double* part_g;           // [F][ntiles][kPartG] workgroup partials
   double* part_p;           // [F][ntiles][kPartP]
   double* part_a;           // [F][ntiles][kPartA] arclength partials
-  unsigned* count_g;        // [F] arrival counters (self-resetting)
-  unsigned* count_p;        // [F]
   double* cubic;            // [F][4] Thomas factors c', m, sub-diagonal, h
   void* cubic_rhs;          // cubic right-hand sides [F][plane][rows][pitch]
   int* plan_k;              // [F] linear redistribution: segment end node
@@ -76,11 +77,11 @@ struct DevString {
   double* cplan_w;          // [F] cubic redistribution: weight of node k
 };
 
-// One work item = one tile of one image of one string.
+// One work item = one wavefront's strip segment of one image of one string.
 struct WorkItem {
   int string;
   int image;
-  int tile;   // tx + tiles_x * ty
+  int tile;   // strip + tiles_x * segment
   int pad;
 };
 

@@ -56,9 +56,16 @@ struct mepgl_ctx {
   int total_images = 0;
   DevString* d_strings = nullptr;
   ImageScalars* d_sc = nullptr;
+  int2* d_img = nullptr;     // global image id -> (string, image)
   LineSearchTable* d_ls = nullptr;
   WorkItem* d_items = nullptr;
   int nitems = 0;
+  // Items sorted by class (components, fixed A); G and P launch once per
+  // non-empty class so each launch runs one specialised kernel.
+  struct Group {
+    int begin, count, nc, nsf;
+  };
+  std::vector<Group> groups;
   double** d_jb = nullptr;
   double* d_out = nullptr;
   std::string err;
@@ -109,18 +116,17 @@ void free_string(mepgl_ctx* c, StringHost& s) {
   s.owned.clear();
 }
 
-// Tile height: 4 * rows_per_thread rows.  Aim for enough workgroups per
-// launch to cover the 256 CUs several times over while keeping tiles tall
-// enough that halo rows stay a small fraction of the rows read.
-int choose_rows(const mepgl_string_desc& d, int total_images_hint) {
-  if (d.rows_per_thread > 0) return d.rows_per_thread;
+// Rows marched by one wavefront.  Taller segments re-read fewer halo rows;
+// shorter ones give more wavefronts per launch.  Pick the tallest segment
+// that still yields about eight wavefronts per SIMD (1024 SIMDs).
+int choose_rows(const mepgl_string_desc& d, long long total_images) {
+  if (d.segment_rows > 0) return std::min(d.segment_rows, d.N);
   const long long cols = (d.N + kTileX - 1) / kTileX;
-  for (int r : {8, 4, 2}) {
-    const long long th = 4LL * r;
-    const long long tiles = cols * ((d.N + th - 1) / th) * (long long)total_images_hint;
-    if (th <= d.N && tiles >= 4096) return r;
+  for (int h : {64, 32, 16, 8}) {
+    const long long items = cols * ((d.N + h - 1) / h) * total_images;
+    if (h <= d.N && items >= 8192) return h;
   }
-  return 1;
+  return std::min(d.N, 4);
 }
 
 bool param_ok(const mepgl_param& p) { return p.plane != nullptr || std::isfinite(p.value); }
@@ -175,8 +181,24 @@ hipEvent_t take_event(mepgl_ctx* c) {
   return t.pool[t.used++];
 }
 
-int run_step(mepgl_ctx* c, const StepArgs& a, int which) {
+int run_step_group(mepgl_ctx* c, const StepArgs& a, int which);
+
+// Launches `which` once per item class.
+int run_step(mepgl_ctx* c, const StepArgs& a0, int which) {
   if (c->dry) return MEPGL_OK;
+  for (const auto& g : c->groups) {
+    StepArgs a = a0;
+    a.items = c->d_items + g.begin;
+    a.nitems = g.count;
+    a.nc = g.nc;
+    a.nsf = g.nsf;
+    int rc = run_step_group(c, a, which);
+    if (rc) return rc;
+  }
+  return MEPGL_OK;
+}
+
+int run_step_group(mepgl_ctx* c, const StepArgs& a, int which) {
   if (c->timing.on && which != kKernelUpdate) {
     const size_t s0 = c->timing.used;
     hipEventRecord(take_event(c), c->stream);
@@ -196,6 +218,35 @@ int run_step(mepgl_ctx* c, const StepArgs& a, int which) {
 int run_aux(mepgl_ctx* c, const AuxArgs& a, int which, int arg) {
   if (c->dry) return MEPGL_OK;
   hipError_t e = launch_aux(a, which, arg, c->fp32(), c->stream);
+  if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, std::string("launch: ") + hipGetErrorString(e));
+  return MEPGL_OK;
+}
+
+FinishArgs finish_args(mepgl_ctx* c) {
+  FinishArgs f{};
+  f.strings = c->d_strings;
+  f.img = c->d_img;
+  f.nimages = c->total_images;
+  f.sc = c->d_sc;
+  f.ls = c->d_ls;
+  f.out = c->d_out;
+  f.nlcg = 1;
+  return f;
+}
+
+// Timing ids of the finishing launches.
+constexpr int kTimeFinishGrad = 3, kTimeFinishPoly = 4;
+
+int run_finish(mepgl_ctx* c, const FinishArgs& f, int which) {
+  if (c->dry) return MEPGL_OK;
+  const bool timed = c->timing.on && (which == kKernelGrad || which == kKernelPoly);
+  const size_t s0 = c->timing.used;
+  if (timed) hipEventRecord(take_event(c), c->stream);
+  hipError_t e = launch_finish(f, which, c->fp32(), c->stream);
+  if (timed) {
+    hipEventRecord(take_event(c), c->stream);
+    c->timing.marks.push_back({which == kKernelGrad ? kTimeFinishGrad : kTimeFinishPoly, {s0, s0 + 1}});
+  }
   if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, std::string("launch: ") + hipGetErrorString(e));
   return MEPGL_OK;
 }
@@ -230,6 +281,11 @@ int nlcg_step(mepgl_ctx* c, int k, int nlcg, int maxwell) {
   a.xout = 1 - c->xp;
   int rc = run_step(c, a, kKernelGrad);
   if (rc) return rc;
+  FinishArgs f = finish_args(c);
+  f.scur = a.scur;
+  f.sold = a.sold;
+  f.nlcg = nlcg;
+  if ((rc = run_finish(c, f, kKernelGrad))) return rc;
   if (c->pending) {
     c->xp = 1 - c->xp;
     c->pending = false;
@@ -240,6 +296,7 @@ int nlcg_step(mepgl_ctx* c, int k, int nlcg, int maxwell) {
   a.dout = 1 - c->dp;
   rc = run_step(c, a, kKernelPoly);
   if (rc) return rc;
+  if ((rc = run_finish(c, f, kKernelPoly))) return rc;
   c->dp = 1 - c->dp;
   c->pending = true;
   return MEPGL_OK;
@@ -264,11 +321,19 @@ int check_finite(mepgl_ctx* c) {
 
 int build_items(mepgl_ctx* c) {
   std::vector<WorkItem> items;
-  for (size_t si = 0; si < c->strings.size(); ++si) {
-    const DevString& s = c->strings[si].dev;
-    for (int n = 0; n < s.F; ++n)
-      for (int t = 0; t < s.tiles_x * s.tiles_y; ++t) items.push_back({(int)si, n, t, 0});
-  }
+  c->groups.clear();
+  for (int nc = 1; nc <= 2; ++nc)
+    for (int nsf = 0; nsf <= 1; ++nsf) {
+      const int begin = (int)items.size();
+      for (size_t si = 0; si < c->strings.size(); ++si) {
+        const DevString& s = c->strings[si].dev;
+        if (s.nc != nc || s.nsf != nsf) continue;
+        for (int n = 0; n < s.F; ++n)
+          for (int t = 0; t < s.tiles_x * s.tiles_y; ++t) items.push_back({(int)si, n, t, 0});
+      }
+      const int count = (int)items.size() - begin;
+      if (count > 0) c->groups.push_back({begin, count, nc, nsf});
+    }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   if (c->d_items) hipFree(c->d_items);
   c->d_items = nullptr;
@@ -341,10 +406,6 @@ int alloc_images(mepgl_ctx* c, StringHost& s, int F) {
   d.part_p = (double*)p;
   if ((rc = dev_alloc(c, &s, &p, nt * kPartA * sizeof(double)))) return rc;
   d.part_a = (double*)p;
-  if ((rc = dev_alloc(c, &s, &p, (size_t)F * sizeof(unsigned) * 2))) return rc;
-  HIP_TRY(c, hipMemsetAsync(p, 0, (size_t)F * sizeof(unsigned) * 2, c->stream));
-  d.count_g = (unsigned*)p;
-  d.count_p = d.count_g + F;
   if ((rc = dev_alloc(c, &s, &p, (size_t)F * 4 * sizeof(double)))) return rc;
   HIP_TRY(c, hipMemsetAsync(p, 0, (size_t)F * 4 * sizeof(double), c->stream));
   d.cubic = (double*)p;
@@ -374,6 +435,14 @@ int alloc_scalars(mepgl_ctx* c) {
   HIP_TRY(c, hipMemsetAsync(c->d_sc, 0, sizeof(ImageScalars) * img, c->stream));
   HIP_TRY(c, hipMalloc(&c->d_out, sizeof(double) * img));
   HIP_TRY(c, hipMemsetAsync(c->d_out, 0, sizeof(double) * img, c->stream));
+  std::vector<int2> map;
+  for (size_t si = 0; si < c->strings.size(); ++si)
+    for (int n = 0; n < c->strings[si].dev.F; ++n) map.push_back(make_int2((int)si, n));
+  if (c->d_img) hipFree(c->d_img);
+  c->d_img = nullptr;
+  HIP_TRY(c, hipMalloc(&c->d_img, sizeof(int2) * map.size()));
+  HIP_TRY(c, hipMemcpyAsync(c->d_img, map.data(), sizeof(int2) * map.size(),
+                            hipMemcpyHostToDevice, c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return MEPGL_OK;
 }
@@ -461,10 +530,14 @@ int copy_out_fields(mepgl_ctx* c, int si, const void* dev, double* a, double* p1
 }
 
 int arc_length_all(mepgl_ctx* c) {
+  for (auto& s : c->strings)
+    if (s.dev.F < 2)
+      return fail(c, MEPGL_ERR_ARG, "arclength and reparametrisation need at least 2 images");
   int rc = flush(c);
   if (rc) return rc;
   AuxArgs a = aux_args(c);
   if ((rc = run_aux(c, a, kKernelArclen, 0))) return rc;
+  if ((rc = run_finish(c, finish_args(c), kKernelArclen))) return rc;
   return run_aux(c, a, kKernelArcFinalize, (int)c->strings.size());
 }
 
@@ -545,7 +618,7 @@ int mepgl_create(const mepgl_string_desc* strings, int n_strings, int precision,
     const mepgl_string_desc& in = strings[si];
     const mepgl_param* ps[] = {&in.a_1, &in.b_1, &in.m_xx_1, &in.m_yy_1, &in.a_2,
                                &in.b_2, &in.m_xx_2, &in.m_yy_2, &in.h};
-    bool ok = in.N >= 2 && in.F >= 2 && (in.ncomp == 1 || in.ncomp == 2) && in.dx > 0 &&
+    bool ok = in.N >= 2 && in.F >= 1 && (in.ncomp == 1 || in.ncomp == 2) && in.dx > 0 &&
               std::isfinite(in.dx) && in.comp_domain && in.sc_domain;
     for (auto* p : ps) ok = ok && param_ok(*p);
     if (!ok) {
@@ -564,9 +637,9 @@ int mepgl_create(const mepgl_string_desc* strings, int n_strings, int precision,
     d.nplanes = 2 + 2 * in.ncomp;
     d.nsf = in.no_self_field ? 1 : 0;
     d.poly_exact = (flags & MEPGL_POLY_EXACT) ? 1 : 0;
-    d.rows_per_thread = choose_rows(in, (int)total_img);
+    d.seg_rows = choose_rows(in, total_img);
     d.tiles_x = (in.N + kTileX - 1) / kTileX;
-    d.tiles_y = (in.N + 4 * d.rows_per_thread - 1) / (4 * d.rows_per_thread);
+    d.tiles_y = (in.N + d.seg_rows - 1) / d.seg_rows;
     s.ntiles = d.tiles_x * d.tiles_y;
     d.plane_stride = (long long)in.N * d.pitch;
     d.image_stride = (long long)d.nplanes * d.plane_stride;
@@ -578,6 +651,21 @@ int mepgl_create(const mepgl_string_desc* strings, int n_strings, int precision,
     d.eta = in.eta;
     d.gamma = in.gamma;
     d.delta = in.delta;
+    const mepgl_param* comp_p[2][4] = {{&in.a_1, &in.b_1, &in.m_xx_1, &in.m_yy_1},
+                                       {&in.a_2, &in.b_2, &in.m_xx_2, &in.m_yy_2}};
+    d.planes = 0;
+    for (int q = 0; q < 2; ++q) {
+      for (int k = 0; k < 4; ++k) d.planes |= comp_p[q][k]->plane != nullptr;
+      const double a = comp_p[q][0]->value, b = comp_p[q][1]->value;
+      const double mx = comp_p[q][2]->value, my = comp_p[q][3]->value;
+      d.kc[q][0] = q == 0 ? in.q_1 : in.q_2;
+      d.kc[q][1] = 2.0 * a;
+      d.kc[q][2] = 2.0 * b;
+      d.kc[q][3] = b;
+      d.kc[q][4] = a / b;
+      d.kc[q][5] = 0.5 / mx;
+      d.kc[q][6] = 0.5 / my;
+    }
     int rc;
     mepgl_ctx* cp = c.get();
 #define UP(field, dst)                                          \
@@ -647,6 +735,7 @@ void mepgl_destroy(mepgl_ctx* c) {
   for (auto& s : c->strings) free_string(c, s);
   hipFree(c->d_strings);
   hipFree(c->d_sc);
+  hipFree(c->d_img);
   hipFree(c->d_ls);
   hipFree(c->d_items);
   hipFree(c->d_jb);
@@ -706,7 +795,8 @@ int mepgl_free_energy(mepgl_ctx* c) {
   int rc = flush(c);
   if (rc) return rc;
   AuxArgs a = aux_args(c);
-  return run_aux(c, a, kKernelEnergy, 0);
+  if ((rc = run_aux(c, a, kKernelEnergy, 0))) return rc;
+  return run_finish(c, finish_args(c), kKernelEnergy);
 }
 
 int mepgl_gradient(mepgl_ctx* c, int si, double* a_g, double* p1_g, double* p2_g) {
@@ -1035,7 +1125,7 @@ int mepgl_synchronize(mepgl_ctx* c) {
 
 int mepgl_info(mepgl_ctx* c, int si, int* tile_rows, int* work_items, long long* bytes) {
   if (!valid_string(c, si)) return MEPGL_ERR_ARG;
-  if (tile_rows) *tile_rows = 4 * c->strings[si].dev.rows_per_thread;
+  if (tile_rows) *tile_rows = c->strings[si].dev.seg_rows;
   if (work_items) *work_items = c->nitems;
   if (bytes) *bytes = c->bytes;
   return MEPGL_OK;

@@ -1,19 +1,27 @@
 // HIP kernels of the MI355X gauged-string-method relaxation loop (gfx950).
 //
-// One NLCG step (nlcgSteps, nlcg.cu:1478-1583) is two launches over a work
-// list of (string, image, tile) items:
+// Work decomposition: one wavefront per work item = one 64-column strip of
+// one row segment of one image of one string.  Lane l owns column
+// 64*strip + l; the wavefront marches down the rows of its segment, keeping
+// the rows j-1, j, j+1 it needs in a register window and prefetching row
+// j+2 while it computes row j.  Neighbours along x come from the adjacent
+// lanes through DPP wave shifts; the two columns just outside the strip are
+// loaded by lanes 0 and 63 only.  Every field value is therefore read from
+// HBM once per launch, plus the halo rows and columns the strip shares with
+// its neighbours (served by L2 when adjacent work items run on the same XCD).
+//
+// One NLCG step (nlcgSteps, nlcg.cu:1478-1583) is two launches:
 //   G  fused pending update x += alpha d (updateFields), gradient
-//      (gradientPsi + gradientA), per-workgroup partials of |g|^2 and
-//      g.(g - g_old) (gradientDensity); the last workgroup of each image
-//      reduces the partials in a fixed order and forms the Polak-Ribiere beta
-//      (computeStepDirection's scalar part).
-//   P  direction d = -g + beta d (computeStepDirection) evaluated on the
+//      (gradientPsi + gradientA), |g|^2 and g.(g - g_old) partials
+//      (gradientDensity); the last wavefront of each image reduces the
+//      partials in a fixed order and forms the Polak-Ribiere beta.
+//   P  direction d = -g + beta d (computeStepDirection), evaluated on the
 //      stencil, line-search quartic partials (polynomialExpansion); the last
-//      workgroup of each image reduces them and runs the line search
+//      wavefront of each image reduces them and runs the line search
 //      (lineSearch) as a parallel scan over a host-built table.
-// Reductions are deterministic (fixed tile order, fixed trees), with two-sum
-// compensation in the mixed-precision path.  The arrival counters reset
-// themselves, so launches can be replayed from a hipGraph.
+// Reductions are deterministic (fixed partial order, fixed trees), with
+// two-sum compensation in the mixed-precision path.  Arrival counters reset
+// themselves, so the launches can be replayed from a hipGraph.
 #include <float.h>
 
 #include "common.hpp"
@@ -24,25 +32,26 @@ namespace mepgl {
 
 // ---------------------------------------------------------------------------
 // Work-item mapping: consecutive items go to the same XCD so that the halo
-// rows a tile shares with its neighbours are served by that XCD's L2.
+// rows and columns a strip shares with its neighbours are served by that
+// XCD's L2.
 __device__ __forceinline__ int xcd_item(int b, int nb) {
   const int q = nb / kNumXcd, r = nb % kNumXcd;
   const int k = b % kNumXcd, l = b / kNumXcd;
   return k * q + (k < r ? k : r) + l;
 }
 
+// Pointers read from descriptors are generic; the hot kernels cast them to
+// the global address space so that loads and stores compile to global_*
+// instructions instead of flat_*.
 template <class T>
-__device__ __forceinline__ T ldg(const T* p, long long off) {
-  return p[off];
+using gptr = __attribute__((address_space(1))) T*;
+
+template <class T>
+__device__ __forceinline__ gptr<T> as_global(T* p) {
+  return (gptr<T>)p;
 }
 
-__device__ __forceinline__ double ld_coherent(const double* p) {
-  unsigned long long v =
-      __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  return __longlong_as_double((long long)v);
-}
-
-// Two-sum accumulation (Knuth), exact error term.
+// Two-sum accumulation (Knuth): exact rounding error kept in c.
 __device__ __forceinline__ void two_sum_acc(double& s, double& c, double v) {
   const double t = s + v;
   const double bp = t - s;
@@ -59,271 +68,351 @@ __device__ __forceinline__ void acc(double& s, double& c, double v) {
     s += v;
 }
 
-// Block-wide reduction of K (sum, compensation) pairs; thread 0 ends with
-// the result.  lds must hold kWavesPerBlock * 2 * K doubles.
+// Wavefront reduction of K (sum, compensation) pairs; every lane ends with
+// the same result (butterfly, operands combined in lane order).
 template <int K, bool COMP>
-__device__ __forceinline__ void block_reduce(double (&s)[K], double (&c)[K], double* lds) {
+__device__ __forceinline__ void wave_reduce(double (&s)[K], double (&c)[K]) {
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) {
+    const bool lo = (threadIdx.x & off) == 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const double so = __shfl_xor(s[k], off);
       if (COMP) {
         const double co = __shfl_xor(c[k], off);
-        two_sum_acc(s[k], c[k], so);
-        c[k] += co;
+        double x = lo ? s[k] : so, y = lo ? so : s[k];
+        double cx = lo ? c[k] : co, cy = lo ? co : c[k];
+        two_sum_acc(x, cx, y);
+        s[k] = x;
+        c[k] = cx + cy;
       } else {
-        s[k] += so;
+        s[k] = lo ? s[k] + so : so + s[k];
       }
     }
   }
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  if (lane == 0) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      lds[w * 2 * K + k] = s[k];
-      lds[w * 2 * K + K + k] = c[k];
-    }
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      double ss = lds[k], cc = lds[K + k];
-      for (int ww = 1; ww < kWavesPerBlock; ++ww) {
-        if (COMP) {
-          two_sum_acc(ss, cc, lds[ww * 2 * K + k]);
-          cc += lds[ww * 2 * K + K + k];
-        } else {
-          ss += lds[ww * 2 * K + k];
-        }
-      }
-      s[k] = ss;
-      c[k] = cc;
-    }
-  }
-  __syncthreads();
 }
 
-// Writes this workgroup's K pairs to part[(n * ntiles + tile) * 2K], then
-// returns true in every thread of the last workgroup of image n to arrive,
-// with the fixed-order reduction over all tiles in fin[K] (valid in every
-// thread of that workgroup).  The counter is reset by the last workgroup.
+// Publishes this wavefront's K (sum, compensation) pairs as partial `tile`
+// of image n.  Partials are combined by a separate finishing launch: a
+// cross-wavefront handshake inside the kernel would need an agent-scope
+// release per wavefront, which on a multi-XCD device writes back the L2.
 template <int K, bool COMP>
-__device__ bool finish_image(double (&s)[K], double (&c)[K], double* lds, double* part,
-                             unsigned* count, int n, int tile, int ntiles, double (&fin)[K]) {
-  __shared__ int s_last;
-  block_reduce<K, COMP>(s, c, lds);
-  double* mine = part + ((long long)n * ntiles + tile) * (2 * K);
+__device__ __forceinline__ void publish_partial(double (&s)[K], double (&c)[K], double* part,
+                                                int n, int tile, int ntiles) {
+  wave_reduce<K, COMP>(s, c);
   if (threadIdx.x == 0) {
+    double* mine = part + ((long long)n * ntiles + tile) * (2 * K);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      __hip_atomic_store(&mine[k], s[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      __hip_atomic_store(&mine[K + k], c[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      mine[k] = s[k];
+      mine[K + k] = c[k];
     }
-    const unsigned prev =
-        __hip_atomic_fetch_add(&count[n], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    s_last = (prev == (unsigned)(ntiles - 1));
   }
-  __syncthreads();
-  if (!s_last) return false;
-  __threadfence();
+}
+
+// Fixed-order total of the ntiles partials of image n (every lane).
+template <int K, bool COMP>
+__device__ __forceinline__ void reduce_partials(const double* part, int n, int ntiles,
+                                                double (&fin)[K]) {
   double ss[K], cc[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) ss[k] = cc[k] = 0.0;
   const double* base = part + (long long)n * ntiles * (2 * K);
-  for (int t = threadIdx.x; t < ntiles; t += kBlock) {
+  for (int t = threadIdx.x; t < ntiles; t += kWave) {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      acc<COMP>(ss[k], cc[k], ld_coherent(&base[(long long)t * 2 * K + k]));
-      if (COMP) cc[k] += ld_coherent(&base[(long long)t * 2 * K + K + k]);
+      acc<COMP>(ss[k], cc[k], base[(long long)t * 2 * K + k]);
+      if (COMP) cc[k] += base[(long long)t * 2 * K + K + k];
     }
   }
-  block_reduce<K, COMP>(ss, cc, lds);
-  __shared__ double s_fin[K];
-  if (threadIdx.x == 0) {
+  wave_reduce<K, COMP>(ss, cc);
 #pragma unroll
-    for (int k = 0; k < K; ++k) s_fin[k] = ss[k] + cc[k];
-    __hip_atomic_store(&count[n], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
-  __syncthreads();
-#pragma unroll
-  for (int k = 0; k < K; ++k) fin[k] = s_fin[k];
-  return true;
+  for (int k = 0; k < K; ++k) fin[k] = ss[k] + cc[k];
 }
 
 // ---------------------------------------------------------------------------
-// Tile geometry of a work item.
-struct TileCtx {
-  int n, tile, i, j0, rstride, nrows;
+// Lane shifts across the wavefront (DPP wave_shr:1 / wave_shl:1).  Lanes
+// without a source keep `edge`, which holds the column outside the strip.
+__device__ __forceinline__ float from_left(float v, float edge) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float from_right(float v, float edge) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double from_left(double v, double edge) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(edge), __double2loint(v), 0x138, 0xF,
+                                             0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(edge), __double2hiint(v), 0x138, 0xF,
+                                             0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double from_right(double v, double edge) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(edge), __double2loint(v), 0x130, 0xF,
+                                             0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(edge), __double2hiint(v), 0x130, 0xF,
+                                             0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Strip geometry of a work item.
+struct Strip {
+  int n, tile;
+  int lane, i, ic;   // lane, column, column clamped into the grid
+  int hc;            // halo column of lanes 0 and 63 (own column elsewhere)
+  bool edge;         // lane 0 or 63
+  int j0, j1;        // rows [j0, j1) of the segment
 };
 
-__device__ __forceinline__ TileCtx tile_of(const DevString& s, const WorkItem& it) {
-  TileCtx t;
+__device__ __forceinline__ Strip strip_of(const DevString& s, const WorkItem& it) {
+  Strip t;
   t.n = it.image;
   t.tile = it.tile;
-  const int tx = it.tile % s.tiles_x, ty = it.tile / s.tiles_x;
-  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  t.i = tx * kTileX + lane;
-  t.j0 = ty * (kWavesPerBlock * s.rows_per_thread) + w;
-  t.rstride = kWavesPerBlock;
-  t.nrows = s.rows_per_thread;
+  const int sx = it.tile % s.tiles_x, sy = it.tile / s.tiles_x;
+  t.lane = threadIdx.x;
+  const int x0 = sx * kTileX;
+  t.i = x0 + t.lane;
+  t.ic = min(t.i, s.N - 1);
+  t.edge = t.lane == 0 || t.lane == kWave - 1;
+  t.hc = t.lane == 0 ? max(x0 - 1, 0) : (t.lane == kWave - 1 ? min(x0 + kWave, s.N - 1) : t.ic);
+  t.j0 = sy * s.seg_rows;
+  t.j1 = min(t.j0 + s.seg_rows, s.N);
   return t;
 }
+
+__device__ __forceinline__ int clampr(int j, int N) { return j < 0 ? 0 : (j >= N ? N - 1 : j); }
 
 __device__ __forceinline__ double hval(const DevString& s, long long off) {
   return s.h.p ? s.h.p[off] : s.h.v;
 }
 
 // ---------------------------------------------------------------------------
-// G kernel body.
+// G kernel.
 template <class T, int NC>
-__device__ void grad_body(const StepArgs& a, const DevString& s, const WorkItem& it,
-                          double* lds) {
+struct GRaw {
+  static constexpr int P = 2 + 2 * NC;
+  T x[P], d[P], hx[P], hd[P];
+  uint16_t nn;
+};
+
+template <class T, int NC>
+struct GRow {
+  T ax, ay, axl, axr, ayl, ayr;  // A and its x-neighbours
+  T u[NC], v[NC];                // psi
+  T eu[NC], ev[NC];              // psi outside the strip (lanes 0 / 63)
+  uint16_t nn;
+};
+
+template <class T, int NC>
+__device__ __forceinline__ void g_load(GRaw<T, NC>& r, gptr<const T> __restrict__ X,
+                                       gptr<const T> __restrict__ D, long long ps, int P_, int row,
+                                       const Strip& t, bool upd, bool nsf,
+                                       gptr<const uint16_t> __restrict__ nn) {
+  const long long off = (long long)row * P_ + t.ic, hoff = (long long)row * P_ + t.hc;
+#pragma unroll
+  for (int p = 0; p < GRaw<T, NC>::P; ++p) {
+    r.x[p] = X[p * ps + off];
+    r.d[p] = (upd && !(nsf && p < 2)) ? D[p * ps + off] : (T)0;
+  }
+  // Halo columns: lanes 0 and 63 read the columns outside the strip, the
+  // other lanes re-read their own column (an L1 hit) so no lane branches.
+#pragma unroll
+  for (int p = 0; p < GRaw<T, NC>::P; ++p) {
+    r.hx[p] = X[p * ps + hoff];
+    r.hd[p] = (upd && !(nsf && p < 2)) ? D[p * ps + hoff] : (T)0;
+  }
+  r.nn = nn[off];
+}
+
+template <class T, int NC>
+__device__ __forceinline__ void g_process(GRow<T, NC>& w, const GRaw<T, NC>& r, T al, bool upd) {
+  T c[GRaw<T, NC>::P], e[GRaw<T, NC>::P];
+#pragma unroll
+  for (int p = 0; p < GRaw<T, NC>::P; ++p) {
+    c[p] = upd ? fma(al, r.d[p], r.x[p]) : r.x[p];
+    e[p] = upd ? fma(al, r.hd[p], r.hx[p]) : r.hx[p];
+  }
+  w.ax = c[PAX];
+  w.ay = c[PAY];
+  w.axl = from_left(c[PAX], e[PAX]);
+  w.axr = from_right(c[PAX], e[PAX]);
+  w.ayl = from_left(c[PAY], e[PAY]);
+  w.ayr = from_right(c[PAY], e[PAY]);
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    w.u[q] = c[PU1 + 2 * q];
+    w.v[q] = c[PV1 + 2 * q];
+    w.eu[q] = e[PU1 + 2 * q];
+    w.ev[q] = e[PV1 + 2 * q];
+  }
+  w.nn = r.nn;
+}
+
+template <class T, int NC, bool UPD, bool NSF>
+__device__ void grad_body(const StepArgs& a, const DevString& s, const WorkItem& it) {
   constexpr bool COMP = sizeof(T) == 4;
-  const TileCtx tc = tile_of(s, it);
-  const int N = s.N, P = s.pitch;
+  const Strip t = strip_of(s, it);
+  const int N = s.N, P_ = s.pitch;
   const long long ps = s.plane_stride;
-  const long long ib = (long long)tc.n * s.image_stride;
-  const T* X = (const T*)s.x[a.xin] + ib;
-  T* XO = (T*)s.x[a.xout] + ib;
-  const T* D = (const T*)s.d[a.din] + ib;
-  T* G = (T*)(a.scratch ? s.gs : s.g[a.gcur]) + ib;
-  const T* GO = (const T*)s.g[a.gold] + ib;
-  ImageScalars& sc = a.sc[s.img0 + tc.n];
-  const bool upd = a.upd;
+  const long long ib = (long long)t.n * s.image_stride;
+  gptr<const T> __restrict__ X = as_global((const T*)s.x[a.xin] + ib);
+  gptr<T> __restrict__ XO = as_global((T*)s.x[a.xout] + ib);
+  gptr<const T> __restrict__ D = as_global((const T*)s.d[a.din] + ib);
+  gptr<T> __restrict__ G = as_global((T*)(a.scratch ? s.gs : s.g[a.gcur]) + ib);
+  gptr<const T> __restrict__ GO = as_global((const T*)s.g[a.gold] + ib);
+  gptr<const uint16_t> __restrict__ NN = as_global(s.nn);
+  ImageScalars& sc = a.sc[s.img0 + t.n];
+  constexpr bool upd = UPD;
   const T al = upd ? (T)sc.alpha : (T)0;
-  const bool do_a = !s.nsf;
+  constexpr bool nsf = NSF;
+  constexpr bool do_a = !NSF;
   const bool do_psi = !a.maxwell;
+  const bool store = t.i < N;
   const T idx = (T)s.inv_dx, dx2 = (T)s.dx2;
   const T eta = (T)s.eta, gam = (T)s.gamma, del = (T)s.delta;
+  CompConst<T> kconst[NC];
+  if (!s.planes) load_comp_const<T, NC>(s, 0, kconst);
   double acc_s[2] = {0, 0}, acc_c[2] = {0, 0};
 
-  auto fld = [&](int p, long long off) -> T {
-    const T xv = X[p * ps + off];
-    return upd ? fma(al, D[p * ps + off], xv) : xv;
-  };
+  GRaw<T, NC> raw;
+  GRow<T, NC> wp, wc, wn;
+  g_load<T, NC>(raw, X, D, ps, P_, clampr(t.j0 - 1, N), t, upd, nsf, NN);
+  g_process<T, NC>(wp, raw, al, upd);
+  g_load<T, NC>(raw, X, D, ps, P_, clampr(t.j0, N), t, upd, nsf, NN);
+  g_process<T, NC>(wc, raw, al, upd);
+  g_load<T, NC>(raw, X, D, ps, P_, clampr(t.j0 + 1, N), t, upd, nsf, NN);
+  g_process<T, NC>(wn, raw, al, upd);
 
-  for (int r = 0; r < tc.nrows; ++r) {
-    const int i = tc.i, j = tc.j0 + r * tc.rstride;
-    if (i >= N || j >= N) continue;
-    const long long off = (long long)j * P + i;
-    const int ip = min(i + 1, N - 1), im = max(i - 1, 0);
-    const int jp = min(j + 1, N - 1), jm = max(j - 1, 0);
-    const long long offs[5] = {off, (long long)j * P + ip, (long long)j * P + im,
-                               (long long)jp * P + i, (long long)jm * P + i};
-    const uint16_t nnv = s.nn[off];
-    const int sc_bits = nnv & 0xff, cm_bits = nnv >> 8;
+#pragma clang loop unroll(disable)
+  for (int j = t.j0; j < t.j1; ++j) {
+    // Prefetch row j + 2 while row j is computed.
+    const bool more = j + 1 < t.j1;
+    if (more) g_load<T, NC>(raw, X, D, ps, P_, clampr(j + 2, N), t, upd, nsf, NN);
+    const long long off = (long long)j * P_ + t.ic;
+    T go[GRaw<T, NC>::P];
+    if (!a.scratch) {
+#pragma unroll
+      for (int p = 0; p < GRaw<T, NC>::P; ++p) go[p] = (do_a || p >= 2) ? GO[p * ps + off] : (T)0;
+    }
+    T gstale[NC][2];
+    if (!do_psi) {
+#pragma unroll
+      for (int q = 0; q < NC; ++q) {
+        gstale[q][0] = G[(PU1 + 2 * q) * ps + off];
+        gstale[q][1] = G[(PV1 + 2 * q) * ps + off];
+      }
+    }
 
     Nbhd<T, NC> nb;
+    nb.ax[0][0] = wp.axl; nb.ax[0][1] = wp.ax; nb.ax[0][2] = wp.axr;
+    nb.ax[1][0] = wc.axl; nb.ax[1][1] = wc.ax; nb.ax[1][2] = wc.axr;
+    nb.ax[2][0] = wn.axl; nb.ax[2][1] = wn.ax; nb.ax[2][2] = wn.axr;
+    nb.ay[0][0] = wp.ayl; nb.ay[0][1] = wp.ay; nb.ay[0][2] = wp.ayr;
+    nb.ay[1][0] = wc.ayl; nb.ay[1][1] = wc.ay; nb.ay[1][2] = wc.ayr;
+    nb.ay[2][0] = wn.ayl; nb.ay[2][1] = wn.ay; nb.ay[2][2] = wn.ayr;
 #pragma unroll
-    for (int dy = 0; dy < 3; ++dy) {
-      const int jj = dy == 0 ? jm : (dy == 1 ? j : jp);
-#pragma unroll
-      for (int dxx = 0; dxx < 3; ++dxx) {
-        const int ii = dxx == 0 ? im : (dxx == 1 ? i : ip);
-        const long long o = (long long)jj * P + ii;
-        const bool need = do_a || (dy == 1) || (dxx == 1);
-        nb.ax[dy][dxx] = need ? fld(PAX, o) : (T)0;
-        nb.ay[dy][dxx] = need ? fld(PAY, o) : (T)0;
-      }
+    for (int q = 0; q < NC; ++q) {
+      nb.u[q][0] = wc.u[q];
+      nb.v[q][0] = wc.v[q];
+      nb.u[q][1] = from_right(wc.u[q], wc.eu[q]);
+      nb.v[q][1] = from_right(wc.v[q], wc.ev[q]);
+      nb.u[q][2] = from_left(wc.u[q], wc.eu[q]);
+      nb.v[q][2] = from_left(wc.v[q], wc.ev[q]);
+      nb.u[q][3] = wn.u[q];
+      nb.v[q][3] = wn.v[q];
+      nb.u[q][4] = wp.u[q];
+      nb.v[q][4] = wp.v[q];
     }
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-#pragma unroll
-      for (int t = 0; t < 5; ++t) {
-        nb.u[c][t] = fld(PU1 + 2 * c, offs[t]);
-        nb.v[c][t] = fld(PV1 + 2 * c, offs[t]);
-      }
-    }
-#pragma unroll
-    for (int t = 0; t < 5; ++t) nb.h[t] = (T)hval(s, offs[t]);
-
-    CompConst<T> k[NC];
-    load_comp_const<T, NC>(s, off, k);
-    T gp[NC][2], ga[2] = {0, 0};
-    site_gradient<T, NC>(nb, k, eta, gam, del, idx, dx2, sc_bits, cm_bits, do_psi, do_a, gp, ga);
-
-    // Stores: gradient, then the updated fields of this site.
-    if (do_psi) {
-#pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        G[(PU1 + 2 * c) * ps + off] = gp[c][0];
-        G[(PV1 + 2 * c) * ps + off] = gp[c][1];
-      }
+    if (s.h.p) {
+      const int ip = min(t.ic + 1, N - 1), im = max(t.ic - 1, 0);
+      nb.h[0] = (T)s.h.p[off];
+      nb.h[1] = (T)s.h.p[(long long)j * P_ + ip];
+      nb.h[2] = (T)s.h.p[(long long)j * P_ + im];
+      nb.h[3] = (T)s.h.p[(long long)clampr(j + 1, N) * P_ + t.ic];
+      nb.h[4] = (T)s.h.p[(long long)clampr(j - 1, N) * P_ + t.ic];
     } else {
 #pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        gp[c][0] = G[(PU1 + 2 * c) * ps + off];
-        gp[c][1] = G[(PV1 + 2 * c) * ps + off];
-      }
+      for (int q = 0; q < 5; ++q) nb.h[q] = (T)s.h.v;
     }
-    if (do_a) {
-      G[PAX * ps + off] = ga[0];
-      G[PAY * ps + off] = ga[1];
-    }
-    if (upd) {
-      if (do_a) {
-        XO[PAX * ps + off] = nb.ax[1][1];
-        XO[PAY * ps + off] = nb.ay[1][1];
-      }
+    CompConst<T> k[NC];
+    if (s.planes)
+      load_comp_const<T, NC>(s, off, k);
+    else
 #pragma unroll
-      for (int c = 0; c < NC; ++c) {
-        XO[(PU1 + 2 * c) * ps + off] = nb.u[c][0];
-        XO[(PV1 + 2 * c) * ps + off] = nb.v[c][0];
+      for (int q = 0; q < NC; ++q) k[q] = kconst[q];
+
+    T gp[NC][2], ga[2] = {0, 0};
+    site_gradient<T, NC>(nb, k, eta, gam, del, idx, dx2, wc.nn & 0xff, wc.nn >> 8, do_psi, do_a,
+                         gp, ga);
+    if (!do_psi) {
+#pragma unroll
+      for (int q = 0; q < NC; ++q) {
+        gp[q][0] = gstale[q][0];
+        gp[q][1] = gstale[q][1];
       }
     }
-    if (!a.scratch) {
-      // gradientDensity, nlcg.cu:919-949
-      double n2 = 0, dg = 0;
-      if (do_a) {
+    if (store) {
+      if (do_psi) {
 #pragma unroll
-        for (int q = 0; q < 2; ++q) {
-          const double gv = (double)ga[q], go = (double)GO[q * ps + off];
-          n2 += gv * gv;
-          dg += gv * (gv - go);
+        for (int q = 0; q < NC; ++q) {
+          G[(PU1 + 2 * q) * ps + off] = gp[q][0];
+          G[(PV1 + 2 * q) * ps + off] = gp[q][1];
         }
       }
+      if (do_a) {
+        G[PAX * ps + off] = ga[0];
+        G[PAY * ps + off] = ga[1];
+      }
+      if (upd) {
+        if (do_a) {
+          XO[PAX * ps + off] = wc.ax;
+          XO[PAY * ps + off] = wc.ay;
+        }
 #pragma unroll
-      for (int c = 0; c < NC; ++c) {
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-          const double gv = (double)gp[c][q], go = (double)GO[(PU1 + 2 * c + q) * ps + off];
-          n2 += gv * gv;
-          dg += gv * (gv - go);
+        for (int q = 0; q < NC; ++q) {
+          XO[(PU1 + 2 * q) * ps + off] = wc.u[q];
+          XO[(PV1 + 2 * q) * ps + off] = wc.v[q];
         }
       }
-      acc<COMP>(acc_s[0], acc_c[0], n2);
-      acc<COMP>(acc_s[1], acc_c[1], dg);
+      if (!a.scratch) {
+        // gradientDensity, nlcg.cu:919-949
+        double n2 = 0, dg = 0;
+        if (do_a) {
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const double gv = (double)ga[q], go_ = (double)go[q];
+            n2 += gv * gv;
+            dg += gv * (gv - go_);
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+#pragma unroll
+          for (int r = 0; r < 2; ++r) {
+            const double gv = (double)gp[q][r], go_ = (double)go[PU1 + 2 * q + r];
+            n2 += gv * gv;
+            dg += gv * (gv - go_);
+          }
+        }
+        acc<COMP>(acc_s[0], acc_c[0], n2);
+        acc<COMP>(acc_s[1], acc_c[1], dg);
+      }
     }
+    // Advance the window.
+    wp = wc;
+    wc = wn;
+    if (more) g_process<T, NC>(wn, raw, al, upd);
   }
   if (a.scratch) return;
 
-  double fin[2];
-  const int ntiles = s.tiles_x * s.tiles_y;
-  if (finish_image<2, COMP>(acc_s, acc_c, lds, s.part_g, s.count_g, tc.n, tc.tile, ntiles, fin)) {
-    if (threadIdx.x == 0) {
-      sc.g_norm2[a.scur] = fin[0];
-      sc.g_delta_g = fin[1];
-      double beta = 0;
-      if (a.nlcg) {
-        beta = fin[1] / sc.g_norm2[a.sold];
-        beta = fmax(beta, 0.0);
-      }
-      sc.beta = beta;
-    }
-  }
+  publish_partial<2, COMP>(acc_s, acc_c, s.part_g, t.n, t.tile, s.tiles_x * s.tiles_y);
 }
 
-template <class T>
+template <class T, int NC, bool UPD, bool NSF>
 __global__ __launch_bounds__(kBlock) void k_grad(StepArgs a) {
-  __shared__ double lds[kWavesPerBlock * 2 * kPartP];
   const WorkItem it = a.items[xcd_item(blockIdx.x, a.nitems)];
-  const DevString& s = a.strings[it.string];
-  if (s.nc == 2)
-    grad_body<T, 2>(a, s, it, lds);
-  else
-    grad_body<T, 1>(a, s, it, lds);
+  grad_body<T, NC, UPD, NSF>(a, a.strings[it.string], it);
 }
 
 // ---------------------------------------------------------------------------
@@ -342,126 +431,277 @@ __device__ __forceinline__ double quartic_at(const LineSearchTable& t, int k, do
   return f;
 }
 
-__device__ int line_search_block(const LineSearchTable& t, double A, double B, double C, double D,
-                                 double E) {
-  __shared__ int s_first;
-  for (int base = 0; base < kLsSteps; base += kBlock) {
-    if (threadIdx.x == 0) s_first = kLsSteps;
-    __syncthreads();
-    const int p = base + threadIdx.x;
+__device__ int line_search_wave(const LineSearchTable& t, double A, double B, double C, double D,
+                                double E) {
+  for (int base = 0; base < kLsSteps; base += kWave) {
+    const int p = base + (int)threadIdx.x;
+    bool rise = false;
     if (p < kLsSteps) {
       const double f_old = quartic_at(t, p, A, B, C, D, E);
       const double f_new = quartic_at(t, p + 1, A, B, C, D, E);
-      if (f_new > f_old) atomicMin(&s_first, p);
+      rise = f_new > f_old;
     }
-    __syncthreads();
-    const int first = s_first;
-    __syncthreads();
-    if (first < kLsSteps) return first;
+    const unsigned long long m = __ballot(rise);
+    if (m) return base + __ffsll((long long)m) - 1;
   }
   return kLsSteps - 1;
 }
 
-// P kernel body.
+// ---------------------------------------------------------------------------
+// P kernel.
 template <class T, int NC>
-__device__ void poly_body(const StepArgs& a, const DevString& s, const WorkItem& it,
-                          double* lds) {
+struct PRaw {
+  static constexpr int P = 2 + 2 * NC;
+  T x[P], g[P], d[P], hx[P], hg[P], hd[P];
+  uint16_t nn;
+};
+
+template <class T, int NC>
+struct PRow {
+  T ax, ay, dax, day;              // A and its new direction
+  T eax, eay, edax, eday;          // the same outside the strip (lanes 0 / 63)
+  T u[NC], v[NC], du[NC], dv[NC];  // psi and its new direction
+  T eu[NC], ev[NC], edu[NC], edv[NC];
+  uint16_t nn;
+};
+
+template <class T, int NC>
+__device__ __forceinline__ void p_load(PRaw<T, NC>& r, gptr<const T> __restrict__ X,
+                                       gptr<const T> __restrict__ G, gptr<const T> __restrict__ D,
+                                       long long ps, int P_, int row, const Strip& t, bool nsf,
+                                       gptr<const uint16_t> __restrict__ nn) {
+  const long long off = (long long)row * P_ + t.ic, hoff = (long long)row * P_ + t.hc;
+#pragma unroll
+  for (int p = 0; p < PRaw<T, NC>::P; ++p) {
+    const bool dir = !(nsf && p < 2);
+    r.x[p] = X[p * ps + off];
+    r.g[p] = dir ? G[p * ps + off] : (T)0;
+    r.d[p] = dir ? D[p * ps + off] : (T)0;
+  }
+#pragma unroll
+  for (int p = 0; p < PRaw<T, NC>::P; ++p) {
+    const bool dir = !(nsf && p < 2);
+    r.hx[p] = X[p * ps + hoff];
+    r.hg[p] = dir ? G[p * ps + hoff] : (T)0;
+    r.hd[p] = dir ? D[p * ps + hoff] : (T)0;
+  }
+  r.nn = nn[off];
+}
+
+// d = -g + beta d  (computeStepDirection, nlcg.cu:985-993)
+template <class T, int NC>
+__device__ __forceinline__ void p_process(PRow<T, NC>& w, const PRaw<T, NC>& r, T beta) {
+  T dn[PRaw<T, NC>::P], en[PRaw<T, NC>::P];
+#pragma unroll
+  for (int p = 0; p < PRaw<T, NC>::P; ++p) {
+    dn[p] = fma(beta, r.d[p], -r.g[p]);
+    en[p] = fma(beta, r.hd[p], -r.hg[p]);
+  }
+  w.ax = r.x[PAX];
+  w.ay = r.x[PAY];
+  w.eax = r.hx[PAX];
+  w.eay = r.hx[PAY];
+  w.dax = dn[PAX];
+  w.day = dn[PAY];
+  w.edax = en[PAX];
+  w.eday = en[PAY];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    w.u[q] = r.x[PU1 + 2 * q];
+    w.v[q] = r.x[PV1 + 2 * q];
+    w.eu[q] = r.hx[PU1 + 2 * q];
+    w.ev[q] = r.hx[PV1 + 2 * q];
+    w.du[q] = dn[PU1 + 2 * q];
+    w.dv[q] = dn[PV1 + 2 * q];
+    w.edu[q] = en[PU1 + 2 * q];
+    w.edv[q] = en[PV1 + 2 * q];
+  }
+  w.nn = r.nn;
+}
+
+template <class T, int NC, bool NSF>
+__device__ void poly_body(const StepArgs& a, const DevString& s, const WorkItem& it) {
   constexpr bool COMP = sizeof(T) == 4;
-  const TileCtx tc = tile_of(s, it);
-  const int N = s.N, P = s.pitch;
+  const Strip t = strip_of(s, it);
+  const int N = s.N, P_ = s.pitch;
   const long long ps = s.plane_stride;
-  const long long ib = (long long)tc.n * s.image_stride;
-  const T* X = (const T*)s.x[a.xin] + ib;
-  const T* G = (const T*)s.g[a.gcur] + ib;
-  const T* D = (const T*)s.d[a.din] + ib;
-  T* DO = (T*)s.d[a.dout] + ib;
-  ImageScalars& sc = a.sc[s.img0 + tc.n];
+  const long long ib = (long long)t.n * s.image_stride;
+  gptr<const T> __restrict__ X = as_global((const T*)s.x[a.xin] + ib);
+  gptr<const T> __restrict__ G = as_global((const T*)s.g[a.gcur] + ib);
+  gptr<const T> __restrict__ D = as_global((const T*)s.d[a.din] + ib);
+  gptr<T> __restrict__ DO = as_global((T*)s.d[a.dout] + ib);
+  gptr<const uint16_t> __restrict__ NN = as_global(s.nn);
+  ImageScalars& sc = a.sc[s.img0 + t.n];
   const T beta = (T)sc.beta;
-  const bool nsf = s.nsf;
+  constexpr bool nsf = NSF;
   const bool exact = s.poly_exact;
+  const bool store = t.i < N;
   const T idx = (T)s.inv_dx, dx2 = (T)s.dx2;
   const T eta = (T)s.eta, gam = (T)s.gamma, del = (T)s.delta;
+  CompConst<T> kconst[NC];
+  if (!s.planes) load_comp_const<T, NC>(s, 0, kconst);
   double acc_s[5] = {0, 0, 0, 0, 0}, acc_c[5] = {0, 0, 0, 0, 0};
 
-  // d = -g + beta d  (computeStepDirection, nlcg.cu:985-993)
-  auto dir = [&](int p, long long off) -> T { return fma(beta, D[p * ps + off], -G[p * ps + off]); };
+  PRaw<T, NC> raw;
+  PRow<T, NC> wc, wn;
+  T pax, pdax;  // row j-1: A_x and its direction
+  p_load<T, NC>(raw, X, G, D, ps, P_, clampr(t.j0 - 1, N), t, nsf, NN);
+  p_process<T, NC>(wc, raw, beta);
+  pax = wc.ax;
+  pdax = wc.dax;
+  p_load<T, NC>(raw, X, G, D, ps, P_, clampr(t.j0, N), t, nsf, NN);
+  p_process<T, NC>(wc, raw, beta);
+  p_load<T, NC>(raw, X, G, D, ps, P_, clampr(t.j0 + 1, N), t, nsf, NN);
+  p_process<T, NC>(wn, raw, beta);
 
-  for (int r = 0; r < tc.nrows; ++r) {
-    const int i = tc.i, j = tc.j0 + r * tc.rstride;
-    if (i >= N || j >= N) continue;
-    const long long off = (long long)j * P + i;
-    const int ip = min(i + 1, N - 1), im = max(i - 1, 0);
-    const int jp = min(j + 1, N - 1), jm = max(j - 1, 0);
-    const long long offs[5] = {off, (long long)j * P + ip, (long long)j * P + im,
-                               (long long)jp * P + i, (long long)jm * P + i};
-    const uint16_t nnv = s.nn[off];
-    const int sc_bits = nnv & 0xff, cm_bits = nnv >> 8;
+#pragma clang loop unroll(disable)
+  for (int j = t.j0; j < t.j1; ++j) {
+    const bool more = j + 1 < t.j1;
+    if (more) p_load<T, NC>(raw, X, G, D, ps, P_, clampr(j + 2, N), t, nsf, NN);
+    const long long off = (long long)j * P_ + t.ic;
 
     PolyNbhd<T, NC> nb;
+    nb.ax[0] = wc.ax;
+    nb.ax[1] = from_right(wc.ax, wc.eax);
+    nb.ax[2] = 0;
+    nb.ax[3] = wn.ax;
+    nb.ax[4] = pax;
+    nb.ay[0] = wc.ay;
+    nb.ay[1] = from_right(wc.ay, wc.eay);
+    nb.ay[2] = from_left(wc.ay, wc.eay);
+    nb.ay[3] = wn.ay;
+    nb.ay[4] = 0;
+    nb.dax[0] = wc.dax;
+    nb.dax[1] = from_right(wc.dax, wc.edax);
+    nb.dax[2] = 0;
+    nb.dax[3] = wn.dax;
+    nb.dax[4] = pdax;
+    nb.day[0] = wc.day;
+    nb.day[1] = from_right(wc.day, wc.eday);
+    nb.day[2] = from_left(wc.day, wc.eday);
+    nb.day[3] = wn.day;
+    nb.day[4] = 0;
 #pragma unroll
-    for (int t = 0; t < 5; ++t) {
-      nb.ax[t] = X[PAX * ps + offs[t]];
-      nb.ay[t] = X[PAY * ps + offs[t]];
-      nb.dax[t] = nsf ? (T)0 : dir(PAX, offs[t]);
-      nb.day[t] = nsf ? (T)0 : dir(PAY, offs[t]);
+    for (int q = 0; q < NC; ++q) {
+      nb.u[q][0] = wc.u[q];
+      nb.v[q][0] = wc.v[q];
+      nb.du[q][0] = wc.du[q];
+      nb.dv[q][0] = wc.dv[q];
+      nb.u[q][1] = from_right(wc.u[q], wc.eu[q]);
+      nb.v[q][1] = from_right(wc.v[q], wc.ev[q]);
+      nb.du[q][1] = from_right(wc.du[q], wc.edu[q]);
+      nb.dv[q][1] = from_right(wc.dv[q], wc.edv[q]);
+      nb.u[q][2] = wn.u[q];
+      nb.v[q][2] = wn.v[q];
+      nb.du[q][2] = wn.du[q];
+      nb.dv[q][2] = wn.dv[q];
     }
-    const int ps3[3] = {0, 1, 3};
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-#pragma unroll
-      for (int t = 0; t < 3; ++t) {
-        const long long o = offs[ps3[t]];
-        nb.u[c][t] = X[(PU1 + 2 * c) * ps + o];
-        nb.v[c][t] = X[(PV1 + 2 * c) * ps + o];
-        nb.du[c][t] = dir(PU1 + 2 * c, o);
-        nb.dv[c][t] = dir(PV1 + 2 * c, o);
-      }
-    }
-    nb.h0 = (T)hval(s, off);
-
-    if (!nsf) {
-      DO[PAX * ps + off] = nb.dax[0];
-      DO[PAY * ps + off] = nb.day[0];
-    }
-#pragma unroll
-    for (int c = 0; c < NC; ++c) {
-      DO[(PU1 + 2 * c) * ps + off] = nb.du[c][0];
-      DO[(PV1 + 2 * c) * ps + off] = nb.dv[c][0];
-    }
-
+    nb.h0 = (T)(s.h.p ? s.h.p[off] : s.h.v);
     CompConst<T> k[NC];
-    load_comp_const<T, NC>(s, off, k);
-    const Quartic<T> q = site_poly<T, NC>(nb, k, eta, gam, del, idx, dx2, sc_bits, cm_bits, nsf,
-                                          exact);
+    if (s.planes)
+      load_comp_const<T, NC>(s, off, k);
+    else
 #pragma unroll
-    for (int t = 0; t < 5; ++t) acc<COMP>(acc_s[t], acc_c[t], (double)q.c[t]);
+      for (int q = 0; q < NC; ++q) k[q] = kconst[q];
+
+    if (store) {
+      if (!nsf) {
+        DO[PAX * ps + off] = wc.dax;
+        DO[PAY * ps + off] = wc.day;
+      }
+#pragma unroll
+      for (int q = 0; q < NC; ++q) {
+        DO[(PU1 + 2 * q) * ps + off] = wc.du[q];
+        DO[(PV1 + 2 * q) * ps + off] = wc.dv[q];
+      }
+      const Quartic<T> qv =
+          site_poly<T, NC>(nb, k, eta, gam, del, idx, dx2, wc.nn & 0xff, wc.nn >> 8, nsf, exact);
+#pragma unroll
+      for (int r = 0; r < 5; ++r) acc<COMP>(acc_s[r], acc_c[r], (double)qv.c[r]);
+    }
+    pax = wc.ax;
+    pdax = wc.dax;
+    wc = wn;
+    if (more) p_process<T, NC>(wn, raw, beta);
   }
 
-  double fin[5];
-  const int ntiles = s.tiles_x * s.tiles_y;
-  if (finish_image<5, COMP>(acc_s, acc_c, lds, s.part_p, s.count_p, tc.n, tc.tile, ntiles, fin)) {
-    const int first = line_search_block(*a.ls, fin[0], fin[1], fin[2], fin[3], fin[4]);
-    if (threadIdx.x == 0) {
-      sc.fenergy = fin[0];
-      sc.poly[0] = fin[1];
-      sc.poly[1] = fin[2];
-      sc.poly[2] = fin[3];
-      sc.poly[3] = fin[4];
-      sc.alpha = a.ls->alpha[first];
-      sc.ls_index = first;
+  publish_partial<5, COMP>(acc_s, acc_c, s.part_p, t.n, t.tile, s.tiles_x * s.tiles_y);
+}
+
+template <class T, int NC, bool NSF>
+__global__ __launch_bounds__(kBlock) void k_poly(StepArgs a) {
+  const WorkItem it = a.items[xcd_item(blockIdx.x, a.nitems)];
+  poly_body<T, NC, NSF>(a, a.strings[it.string], it);
+}
+
+// ---------------------------------------------------------------------------
+// Finishing launches: one wavefront per image (global image id = block).
+__device__ __forceinline__ const DevString& string_of_image(const FinishArgs& f, int& n) {
+  const int2 si = f.img[blockIdx.x];
+  n = si.y;
+  return f.strings[si.x];
+}
+
+// After G: |g|^2, g.(g - g_old) and the Polak-Ribiere beta
+// (computeStepDirection, nlcg.cu:972-982).
+template <bool COMP>
+__global__ __launch_bounds__(kBlock) void k_finish_grad(FinishArgs f) {
+  int n;
+  const DevString& s = string_of_image(f, n);
+  double fin[2];
+  reduce_partials<2, COMP>(s.part_g, n, s.tiles_x * s.tiles_y, fin);
+  if (threadIdx.x == 0) {
+    ImageScalars& sc = f.sc[blockIdx.x];
+    sc.g_norm2[f.scur] = fin[0];
+    sc.g_delta_g = fin[1];
+    double beta = 0;
+    if (f.nlcg) {
+      beta = fin[1] / sc.g_norm2[f.sold];
+      beta = fmax(beta, 0.0);
     }
+    sc.beta = beta;
   }
 }
 
-template <class T>
-__global__ __launch_bounds__(kBlock) void k_poly(StepArgs a) {
-  __shared__ double lds[kWavesPerBlock * 2 * kPartP];
-  const WorkItem it = a.items[xcd_item(blockIdx.x, a.nitems)];
-  const DevString& s = a.strings[it.string];
-  if (s.nc == 2)
-    poly_body<T, 2>(a, s, it, lds);
-  else
-    poly_body<T, 1>(a, s, it, lds);
+// After P: the quartic A..E and the line search (lineSearch).
+template <bool COMP>
+__global__ __launch_bounds__(kBlock) void k_finish_poly(FinishArgs f) {
+  int n;
+  const DevString& s = string_of_image(f, n);
+  double fin[5];
+  reduce_partials<5, COMP>(s.part_p, n, s.tiles_x * s.tiles_y, fin);
+  const int first = line_search_wave(*f.ls, fin[0], fin[1], fin[2], fin[3], fin[4]);
+  if (threadIdx.x == 0) {
+    ImageScalars& sc = f.sc[blockIdx.x];
+    sc.fenergy = fin[0];
+    sc.poly[0] = fin[1];
+    sc.poly[1] = fin[2];
+    sc.poly[2] = fin[3];
+    sc.poly[3] = fin[4];
+    sc.alpha = f.ls->alpha[first];
+    sc.ls_index = first;
+  }
+}
+
+// After the energy pass (computeFreeEnergy).
+__global__ __launch_bounds__(kBlock) void k_finish_energy(FinishArgs f) {
+  int n;
+  const DevString& s = string_of_image(f, n);
+  double fin[1];
+  reduce_partials<1, true>(s.part_p, n, s.tiles_x * s.tiles_y, fin);
+  if (threadIdx.x == 0) f.out[blockIdx.x] = fin[0];
+}
+
+// After the arclength pass: squared distances to the previous image.
+__global__ __launch_bounds__(kBlock) void k_finish_arclen(FinishArgs f) {
+  int n;
+  const DevString& s = string_of_image(f, n);
+  double fin[2];
+  reduce_partials<2, true>(s.part_a, n, s.tiles_x * s.tiles_y, fin);
+  if (threadIdx.x == 0) {
+    f.sc[blockIdx.x].ds2 = fin[0];
+    f.sc[blockIdx.x].ds2_jb = fin[1];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -471,16 +711,15 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_update(StepArgs a) {
   const WorkItem it = a.items[xcd_item(blockIdx.x, a.nitems)];
   const DevString& s = a.strings[it.string];
-  const TileCtx tc = tile_of(s, it);
-  const long long ib = (long long)tc.n * s.image_stride;
+  const Strip t = strip_of(s, it);
+  if (t.i >= s.N) return;
+  const long long ib = (long long)t.n * s.image_stride;
   T* X = (T*)s.x[a.xin] + ib;
   const T* D = (const T*)s.d[a.din] + ib;
-  const T al = (T)a.sc[s.img0 + tc.n].alpha;
+  const T al = (T)a.sc[s.img0 + t.n].alpha;
   const int p0 = s.nsf ? 2 : 0;
-  for (int r = 0; r < tc.nrows; ++r) {
-    const int i = tc.i, j = tc.j0 + r * tc.rstride;
-    if (i >= s.N || j >= s.N) continue;
-    const long long off = (long long)j * s.pitch + i;
+  for (int j = t.j0; j < t.j1; ++j) {
+    const long long off = (long long)j * s.pitch + t.i;
     for (int p = p0; p < s.nplanes; ++p) {
       const long long o = p * s.plane_stride + off;
       X[o] = fma(al, D[o], X[o]);
@@ -527,58 +766,51 @@ __device__ __forceinline__ void gather5(const DevString& s, const T* X, int i, i
 
 // computeFreeEnergy (nlcg.cu:105-240), always evaluated in double.
 template <class T, int NC>
-__device__ void energy_body(const AuxArgs& a, const DevString& s, const WorkItem& it,
-                            double* lds) {
-  const TileCtx tc = tile_of(s, it);
-  const T* X = (const T*)s.x[a.xin] + (long long)tc.n * s.image_stride;
+__device__ void energy_body(const AuxArgs& a, const DevString& s, const WorkItem& it) {
+  const Strip t = strip_of(s, it);
+  const T* X = (const T*)s.x[a.xin] + (long long)t.n * s.image_stride;
   const double idx = s.inv_dx, dx2 = s.dx2;
   double acc_s[1] = {0}, acc_c[1] = {0};
-  for (int r = 0; r < tc.nrows; ++r) {
-    const int i = tc.i, j = tc.j0 + r * tc.rstride;
-    if (i >= s.N || j >= s.N) continue;
-    const long long off = (long long)j * s.pitch + i;
-    PolyNbhd<double, NC> nb;
-    double um[NC], vm[NC], uym[NC], vym[NC];
-    gather5<T, NC>(s, X, i, j, nb, um, vm, uym, vym);
-    CompConst<double> k[NC];
-    load_comp_const<double, NC>(s, off, k);
-    const uint16_t nnv = s.nn[off];
-    const double e = site_energy<NC>(nb, k, s.eta, s.gamma, s.delta, idx, dx2, nnv & 0xff,
-                                     nnv >> 8, s.nsf);
-    two_sum_acc(acc_s[0], acc_c[0], e);
+  if (t.i < s.N) {
+    for (int j = t.j0; j < t.j1; ++j) {
+      const long long off = (long long)j * s.pitch + t.i;
+      PolyNbhd<double, NC> nb;
+      double um[NC], vm[NC], uym[NC], vym[NC];
+      gather5<T, NC>(s, X, t.i, j, nb, um, vm, uym, vym);
+      CompConst<double> k[NC];
+      load_comp_const<double, NC>(s, off, k);
+      const uint16_t nnv = s.nn[off];
+      const double e = site_energy<NC>(nb, k, s.eta, s.gamma, s.delta, idx, dx2, nnv & 0xff,
+                                       nnv >> 8, s.nsf);
+      two_sum_acc(acc_s[0], acc_c[0], e);
+    }
   }
-  double fin[1];
-  if (finish_image<1, true>(acc_s, acc_c, lds, s.part_p, s.count_p, tc.n, tc.tile,
-                            s.tiles_x * s.tiles_y, fin)) {
-    if (threadIdx.x == 0) a.out[s.img0 + tc.n] = fin[0];
-  }
+  publish_partial<1, true>(acc_s, acc_c, s.part_p, t.n, t.tile, s.tiles_x * s.tiles_y);
 }
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_energy(AuxArgs a) {
-  __shared__ double lds[kWavesPerBlock * 2 * kPartP];
   const WorkItem it = a.items[xcd_item(blockIdx.x, a.nitems)];
   const DevString& s = a.strings[it.string];
   if (s.nc == 2)
-    energy_body<T, 2>(a, s, it, lds);
+    energy_body<T, 2>(a, s, it);
   else
-    energy_body<T, 1>(a, s, it, lds);
+    energy_body<T, 1>(a, s, it);
 }
 
 // computeJB (nlcg.cu:243-380) into planes b, j_1, j_2 of a double buffer
 // laid out [image][plane][row][pitch] with 1 + 2*nc planes.
 template <class T, int NC>
 __device__ void jb_body(const AuxArgs& a, const DevString& s, const WorkItem& it) {
-  const TileCtx tc = tile_of(s, it);
-  const T* X = (const T*)s.x[a.xin] + (long long)tc.n * s.image_stride;
-  double* O = a.jb[it.string] + (long long)tc.n * (1 + 2 * NC) * s.plane_stride;
-  for (int r = 0; r < tc.nrows; ++r) {
-    const int i = tc.i, j = tc.j0 + r * tc.rstride;
-    if (i >= s.N || j >= s.N) continue;
-    const long long off = (long long)j * s.pitch + i;
+  const Strip t = strip_of(s, it);
+  if (t.i >= s.N) return;
+  const T* X = (const T*)s.x[a.xin] + (long long)t.n * s.image_stride;
+  double* O = a.jb[it.string] + (long long)t.n * (1 + 2 * NC) * s.plane_stride;
+  for (int j = t.j0; j < t.j1; ++j) {
+    const long long off = (long long)j * s.pitch + t.i;
     PolyNbhd<double, NC> nb;
     double um[NC], vm[NC], uym[NC], vym[NC];
-    gather5<T, NC>(s, X, i, j, nb, um, vm, uym, vym);
+    gather5<T, NC>(s, X, t.i, j, nb, um, vm, uym, vym);
     CompConst<double> k[NC];
     load_comp_const<double, NC>(s, off, k);
     const uint16_t nnv = s.nn[off];
@@ -606,22 +838,20 @@ __global__ __launch_bounds__(kBlock) void k_jb(AuxArgs a) {
 // Arclength partials (arcLengthParametrization + arcLengthParametrizationJB,
 // interpolation.cu:39-92): squared distance of image n to image n-1 in the
 // plain metric and in the gauge-invariant (b, j) metric, with b and j
-// recomputed on the fly for both images.  Images 0 contribute nothing.
+// recomputed on the fly for both images.  Image 0 contributes nothing.
 template <class T, int NC>
-__device__ void arclen_body(const AuxArgs& a, const DevString& s, const WorkItem& it,
-                            double* lds) {
-  const TileCtx tc = tile_of(s, it);
+__device__ void arclen_body(const AuxArgs& a, const DevString& s, const WorkItem& it) {
+  const Strip t = strip_of(s, it);
   double acc_s[2] = {0, 0}, acc_c[2] = {0, 0};
-  if (tc.n > 0) {
-    const T* X1 = (const T*)s.x[a.xin] + (long long)tc.n * s.image_stride;
+  if (t.n > 0 && t.i < s.N) {
+    const T* X1 = (const T*)s.x[a.xin] + (long long)t.n * s.image_stride;
     const T* X0 = X1 - s.image_stride;
-    for (int r = 0; r < tc.nrows; ++r) {
-      const int i = tc.i, j = tc.j0 + r * tc.rstride;
-      if (i >= s.N || j >= s.N) continue;
-      const long long off = (long long)j * s.pitch + i;
+    for (int j = t.j0; j < t.j1; ++j) {
+      const long long off = (long long)j * s.pitch + t.i;
       double d2 = 0;
       for (int p = 0; p < 2 + 2 * NC; ++p) {
-        const double dv = (double)X1[p * s.plane_stride + off] - (double)X0[p * s.plane_stride + off];
+        const double dv =
+            (double)X1[p * s.plane_stride + off] - (double)X0[p * s.plane_stride + off];
         d2 += dv * dv;
       }
       CompConst<double> k[NC];
@@ -632,7 +862,7 @@ __device__ void arclen_body(const AuxArgs& a, const DevString& s, const WorkItem
       for (int m = 0; m < 2; ++m) {
         PolyNbhd<double, NC> nb;
         double um[NC], vm[NC], uym[NC], vym[NC];
-        gather5<T, NC>(s, m ? X1 : X0, i, j, nb, um, vm, uym, vym);
+        gather5<T, NC>(s, m ? X1 : X0, t.i, j, nb, um, vm, uym, vym);
         site_jb<NC>(nb, um, vm, uym, vym, k, s.inv_dx, nnv & 0xff, nnv >> 8, b[m], jj[m]);
       }
       double djb = sq(b[1] - b[0]);
@@ -643,26 +873,17 @@ __device__ void arclen_body(const AuxArgs& a, const DevString& s, const WorkItem
       two_sum_acc(acc_s[1], acc_c[1], djb);
     }
   }
-  double fin[2];
-  if (finish_image<2, true>(acc_s, acc_c, lds, s.part_a, s.count_g, tc.n, tc.tile,
-                            s.tiles_x * s.tiles_y, fin)) {
-    if (threadIdx.x == 0) {
-      ImageScalars& sc = a.sc[s.img0 + tc.n];
-      sc.ds2 = fin[0];
-      sc.ds2_jb = fin[1];
-    }
-  }
+  publish_partial<2, true>(acc_s, acc_c, s.part_a, t.n, t.tile, s.tiles_x * s.tiles_y);
 }
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_arclen(AuxArgs a) {
-  __shared__ double lds[kWavesPerBlock * 2 * kPartP];
   const WorkItem it = a.items[xcd_item(blockIdx.x, a.nitems)];
   const DevString& s = a.strings[it.string];
   if (s.nc == 2)
-    arclen_body<T, 2>(a, s, it, lds);
+    arclen_body<T, 2>(a, s, it);
   else
-    arclen_body<T, 1>(a, s, it, lds);
+    arclen_body<T, 1>(a, s, it);
 }
 
 // computeArcLength (interpolation.cu:126-140) for both metrics, then the
@@ -677,6 +898,7 @@ __global__ void k_arc_finalize(AuxArgs a, int nstrings) {
   const DevString& s = a.strings[si];
   ImageScalars* sc = a.sc + s.img0;
   const int F = s.F;
+  if (F < 2) return;
   sc[0].s = 0;
   sc[0].s_jb = 0;
   for (int n = 1; n < F; ++n) {
@@ -704,22 +926,22 @@ __global__ void k_arc_finalize(AuxArgs a, int nstrings) {
   }
   // Cubic plan and spline factors (need F >= 4).
   if (F >= 4) {
-    double* cb = s.cubic;  // [F][4]: c', m, dl, h
+    double* cb = s.cubic;  // [F][4]: c', m, sub-diagonal, h
     for (int n = 0; n < F; ++n) cb[4 * n + 3] = n == 0 ? 0.0 : sc[n].s_jb - sc[n - 1].s_jb;
     auto h = [&](int n) { return cb[4 * n + 3]; };
-    double dl[2], d, du;
     for (int n = 0; n < F; ++n) {
+      double dl, d, du;
       if (n == 0) {
-        dl[0] = 0; d = -h(1); du = h(1);
+        dl = 0; d = -h(1); du = h(1);
       } else if (n == F - 1) {
-        dl[0] = h(F - 1); d = -h(F - 1); du = 0;
+        dl = h(F - 1); d = -h(F - 1); du = 0;
       } else {
-        dl[0] = h(n); d = 2 * (h(n) + h(n + 1)); du = h(n + 1);
+        dl = h(n); d = 2 * (h(n) + h(n + 1)); du = h(n + 1);
       }
-      const double m = n == 0 ? d : d - dl[0] * cb[4 * (n - 1) + 0];
+      const double m = n == 0 ? d : d - dl * cb[4 * (n - 1) + 0];
       cb[4 * n + 0] = du / m;
       cb[4 * n + 1] = m;
-      cb[4 * n + 2] = dl[0];
+      cb[4 * n + 2] = dl;
     }
     int k = 1;
     double seg_start = 0, seg_end = h(1);
@@ -742,20 +964,18 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_redistribute_linear(AuxArgs a) {
   const WorkItem it = a.items[xcd_item(blockIdx.x, a.nitems)];
   const DevString& s = a.strings[it.string];
-  const TileCtx tc = tile_of(s, it);
-  const int k = s.plan_k[tc.n];
-  const double w = s.plan_w[tc.n];
+  const Strip t = strip_of(s, it);
+  if (t.i >= s.N) return;
+  const int k = s.plan_k[t.n];
+  const double w = s.plan_w[t.n];
   const T* Xk = (const T*)s.x[a.xin] + (long long)k * s.image_stride;
   const T* Xk1 = Xk - s.image_stride;
-  T* O = (T*)s.x[a.xout] + (long long)tc.n * s.image_stride;
-  for (int r = 0; r < tc.nrows; ++r) {
-    const int i = tc.i, j = tc.j0 + r * tc.rstride;
-    if (i >= s.N || j >= s.N) continue;
-    const long long off = (long long)j * s.pitch + i;
+  T* O = (T*)s.x[a.xout] + (long long)t.n * s.image_stride;
+  for (int j = t.j0; j < t.j1; ++j) {
+    const long long off = (long long)j * s.pitch + t.i;
     for (int p = 0; p < s.nplanes; ++p) {
       const long long o = p * s.plane_stride + off;
-      const double v = fma(w, (double)Xk[o], (1 - w) * (double)Xk1[o]);
-      O[o] = (T)v;
+      O[o] = (T)fma(w, (double)Xk[o], (1 - w) * (double)Xk1[o]);
     }
   }
 }
@@ -767,20 +987,19 @@ __global__ __launch_bounds__(kBlock) void k_copy_a(AuxArgs a) {
   const WorkItem it = a.items[xcd_item(blockIdx.x, a.nitems)];
   const DevString& s = a.strings[it.string];
   if (!s.nsf) return;
-  const TileCtx tc = tile_of(s, it);
-  const T* S = (const T*)s.x[a.xout] + (long long)tc.n * s.image_stride;
-  T* O = (T*)s.x[a.xin] + (long long)tc.n * s.image_stride;
-  for (int r = 0; r < tc.nrows; ++r) {
-    const int i = tc.i, j = tc.j0 + r * tc.rstride;
-    if (i >= s.N || j >= s.N) continue;
-    const long long off = (long long)j * s.pitch + i;
+  const Strip t = strip_of(s, it);
+  if (t.i >= s.N) return;
+  const T* S = (const T*)s.x[a.xout] + (long long)t.n * s.image_stride;
+  T* O = (T*)s.x[a.xin] + (long long)t.n * s.image_stride;
+  for (int j = t.j0; j < t.j1; ++j) {
+    const long long off = (long long)j * s.pitch + t.i;
     O[off] = S[off];
     O[s.plane_stride + off] = S[s.plane_stride + off];
   }
 }
 
 // redistributeCubic (interpolation.cu:292-481) in double, one thread per
-// site and plane: divided differences and end third differences
+// site: for each plane, divided differences and end third differences
 // (computeDeltaX/Y), the Thomas solve of the spline system along the image
 // axis (in place of cusparseSgtsv2), and the spline evaluation
 // (generateNewFramesCubic).  textbook = 0 keeps the reference's
@@ -789,18 +1008,16 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_redistribute_cubic(AuxArgs a, int textbook) {
   const WorkItem it = a.items[xcd_item(blockIdx.x, a.nitems)];
   const DevString& s = a.strings[it.string];
-  const TileCtx tc = tile_of(s, it);
+  const Strip t = strip_of(s, it);
   // The solve runs along the image axis: only the image-0 items of the
   // shared work list take part, one thread per site.
-  if (tc.n != 0) return;
+  if (t.n != 0 || t.i >= s.N) return;
   const int F = s.F;
   const double* cb = s.cubic;
   const long long cells = s.plane_stride;
   double* Dl = (double*)s.cubic_rhs;  // [F][nplanes][cells]
-  for (int r = 0; r < tc.nrows; ++r) {
-    const int i = tc.i, j = tc.j0 + r * tc.rstride;
-    if (i >= s.N || j >= s.N) continue;
-    const long long off = (long long)j * s.pitch + i;
+  for (int j = t.j0; j < t.j1; ++j) {
+    const long long off = (long long)j * s.pitch + t.i;
     for (int p = 0; p < s.nplanes; ++p) {
       const T* Xp = (const T*)s.x[a.xin] + p * s.plane_stride + off;
       auto f = [&](int n) { return (double)Xp[(long long)n * s.image_stride]; };
@@ -872,27 +1089,37 @@ __global__ void k_reset_norms(AuxArgs a, int nstrings, int interior_only) {
 }
 
 template <class T>
-__global__ __launch_bounds__(kBlock) void k_fill(T* p, long long n, T v) {
+__global__ __launch_bounds__(256) void k_fill(T* p, long long n, T v) {
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < n; e += stride) p[e] = v;
 }
 
 // ---------------------------------------------------------------------------
 // Host launchers.
-static inline unsigned grid_of(int nitems) { return (unsigned)nitems; }
+template <class T, int NC, bool NSF>
+static void launch_gp(const StepArgs& a, int which, const dim3& g, const dim3& b, hipStream_t st) {
+  if (which == kKernelPoly)
+    hipLaunchKernelGGL((k_poly<T, NC, NSF>), g, b, 0, st, a);
+  else if (a.upd)
+    hipLaunchKernelGGL((k_grad<T, NC, true, NSF>), g, b, 0, st, a);
+  else
+    hipLaunchKernelGGL((k_grad<T, NC, false, NSF>), g, b, 0, st, a);
+}
 
 template <class T>
 static hipError_t launch_step_impl(const StepArgs& a, int which, hipStream_t st) {
   if (a.nitems <= 0) return hipSuccess;
+  const dim3 g((unsigned)a.nitems), b(kBlock);
   switch (which) {
     case kKernelGrad:
-      hipLaunchKernelGGL(k_grad<T>, dim3(grid_of(a.nitems)), dim3(kBlock), 0, st, a);
-      break;
     case kKernelPoly:
-      hipLaunchKernelGGL(k_poly<T>, dim3(grid_of(a.nitems)), dim3(kBlock), 0, st, a);
+      if (a.nc == 2)
+        a.nsf ? launch_gp<T, 2, true>(a, which, g, b, st) : launch_gp<T, 2, false>(a, which, g, b, st);
+      else
+        a.nsf ? launch_gp<T, 1, true>(a, which, g, b, st) : launch_gp<T, 1, false>(a, which, g, b, st);
       break;
     case kKernelUpdate:
-      hipLaunchKernelGGL(k_update<T>, dim3(grid_of(a.nitems)), dim3(kBlock), 0, st, a);
+      hipLaunchKernelGGL(k_update<T>, g, b, 0, st, a);
       break;
     default:
       return hipErrorInvalidValue;
@@ -906,7 +1133,7 @@ hipError_t launch_step(const StepArgs& a, int which, bool fp32, hipStream_t st) 
 
 template <class T>
 static hipError_t launch_aux_impl(const AuxArgs& a, int which, int arg, hipStream_t st) {
-  const dim3 g(grid_of(a.nitems)), b(kBlock);
+  const dim3 g((unsigned)a.nitems), b(kBlock);
   switch (which) {
     case kKernelEnergy:
       if (a.nitems > 0) hipLaunchKernelGGL(k_energy<T>, g, b, 0, st, a);
@@ -943,6 +1170,34 @@ hipError_t launch_aux(const AuxArgs& a, int which, int arg, bool fp32, hipStream
               : launch_aux_impl<double>(a, which, arg, st);
 }
 
+hipError_t launch_finish(const FinishArgs& f, int which, bool fp32, hipStream_t st) {
+  if (f.nimages <= 0) return hipSuccess;
+  const dim3 g((unsigned)f.nimages), b(kBlock);
+  switch (which) {
+    case kKernelGrad:
+      if (fp32)
+        hipLaunchKernelGGL(k_finish_grad<true>, g, b, 0, st, f);
+      else
+        hipLaunchKernelGGL(k_finish_grad<false>, g, b, 0, st, f);
+      break;
+    case kKernelPoly:
+      if (fp32)
+        hipLaunchKernelGGL(k_finish_poly<true>, g, b, 0, st, f);
+      else
+        hipLaunchKernelGGL(k_finish_poly<false>, g, b, 0, st, f);
+      break;
+    case kKernelEnergy:
+      hipLaunchKernelGGL(k_finish_energy, g, b, 0, st, f);
+      break;
+    case kKernelArclen:
+      hipLaunchKernelGGL(k_finish_arclen, g, b, 0, st, f);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
 hipError_t launch_nn_map(const int* sc_dom, const int* comp_dom, int N, int pitch, uint16_t* nn,
                          hipStream_t st) {
   hipLaunchKernelGGL(k_nn_map, dim3((N + 63) / 64, N), dim3(64), 0, st, sc_dom, comp_dom, N,
@@ -951,19 +1206,19 @@ hipError_t launch_nn_map(const int* sc_dom, const int* comp_dom, int N, int pitc
 }
 
 hipError_t launch_fill(void* p, long long n, bool fp32, double v, hipStream_t st) {
-  const unsigned blocks = (unsigned)std::min<long long>((n + kBlock - 1) / kBlock, 4096);
   if (n <= 0) return hipSuccess;
+  const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 4096);
   if (fp32)
-    hipLaunchKernelGGL(k_fill<float>, dim3(blocks), dim3(kBlock), 0, st, (float*)p, n, (float)v);
+    hipLaunchKernelGGL(k_fill<float>, dim3(blocks), dim3(256), 0, st, (float*)p, n, (float)v);
   else
-    hipLaunchKernelGGL(k_fill<double>, dim3(blocks), dim3(kBlock), 0, st, (double*)p, n, v);
+    hipLaunchKernelGGL(k_fill<double>, dim3(blocks), dim3(256), 0, st, (double*)p, n, v);
   return hipGetLastError();
 }
 
 hipError_t launch_fill_u32(unsigned* p, long long n, hipStream_t st) {
   if (n <= 0) return hipSuccess;
-  const unsigned blocks = (unsigned)std::min<long long>((n + kBlock - 1) / kBlock, 4096);
-  hipLaunchKernelGGL(k_fill<unsigned>, dim3(blocks), dim3(kBlock), 0, st, p, n, 0u);
+  const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_fill<unsigned>, dim3(blocks), dim3(256), 0, st, p, n, 0u);
   return hipGetLastError();
 }
 

@@ -37,6 +37,7 @@ struct StepArgs {
   int maxwell;      // Maxwell mode: psi gradient not recomputed
   int nlcg;         // Polak-Ribiere beta (1) or steepest descent (0)
   int scratch;      // G writes the diagnostic gradient buffer only
+  int nc, nsf;      // class of every item in `items` (components, fixed A)
 };
 
 // Arguments of the auxiliary launches (energy, JB, arclength,
@@ -52,7 +53,19 @@ struct AuxArgs {
   double* const* jb;    // per string JB planes (device array of pointers)
 };
 
+// Arguments of the per-image finishing launches (one wavefront per image).
+struct FinishArgs {
+  const DevString* strings;
+  const int2* img;   // global image id -> (string, image)
+  int nimages;
+  ImageScalars* sc;
+  const LineSearchTable* ls;
+  double* out;
+  int scur, sold, nlcg;
+};
+
 hipError_t launch_step(const StepArgs& a, int which, bool fp32, hipStream_t st);
+hipError_t launch_finish(const FinishArgs& f, int which, bool fp32, hipStream_t st);
 hipError_t launch_aux(const AuxArgs& a, int which, int arg, bool fp32, hipStream_t st);
 hipError_t launch_nn_map(const int* sc_dom, const int* comp_dom, int N, int pitch, uint16_t* nn,
                          hipStream_t st);

@@ -41,6 +41,19 @@ __device__ __forceinline__ double param_at(const DevParam& p, long long off) {
 template <class T, int NC>
 __device__ __forceinline__ void load_comp_const(const DevString& s, long long off,
                                                 CompConst<T> (&k)[NC]) {
+  if (!s.planes) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      k[c].q = (T)s.kc[c][0];
+      k[c].two_a = (T)s.kc[c][1];
+      k[c].two_b = (T)s.kc[c][2];
+      k[c].b = (T)s.kc[c][3];
+      k[c].ab = (T)s.kc[c][4];
+      k[c].hx = (T)s.kc[c][5];
+      k[c].hy = (T)s.kc[c][6];
+    }
+    return;
+  }
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const double a = param_at<T>(s.a[c], off), b = param_at<T>(s.b[c], off);
@@ -112,27 +125,36 @@ __device__ __forceinline__ void site_gradient(const Nbhd<T, NC>& s, const CompCo
   const T AX0 = s.ax[1][1], AY0 = s.ay[1][1];
   T gax = 0, gay = 0;
 
+  const bool bx0 = bitof(sc, 0), bx4 = bitof(sc, 4), by2 = bitof(sc, 2), by6 = bitof(sc, 6);
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const T u0 = s.u[c][0], v0 = s.v[c][0];
     const T n2 = u0 * u0 + v0 * v0;
     T gu = (k[c].two_a * u0 + k[c].two_b * u0 * n2) * af;
     T gv = (k[c].two_a * v0 + k[c].two_b * v0 * n2) * af;
-    T dummy = 0;
-    T& gAx = do_a ? gax : dummy;
-    if (bitof(sc, 0))
-      bond_fwd(u0, v0, AX0, s.u[c][1], s.v[c][1], s.ax[1][2], k[c].q, k[c].hx, wxf, idx, gu, gv,
-               gax);
-    if (bitof(sc, 4))
-      bond_bwd(u0, v0, AX0, s.u[c][2], s.v[c][2], s.ax[1][0], k[c].q, k[c].hx, wxb, idx, gu, gv,
-               gax);
-    if (bitof(sc, 2))
-      bond_fwd(u0, v0, AY0, s.u[c][3], s.v[c][3], s.ay[2][1], k[c].q, k[c].hy, wyf, idx, gu, gv,
-               gay);
-    if (bitof(sc, 6))
-      bond_bwd(u0, v0, AY0, s.u[c][4], s.v[c][4], s.ay[0][1], k[c].q, k[c].hy, wyb, idx, gu, gv,
-               gay);
-    (void)gAx;
+    // Each bond contributes only when its neighbour exists; the terms are
+    // always evaluated and selected, so no lane diverges.
+    T bu, bv, bA;
+    bu = bv = bA = 0;
+    bond_fwd(u0, v0, AX0, s.u[c][1], s.v[c][1], s.ax[1][2], k[c].q, k[c].hx, wxf, idx, bu, bv, bA);
+    gu += bx0 ? bu : (T)0;
+    gv += bx0 ? bv : (T)0;
+    gax += bx0 ? bA : (T)0;
+    bu = bv = bA = 0;
+    bond_bwd(u0, v0, AX0, s.u[c][2], s.v[c][2], s.ax[1][0], k[c].q, k[c].hx, wxb, idx, bu, bv, bA);
+    gu += bx4 ? bu : (T)0;
+    gv += bx4 ? bv : (T)0;
+    gax += bx4 ? bA : (T)0;
+    bu = bv = bA = 0;
+    bond_fwd(u0, v0, AY0, s.u[c][3], s.v[c][3], s.ay[2][1], k[c].q, k[c].hy, wyf, idx, bu, bv, bA);
+    gu += by2 ? bu : (T)0;
+    gv += by2 ? bv : (T)0;
+    gay += by2 ? bA : (T)0;
+    bu = bv = bA = 0;
+    bond_bwd(u0, v0, AY0, s.u[c][4], s.v[c][4], s.ay[0][1], k[c].q, k[c].hy, wyb, idx, bu, bv, bA);
+    gu += by6 ? bu : (T)0;
+    gv += by6 ? bv : (T)0;
+    gay += by6 ? bA : (T)0;
     gpsi[c][0] = gu;
     gpsi[c][1] = gv;
   }
@@ -164,33 +186,37 @@ __device__ __forceinline__ void site_gradient(const Nbhd<T, NC>& s, const CompCo
     const T dxf = (X[2][1] - X[1][1]) * idx;  // forward y-difference of A_x
     const T dxb = (X[1][1] - X[0][1]) * idx;
     const T h0 = s.h[0];
-    if (bitof(cm, 0) && bitof(cm, 1) && bitof(cm, 2)) {
+    const bool q0 = bitof(cm, 0) && bitof(cm, 1) && bitof(cm, 2);
+    const bool q1 = bitof(cm, 2) && bitof(cm, 3) && bitof(cm, 4);
+    const bool q2 = bitof(cm, 4) && bitof(cm, 5) && bitof(cm, 6);
+    const bool q3 = bitof(cm, 6) && bitof(cm, 7) && bitof(cm, 0);
+    {
       const T e = dyf - dxf - h0;
       const T n1 = dyf - (X[2][2] - X[1][2]) * idx - s.h[1];
       const T n2 = (Y[2][2] - Y[2][1]) * idx - dxf - s.h[3];
-      gax += c4 * e + c4 * n2;
-      gay += -c4 * e - c4 * n1;
+      gax += q0 ? c4 * e + c4 * n2 : (T)0;
+      gay += q0 ? -c4 * e - c4 * n1 : (T)0;
     }
-    if (bitof(cm, 2) && bitof(cm, 3) && bitof(cm, 4)) {
+    {
       const T e = dyb - dxf - h0;
       const T n1 = (Y[2][1] - Y[2][0]) * idx - dxf - s.h[3];
       const T n2 = dyb - (X[2][0] - X[1][0]) * idx - s.h[2];
-      gax += c4 * e + c4 * n1;
-      gay += c4 * e + c4 * n2;
+      gax += q1 ? c4 * e + c4 * n1 : (T)0;
+      gay += q1 ? c4 * e + c4 * n2 : (T)0;
     }
-    if (bitof(cm, 4) && bitof(cm, 5) && bitof(cm, 6)) {
+    {
       const T e = dyb - dxb - h0;
       const T n1 = dyb - (X[1][0] - X[0][0]) * idx - s.h[2];
       const T n2 = (Y[0][1] - Y[0][0]) * idx - dxb - s.h[4];
-      gax += -c4 * e - c4 * n2;
-      gay += c4 * e + c4 * n1;
+      gax += q2 ? -c4 * e - c4 * n2 : (T)0;
+      gay += q2 ? c4 * e + c4 * n1 : (T)0;
     }
-    if (bitof(cm, 6) && bitof(cm, 7) && bitof(cm, 0)) {
+    {
       const T e = dyf - dxb - h0;
       const T n1 = (Y[0][2] - Y[0][1]) * idx - dxb - s.h[4];
       const T n2 = dyf - (X[1][2] - X[0][2]) * idx - s.h[1];
-      gax += -c4 * e - c4 * n1;
-      gay += -c4 * e - c4 * n2;
+      gax += q3 ? -c4 * e - c4 * n1 : (T)0;
+      gay += q3 ? -c4 * e - c4 * n2 : (T)0;
     }
     ga[0] = gax * dx2;
     ga[1] = gay * dx2;
@@ -321,29 +347,31 @@ __device__ __forceinline__ Quartic<T> site_poly(const PolyNbhd<T, NC>& s,
     e0[3] = yf - xb - s.h0;  e1[3] = dyf - dxb;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      if (on[t]) {
-        r.c[0] += (T)0.5 * e0[t] * e0[t] * f;
-        r.c[1] += e0[t] * e1[t] * f;
-        r.c[2] += (T)0.5 * e1[t] * e1[t] * f;
-      }
+      r.c[0] += on[t] ? (T)0.5 * e0[t] * e0[t] * f : (T)0;
+      r.c[1] += on[t] ? e0[t] * e1[t] * f : (T)0;
+      r.c[2] += on[t] ? (T)0.5 * e1[t] * e1[t] * f : (T)0;
     }
   }
   // Kinetic bonds, nlcg.cu:1160-1366.
   const T cxf = (T)(bitof(sc, 1) * bitof(sc, 2) + bitof(sc, 6) * bitof(sc, 7)) * (T)0.5;
   const T cyf = (T)(bitof(sc, 0) * bitof(sc, 1) + bitof(sc, 3) * bitof(sc, 4)) * (T)0.5;
   const T c2y = exact ? (T)1 : (T)0.5;
+  const bool bx = bitof(sc, 0), by = bitof(sc, 2);
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const T q = k[c].q;
     const T c1x = exact ? -q : (T)1;
-    if (bitof(sc, 0))
-      poly_bond(s.u[c][0], s.v[c][0], s.u[c][1], s.v[c][1], s.du[c][0], s.dv[c][0], s.du[c][1],
-                s.dv[c][1], s.ax[0], s.ax[1], s.dax[0], s.dax[1], q, c1x, (T)1,
-                k[c].hx * cxf * dx2, idx, r);
-    if (bitof(sc, 2))
-      poly_bond(s.u[c][0], s.v[c][0], s.u[c][2], s.v[c][2], s.du[c][0], s.dv[c][0], s.du[c][2],
-                s.dv[c][2], s.ay[0], s.ay[3], s.day[0], s.day[3], q, -q, c2y,
-                k[c].hy * cyf * dx2, idx, r);
+    Quartic<T> rx, ry;
+#pragma unroll
+    for (int t = 0; t < 5; ++t) rx.c[t] = ry.c[t] = 0;
+    poly_bond(s.u[c][0], s.v[c][0], s.u[c][1], s.v[c][1], s.du[c][0], s.dv[c][0], s.du[c][1],
+              s.dv[c][1], s.ax[0], s.ax[1], s.dax[0], s.dax[1], q, c1x, (T)1,
+              k[c].hx * cxf * dx2, idx, rx);
+    poly_bond(s.u[c][0], s.v[c][0], s.u[c][2], s.v[c][2], s.du[c][0], s.dv[c][0], s.du[c][2],
+              s.dv[c][2], s.ay[0], s.ay[3], s.day[0], s.day[3], q, -q, c2y,
+              k[c].hy * cyf * dx2, idx, ry);
+#pragma unroll
+    for (int t = 0; t < 5; ++t) r.c[t] += (bx ? rx.c[t] : (T)0) + (by ? ry.c[t] : (T)0);
   }
   return r;
 }

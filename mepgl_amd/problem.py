@@ -56,8 +56,8 @@ class GLString:
 
     def validate(self) -> "GLString":
         N, F = int(self.N), int(self.F)
-        if N < 2 or F < 2:
-            raise ValueError(f"need N >= 2 and F >= 2 (got N={N}, F={F})")
+        if N < 2 or F < 1:
+            raise ValueError(f"need N >= 2 and F >= 1 (got N={N}, F={F})")
         for nm in ("comp_domain", "sc_domain"):
             d = np.asarray(getattr(self, nm))
             if d.shape != (N, N):

@@ -34,7 +34,7 @@ class Solver:
 
     def __init__(self, strings: Sequence[GLString], precision: str = "fp64",
                  poly_exact: bool = False, cubic_textbook: bool = False,
-                 use_graph: bool = True, device: int = 0, rows_per_thread: int = 0):
+                 use_graph: bool = True, device: int = 0, segment_rows: int = 0):
         self._L = _lib.load()
         self.strings: List[GLString] = [s.validate() for s in strings]
         self.precision = precision
@@ -45,7 +45,7 @@ class Solver:
         self._keep = []
         descs = (_lib.StringDesc * len(self.strings))()
         for d, s in zip(descs, self.strings):
-            self._fill_desc(d, s, rows_per_thread)
+            self._fill_desc(d, s, segment_rows)
         self._ctx = C.c_void_p()
         rc = self._L.mepgl_create(descs, len(self.strings), prec, flags, device,
                                   C.byref(self._ctx))
@@ -81,7 +81,7 @@ class Solver:
             arr = np.ascontiguousarray(getattr(s, name), dtype=np.int32)
             self._keep.append(arr)
             setattr(d, name, arr.ctypes.data_as(C.POINTER(C.c_int32)))
-        d.rows_per_thread = int(rows)
+        d.segment_rows = int(rows)
 
     def _check(self, rc):
         if rc != 0:

@@ -1,0 +1,254 @@
+"""Pinning the oracle's energy, gradient, JB and neighbour map
+(oracle/mepgl_oracle.c) with closed forms, finite differences and
+invariants that do not restate its formulas."""
+import itertools
+
+import numpy as np
+import pytest
+
+from conftest import small_string
+
+
+def full_box(N):
+    return np.ones((N, N), dtype=np.int32)
+
+
+def geometric_quadrants(dom):
+    """Number of the four quadrant plaquettes of each site whose three other
+    corners lie in the domain (site itself in the domain), by brute force."""
+    N = dom.shape[0]
+    out = np.zeros((N, N), dtype=np.int64)
+    for j in range(N):
+        for i in range(N):
+            if not dom[j, i]:
+                continue
+            for sx, sy in ((1, 1), (-1, 1), (-1, -1), (1, -1)):
+                corners = [(j, i + sx), (j + sy, i + sx), (j + sy, i)]
+                if all(0 <= a < N and 0 <= b < N and dom[a, b] for a, b in corners):
+                    out[j, i] += 1
+    return out
+
+
+def test_nn_map_all_neighbourhoods(oracle):
+    """Every 3x3 domain pattern (512) around the centre of a 5x5 grid, and
+    every pattern at a corner of the grid: bit k set iff neighbour k exists."""
+    offsets = [(1, 0), (1, 1), (0, 1), (-1, 1), (-1, 0), (-1, -1), (0, -1), (1, -1)]
+    for bits in range(512):
+        dom = np.zeros((5, 5), dtype=np.int32)
+        patch = [(bits >> k) & 1 for k in range(9)]
+        for k, (dy, dx) in enumerate(itertools.product((-1, 0, 1), (-1, 0, 1))):
+            dom[2 + dy, 2 + dx] = patch[k]
+        nn = oracle.build_nn_map(dom)
+        expect = 0
+        if dom[2, 2]:
+            for k, (ox, oy) in enumerate(offsets):
+                expect |= int(dom[2 + oy, 2 + ox] != 0) << k
+        assert nn[2, 2] == expect
+    dom = np.ones((3, 3), dtype=np.int32)
+    nn = oracle.build_nn_map(dom)
+    assert nn[0, 0] == (1 << 0) | (1 << 1) | (1 << 2)          # x right, y up
+    assert nn[2, 2] == (1 << 4) | (1 << 5) | (1 << 6)
+    assert nn[1, 1] == 0xFF
+
+
+def _uniform(N, F, u=0.0, v=0.0, ax=0.0, ay=0.0, **kw):
+    from mepgl_amd.problem import GLString
+
+    a = np.zeros((F, N, N, 2))
+    a[..., 0], a[..., 1] = ax, ay
+    p = np.zeros((F, N, N, 2))
+    p[..., 0], p[..., 1] = u, v
+    box = kw.pop("domain", full_box(N))
+    return GLString(N=N, dx=kw.pop("dx", 0.3), F=F, multicomponent=False, comp_domain=box,
+                    sc_domain=box.copy(), a=a, psi_1=p, **kw).validate()
+
+
+@pytest.mark.parametrize("masked", [False, True])
+def test_normal_state_closed_form(oracle, masked):
+    N, dx, H = 12, 0.3, 0.8
+    dom = full_box(N)
+    if masked:
+        dom[:3, :4] = 0
+        dom[8:, 9] = 0
+    s = _uniform(N, 2, a_1=-1.3, b_1=0.7, h=H, dx=dx, domain=dom)
+    nq = geometric_quadrants(dom)
+    expect = dx * dx * np.sum(nq / 4.0 * (1.3 ** 2 / (2 * 0.7)) + 0.125 * H * H * nq)
+    e = oracle.free_energy(s)
+    np.testing.assert_allclose(e, expect, rtol=1e-14)
+
+
+def test_meissner_state_is_zero(oracle):
+    s = _uniform(10, 2, u=np.sqrt(1.5 / 0.5), a_1=-1.5, b_1=0.5, h=0.0)
+    assert np.all(np.abs(oracle.free_energy(s)) < 1e-28)
+
+
+def test_uniform_field_gauge_has_no_magnetic_energy(oracle):
+    """A = (-H y/2, H x/2) has curl H in every plaquette, so with psi = 0
+    only the condensation term remains."""
+    from mepgl_amd import configs
+
+    N, dx, H = 14, 0.25, 1.1
+    s = _uniform(N, 1, a_1=-1.0, b_1=1.0, h=H, dx=dx)
+    s.a = configs.uniform_gauge(N, dx, 1, H)
+    nq = geometric_quadrants(full_box(N))
+    expect = dx * dx * np.sum(nq / 4.0 * 0.5)
+    np.testing.assert_allclose(oracle.free_energy(s), expect, rtol=1e-12)
+
+
+def test_constant_real_psi_with_constant_ax(oracle):
+    """psi = c real, A = (a0, 0): each complete x-bond carries
+    (1/4m)(2 q^2 a0^2 c^2), weighted by the two plaquettes beside it."""
+    N, dx, c, a0, q, m, H = 10, 0.2, 0.7, 0.3, 1.4, 1.7, 0.5
+    s = _uniform(N, 1, u=c, ax=a0, a_1=-1.0, b_1=1.0, m_xx_1=m, h=H, dx=dx, q_1=q)
+    dom = full_box(N)
+    nq = geometric_quadrants(dom)
+    kin = 0.0
+    for j in range(N):
+        for i in range(N - 1):
+            below = j > 0
+            above = j < N - 1
+            w = (int(above) + int(below)) / 2.0
+            kin += 0.25 / m * 2 * q * q * a0 * a0 * c * c * w
+    nl = np.sum(nq / 4.0 * 0.5 * (-1.0 + c * c) ** 2)
+    em = np.sum(0.125 * H * H * nq)
+    np.testing.assert_allclose(oracle.free_energy(s), dx * dx * (kin + nl + em), rtol=1e-13)
+
+
+def test_terms_are_additive(oracle):
+    s = small_string(N=14, F=2, multicomponent=True, seed=4)
+    tot = oracle.free_energy(s, oracle.TERM_ALL)
+    parts = sum(oracle.free_energy(s, t) for t in (oracle.TERM_NL, oracle.TERM_KIN, oracle.TERM_EM))
+    np.testing.assert_allclose(tot, parts, rtol=1e-13)
+
+
+def test_two_component_global_phase_and_relative_phase(oracle):
+    """Uniform two-component state, A = 0, h = 0: a common phase leaves the
+    energy unchanged; a relative phase of pi flips the Josephson term."""
+    s = small_string(N=10, F=1, multicomponent=True, seed=1)
+    s.a[:] = 0
+    s.h = 0.0
+    rng = np.random.default_rng(3)
+    z1 = 0.8 * np.exp(1j * 0.4)
+    z2 = 0.6 * np.exp(1j * 1.1)
+    for ph in (0.0, 0.9, 2.5):
+        t = s.copy()
+        w1, w2 = z1 * np.exp(1j * ph), z2 * np.exp(1j * ph)
+        t.psi_1[..., 0], t.psi_1[..., 1] = w1.real, w1.imag
+        t.psi_2[..., 0], t.psi_2[..., 1] = w2.real, w2.imag
+        e = oracle.free_energy(t)[0]
+        if ph == 0.0:
+            e0 = e
+        np.testing.assert_allclose(e, e0, rtol=1e-13)
+    t = s.copy()
+    w1, w2 = z1, -z2
+    t.psi_1[..., 0], t.psi_1[..., 1] = w1.real, w1.imag
+    t.psi_2[..., 0], t.psi_2[..., 1] = w2.real, w2.imag
+    e_pi = oracle.free_energy(t)[0]
+    t.eta = 0.0
+    e_noeta = oracle.free_energy(t)[0]
+    s0 = s.copy()
+    s0.psi_1[..., 0], s0.psi_1[..., 1] = z1.real, z1.imag
+    s0.psi_2[..., 0], s0.psi_2[..., 1] = z2.real, z2.imag
+    s0.eta = 0.0
+    np.testing.assert_allclose(e_noeta, oracle.free_energy(s0)[0], rtol=1e-13)
+    np.testing.assert_allclose(e_pi - e_noeta, -(e0 - e_noeta), rtol=1e-11)
+    del rng
+
+
+@pytest.mark.parametrize("case", [dict(multicomponent=False, planes=True),
+                                  dict(multicomponent=True),
+                                  dict(multicomponent=True, no_self_field=True)])
+def test_gradient_is_energy_derivative_in_the_interior(oracle, case):
+    """Central differences of the energy at sites two cells away from the
+    boundary, where the analytic gradient is the exact derivative."""
+    s = small_string(N=14, F=2, seed=21, **case)
+    # The reference's backward bonds use the own site's mass (nlcg.cu:551-571),
+    # so the gradient is the exact derivative only where m is locally uniform.
+    s.m_xx_1 = 1.3
+    grads = dict(zip(("a", "psi_1", "psi_2"), oracle.gradient(s)))
+    rng = np.random.default_rng(5)
+    fields = ["psi_1"] + (["psi_2"] if s.multicomponent else []) + \
+             ([] if s.no_self_field else ["a"])
+    eps = 1e-5
+    for f in fields:
+        for _ in range(6):
+            n = int(rng.integers(0, s.F))
+            j, i = (int(v) for v in rng.integers(2, s.N - 2, 2))
+            c = int(rng.integers(0, 2))
+            up, dn = s.copy(), s.copy()
+            getattr(up, f)[n, j, i, c] += eps
+            getattr(dn, f)[n, j, i, c] -= eps
+            fd = (oracle.free_energy(up)[n] - oracle.free_energy(dn)[n]) / (2 * eps)
+            g = grads[f][n, j, i, c]
+            assert abs(fd - g) <= 1e-7 * max(1.0, abs(g)), (f, n, j, i, c, fd, g)
+
+
+def test_gradient_vanishes_in_meissner_state(oracle):
+    s = _uniform(10, 1, u=1.0, a_1=-1.0, b_1=1.0, h=0.0)
+    ga, g1, _ = oracle.gradient(s)
+    assert np.max(np.abs(g1)) < 1e-15 and np.max(np.abs(ga)) < 1e-15
+
+
+def test_jb_closed_forms(oracle):
+    from mepgl_amd import configs
+
+    N, dx, H = 12, 0.25, 0.9
+    s = _uniform(N, 1, u=0.8, h=H, dx=dx)
+    s.a = configs.uniform_gauge(N, dx, 1, H)
+    b, j1, _ = oracle.compute_jb(s)
+    np.testing.assert_allclose(b[0], H, rtol=1e-13)       # every site, linear A
+    t = _uniform(N, 1, u=0.8, h=H, dx=dx)                 # A = 0, psi real constant
+    _, jt, _ = oracle.compute_jb(t)
+    assert np.max(np.abs(jt)) == 0.0
+
+
+def test_current_is_minus_kinetic_derivative(oracle):
+    """j_x(i) = -(1/dx^2) dF_kin/dA_x(i) at interior sites."""
+    s = small_string(N=14, F=1, seed=8, multicomponent=True)
+    _, j1, j2 = oracle.compute_jb(s)
+    rng = np.random.default_rng(2)
+    eps = 1e-5
+    for _ in range(6):
+        j, i = (int(v) for v in rng.integers(2, s.N - 2, 2))
+        c = int(rng.integers(0, 2))
+        up, dn = s.copy(), s.copy()
+        up.a[0, j, i, c] += eps
+        dn.a[0, j, i, c] -= eps
+        dF = (oracle.free_energy(up, oracle.TERM_KIN)[0] -
+              oracle.free_energy(dn, oracle.TERM_KIN)[0]) / (2 * eps)
+        jj = j1[0, j, i, c] + j2[0, j, i, c]
+        assert abs(-dF / s.dx ** 2 - jj) < 1e-6 * max(1, abs(jj))
+
+
+def test_current_invariant_under_constant_phase(oracle):
+    s = small_string(N=12, F=1, seed=9)
+    _, j_ref, _ = oracle.compute_jb(s)
+    t = s.copy()
+    z = (t.psi_1[..., 0] + 1j * t.psi_1[..., 1]) * np.exp(1j * 0.77)
+    t.psi_1[..., 0], t.psi_1[..., 1] = z.real, z.imag
+    _, j_rot, _ = oracle.compute_jb(t)
+    np.testing.assert_allclose(j_rot, j_ref, atol=1e-13 * np.max(np.abs(j_ref)))
+
+
+def test_gradient_uses_own_site_mass_on_backward_bonds(oracle):
+    """With a non-uniform m_xx the analytic gradient differs from the energy
+    derivative by the backward-bond mass: replacing m(i-1) by m(i) in the
+    derivative of the (i-1, i) bond accounts for the whole difference."""
+    s = small_string(N=12, F=1, seed=22, planes=True)
+    s.a[:] = 0.0
+    s.h = 0.0
+    _, g1, _ = oracle.gradient(s)
+    j, i, c = 6, 5, 0
+    eps = 1e-6
+    up, dn = s.copy(), s.copy()
+    up.psi_1[0, j, i, c] += eps
+    dn.psi_1[0, j, i, c] -= eps
+    fd = (oracle.free_energy(up)[0] - oracle.free_energy(dn)[0]) / (2 * eps)
+    # Derivative of the backward x-bond energy (owned by site i-1) with A = 0:
+    # 0.25/m(i-1) * 2 * 2 (u_i - u_{i-1}) / dx^2 * dx^2 = (u_i - u_{i-1}) / m(i-1);
+    # the analytic gradient uses m(i) in its place.
+    u = s.psi_1[0, j, :, 0]
+    m = s.m_xx_1[j]
+    du = u[i] - u[i - 1]
+    np.testing.assert_allclose(g1[0, j, i, c] - fd, du / m[i] - du / m[i - 1], rtol=1e-5,
+                               atol=1e-9)
