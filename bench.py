@@ -35,10 +35,14 @@ def barrier(e):
     return float(np.max(e - e[0])), float(np.max(e - e[-1]))
 
 
-def run_path(strings, precision, steps, warmup, profile_iters=3):
+KERNEL_NAMES = ((0, "grad_update"), (1, "poly"), (2, "grad_first"), (3, "finish_grad"),
+                (4, "finish_poly"))
+
+
+def run_path(strings, precision, steps, warmup, profile_iters=3, segment_rows=0):
     from mepgl_amd.solver import Solver
 
-    sv = Solver(strings, precision=precision)
+    sv = Solver(strings, precision=precision, segment_rows=segment_rows)
     iters = max(steps // R, 1)
     w_iters = max(warmup // R, 1)
     sv.loop(w_iters, R, "L")
@@ -60,9 +64,10 @@ def run_path(strings, precision, steps, warmup, profile_iters=3):
     sv.loop(profile_iters, R, "L")
     sv.synchronize()
     times = {}
-    for k, name in ((0, "grad_update"), (1, "poly_linesearch"), (2, "grad_first")):
+    for k, name in KERNEL_NAMES:
         ms, cnt = sv.profile_read(k)
         times[name] = (ms, cnt)
+        print(f"# {precision} {name}: {1e3 * ms:.2f} us x {cnt}", flush=True)
     sv.profile(False)
     info = sv.info(0)
     sv.close()
@@ -100,6 +105,8 @@ def main(argv=None):
     ap.add_argument("--workload", default="vortex_entry_256")
     ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", default="both", choices=("both", "fp64", "mixed"))
+    ap.add_argument("--segment-rows", type=int, default=0)
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: multi-GPU runs are not part of "
@@ -114,9 +121,17 @@ def main(argv=None):
     cf = C_f[0]
 
     res = {}
-    for prec in ("fp64", "mixed"):
-        res[prec] = run_path(strings, prec, args.steps, args.warmup)
+    precs = ("fp64", "mixed") if args.precision == "both" else (args.precision,)
+    for prec in precs:
+        res[prec] = run_path(strings, prec, args.steps, args.warmup,
+                             segment_rows=args.segment_rows)
         print(f"# {prec}: {res[prec]['steps']} steps in {res[prec]['seconds']:.4f} s", flush=True)
+    if args.precision != "both":
+        r = res[args.precision]
+        print(json.dumps({"precision": args.precision, "ms_per_step": 1e3 * r["seconds"] / r["steps"],
+                          "value": cells * r["steps"] / r["seconds"],
+                          "segment_rows": r["info"]["tile_rows"]}))
+        return 0
 
     mixed, fp64 = res["mixed"], res["fp64"]
     value = cells * mixed["steps"] / mixed["seconds"]
@@ -127,8 +142,8 @@ def main(argv=None):
     # P reads x, g, d and writes d: 4 C_f s.
     def roof(r, sbytes):
         g_ms, g_n = r["times"]["grad_update"]
-        p_ms, p_n = r["times"]["poly_linesearch"]
-        cand = [("grad_update", g_ms, g_n, 5 * cf * sbytes), ("poly_linesearch", p_ms, p_n, 4 * cf * sbytes)]
+        p_ms, p_n = r["times"]["poly"]
+        cand = [("grad_update", g_ms, g_n, 5 * cf * sbytes), ("poly", p_ms, p_n, 4 * cf * sbytes)]
         name, ms, n, per = max(cand, key=lambda t: t[1] * t[2])
         ach = cells * per / (ms * 1e-3) / 1e9
         return name, ms, ach, per

@@ -50,6 +50,9 @@ extern "C" {
                                    reference's (interpolation.cu:394-400)  */
 #define MEPGL_NO_GRAPH 4u       /* mepgl_loop launches eagerly instead of
                                    replaying a captured hipGraph           */
+#define MEPGL_GAUGE_ALIGN 8u    /* modes L and C align the constant phase of
+                                   neighbouring images before the arclength
+                                   (mepgl_gauge_align); off = reference    */
 
 /* A per-site parameter: a uniform value, or an N*N plane when plane != NULL. */
 typedef struct {
@@ -62,7 +65,7 @@ typedef struct {
  * parameter files loaded by loadInput (io.cu:10-126). */
 typedef struct {
   int N;                 /* grid points per side                          */
-  int F;                 /* images                                        */
+  int F;                 /* images, 1 .. 2048                             */
   int ncomp;             /* 1 or 2 order-parameter components             */
   int no_self_field;     /* 1: A fixed, magnetic term and dF/dA omitted   */
   double dx;             /* lattice spacing                               */
@@ -128,6 +131,15 @@ int mepgl_redistribute(mepgl_ctx *ctx, int cubic);
  * redistribution of one string to F_new images (reallocates its buffers). */
 int mepgl_resample(mepgl_ctx *ctx, int string, int F_new);
 
+/* Aligns the constant phase of neighbouring images: image n is rotated by
+ * Theta_n = sum_{m<=n} arg(sum conj(psi_m) psi_{m-1}), both components by
+ * the same angle, together with its direction and gradients.  No counterpart
+ * in the reference, which measures image distances with the gauge-invariant
+ * metric s_jb (mepgl.cu:31-32); the rotation is an exact symmetry of the
+ * discrete energy and never increases a neighbour distance.  Read the
+ * angles with MEPGL_SCALAR_THETA. */
+int mepgl_gauge_align(mepgl_ctx *ctx);
+
 /* Resets the Polak-Ribiere memory of the interior images
  * (mepgl.cu:231-234). */
 int mepgl_reset_interior_beta(mepgl_ctx *ctx);
@@ -151,6 +163,7 @@ int mepgl_loop(mepgl_ctx *ctx, int iterations, int relaxation_steps, char mode, 
 #define MEPGL_SCALAR_ENERGY 8    /* last mepgl_free_energy                  */
 #define MEPGL_SCALAR_G_NORM2_OLD 9
 #define MEPGL_SCALAR_LS_INDEX 10 /* scan index where the line search stopped */
+#define MEPGL_SCALAR_THETA 11    /* phase applied by the last alignment     */
 int mepgl_get_scalars(mepgl_ctx *ctx, int string, int which, double *out);
 
 /* Internal solver buffers of one string, for state checks:
@@ -167,8 +180,10 @@ int mepgl_get_control(mepgl_ctx *ctx, int key, int *value);
 
 /* Per-kernel timing with HIP events on the context's stream.  enable != 0
  * brackets every launch with events and disables graph replay; read returns
- * the average duration in ms and the launch count of kernel `kernel`
- * (0 = G: update + gradient, 1 = P: direction + quartic + line search). */
+ * the average duration in ms and the launch count of kernel `kernel`:
+ * 0 = G with the pending update, 1 = P (direction + quartic),
+ * 2 = G without an update, 3 = finishing launch after G (beta),
+ * 4 = finishing launch after P (quartic totals + line search). */
 int mepgl_profile(mepgl_ctx *ctx, int enable);
 int mepgl_profile_read(mepgl_ctx *ctx, int kernel, double *avg_ms, int *count);
 

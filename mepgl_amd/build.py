@@ -15,6 +15,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libmepgl.so")
 CLI = os.path.join(HERE, "mepgl")
+COPY_CALIB = os.path.join(HERE, "copy_calib")
 
 ARCH = "gfx950"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
@@ -51,6 +52,14 @@ def build_library(verbose=False) -> str:
     return LIB
 
 
+def build_copy_calib(verbose=False) -> str:
+    """Plain copy kernel used to calibrate rocprofv3 byte counters."""
+    _run([HIPCC, *HIP_FLAGS, os.path.join(CSRC, "copy_calib.hip"), "-o", COPY_CALIB])
+    if verbose:
+        print("built", COPY_CALIB)
+    return COPY_CALIB
+
+
 def build_cli(verbose=False) -> str:
     srcs = [os.path.join(CSRC, s) for s in CLI_SOURCES]
     _run([HIPCC, *HIP_FLAGS, "-x", "hip", *srcs, "-o", CLI, "-L", HERE, "-lmepgl",
@@ -65,6 +74,7 @@ def main(argv=None):
     ap.add_argument("--cli", action="store_true")
     args = ap.parse_args(argv)
     build_library(verbose=True)
+    build_copy_calib(verbose=True)
     if args.cli:
         build_cli(verbose=True)
 

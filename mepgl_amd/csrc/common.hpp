@@ -21,6 +21,8 @@ constexpr int kTileX = 64;         // strip width: one wavefront row segment
 constexpr int kBlock = kWave;      // one wavefront per workgroup / work item
 constexpr int kLsSteps = 2000;     // line-search scan length (nlcg.cu:1421)
 constexpr int kNumXcd = 8;         // MI355X accelerator complex dies
+constexpr int kMaxImages = 2048;   // images per string (reparametrisation in LDS)
+constexpr int kArcBlock = 256;     // threads of the per-string arclength pass
 
 enum Plane { PAX = 0, PAY = 1, PU1 = 2, PV1 = 3, PU2 = 4, PV2 = 5 };
 
@@ -39,8 +41,8 @@ struct ImageScalars {
   double poly[4];      // B, C, D, E
   double s, s_jb;      // arclength (plain metric, gauge-invariant metric)
   double ds2, ds2_jb;  // squared distance to the previous image
-  double lin_w;        // linear redistribution weight of this new image
-  int lin_k;           // linear redistribution segment end node
+  double align_re, align_im;  // sum of conj(psi_n) psi_{n-1} (alignment)
+  double theta;        // constant phase applied by the last alignment
   int ls_index;        // scan index at which the line search stopped
 };
 

@@ -166,6 +166,7 @@ AuxArgs aux_args(mepgl_ctx* c) {
   a.sc = c->d_sc;
   a.xin = c->xp;
   a.xout = 1 - c->xp;
+  a.dcur = c->dp;
   a.out = c->d_out;
   a.jb = c->d_jb;
   return a;
@@ -533,6 +534,7 @@ int arc_length_all(mepgl_ctx* c) {
   for (auto& s : c->strings)
     if (s.dev.F < 2)
       return fail(c, MEPGL_ERR_ARG, "arclength and reparametrisation need at least 2 images");
+  // k_arc_finalize reads the arclength partial totals of every image.
   int rc = flush(c);
   if (rc) return rc;
   AuxArgs a = aux_args(c);
@@ -574,10 +576,25 @@ int redistribute_all(mepgl_ctx* c, int cubic) {
   return MEPGL_OK;
 }
 
+// Constant-phase alignment of neighbouring images (optional; the reference
+// has no alignment step and relies on the gauge-invariant metric).
+int gauge_align_all(mepgl_ctx* c) {
+  for (auto& s : c->strings)
+    if (s.dev.F < 2) return fail(c, MEPGL_ERR_ARG, "alignment needs at least 2 images");
+  int rc = flush(c);
+  if (rc) return rc;
+  AuxArgs a = aux_args(c);
+  if ((rc = run_aux(c, a, kKernelAlign, 0))) return rc;
+  if ((rc = run_finish(c, finish_args(c), kKernelAlign))) return rc;
+  if ((rc = run_aux(c, a, kKernelAlignFinalize, (int)c->strings.size()))) return rc;
+  return run_aux(c, a, kKernelAlignApply, 0);
+}
+
 // One outer iteration of loop() (mepgl.cu:153-303).
 int outer_iteration(mepgl_ctx* c, int relax, char mode, int nlcg) {
   int rc;
   if (mode == 'L' || mode == 'C') {
+    if ((c->flags & MEPGL_GAUGE_ALIGN) && (rc = gauge_align_all(c))) return rc;
     if ((rc = arc_length_all(c))) return rc;
     if ((rc = redistribute_all(c, mode == 'C'))) return rc;
     AuxArgs a = aux_args(c);
@@ -618,7 +635,7 @@ int mepgl_create(const mepgl_string_desc* strings, int n_strings, int precision,
     const mepgl_string_desc& in = strings[si];
     const mepgl_param* ps[] = {&in.a_1, &in.b_1, &in.m_xx_1, &in.m_yy_1, &in.a_2,
                                &in.b_2, &in.m_xx_2, &in.m_yy_2, &in.h};
-    bool ok = in.N >= 2 && in.F >= 1 && (in.ncomp == 1 || in.ncomp == 2) && in.dx > 0 &&
+    bool ok = in.N >= 2 && in.F >= 1 && in.F <= kMaxImages && (in.ncomp == 1 || in.ncomp == 2) && in.dx > 0 &&
               std::isfinite(in.dx) && in.comp_domain && in.sc_domain;
     for (auto* p : ps) ok = ok && param_ok(*p);
     if (!ok) {
@@ -888,6 +905,12 @@ int mepgl_redistribute(mepgl_ctx* c, int cubic) {
   return redistribute_all(c, cubic);
 }
 
+int mepgl_gauge_align(mepgl_ctx* c) {
+  if (!c) return MEPGL_ERR_ARG;
+  hipSetDevice(c->device);
+  return gauge_align_all(c);
+}
+
 int mepgl_reset_interior_beta(mepgl_ctx* c) {
   if (!c) return MEPGL_ERR_ARG;
   hipSetDevice(c->device);
@@ -896,7 +919,8 @@ int mepgl_reset_interior_beta(mepgl_ctx* c) {
 }
 
 int mepgl_resample(mepgl_ctx* c, int si, int F_new) {
-  if (!valid_string(c, si) || F_new < 2) return fail(c, MEPGL_ERR_ARG, "resample: bad arguments");
+  if (!valid_string(c, si) || F_new < 2 || F_new > kMaxImages)
+    return fail(c, MEPGL_ERR_ARG, "resample: bad arguments");
   hipSetDevice(c->device);
   int rc = arc_length_all(c);
   if (rc) return rc;
@@ -1042,6 +1066,7 @@ int mepgl_get_scalars(mepgl_ctx* c, int si, int which, double* out) {
       case MEPGL_SCALAR_BETA: out[n] = x.beta; break;
       case MEPGL_SCALAR_G_DELTA_G: out[n] = x.g_delta_g; break;
       case MEPGL_SCALAR_LS_INDEX: out[n] = x.ls_index; break;
+      case MEPGL_SCALAR_THETA: out[n] = x.theta; break;
       case MEPGL_SCALAR_ENERGY: out[n] = e[n]; break;
       case MEPGL_SCALAR_POLY:
         for (int q = 0; q < 4; ++q) out[4 * n + q] = x.poly[q];

@@ -13,15 +13,14 @@
 // One NLCG step (nlcgSteps, nlcg.cu:1478-1583) is two launches:
 //   G  fused pending update x += alpha d (updateFields), gradient
 //      (gradientPsi + gradientA), |g|^2 and g.(g - g_old) partials
-//      (gradientDensity); the last wavefront of each image reduces the
-//      partials in a fixed order and forms the Polak-Ribiere beta.
+//      (gradientDensity).
 //   P  direction d = -g + beta d (computeStepDirection), evaluated on the
-//      stencil, line-search quartic partials (polynomialExpansion); the last
-//      wavefront of each image reduces them and runs the line search
-//      (lineSearch) as a parallel scan over a host-built table.
+//      stencil, line-search quartic partials (polynomialExpansion).
+// Each is followed by a finishing launch with one workgroup per image that
+// reduces the partials in a fixed order: beta after G; A..E and the line
+// search (lineSearch, a parallel scan over a host-built table) after P.
 // Reductions are deterministic (fixed partial order, fixed trees), with
-// two-sum compensation in the mixed-precision path.  Arrival counters reset
-// themselves, so the launches can be replayed from a hipGraph.
+// two-sum compensation in the mixed-precision path.
 #include <float.h>
 
 #include "common.hpp"
@@ -416,38 +415,6 @@ __global__ __launch_bounds__(kBlock) void k_grad(StepArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Line search, lineSearch nlcg.cu:1398-1445, as a parallel scan: candidate p
-// compares f(x_{p+1}) with f(x_p); the first rise ends the scan and
-// alpha = x_{p+1} / m.  f is evaluated without contraction, in the
-// reference's operation order, so the result equals the sequential loop's.
-__device__ __forceinline__ double quartic_at(const LineSearchTable& t, int k, double A, double B,
-                                             double C, double D, double E) {
-  const double x = t.x[k];
-  double f = __dmul_rn(E, t.x4[k]);
-  f = __dadd_rn(f, __dmul_rn(__dmul_rn(__dmul_rn(D, x), x), x));
-  f = __dadd_rn(f, __dmul_rn(__dmul_rn(C, x), x));
-  f = __dadd_rn(f, __dmul_rn(B, x));
-  f = __dadd_rn(f, A);
-  return f;
-}
-
-__device__ int line_search_wave(const LineSearchTable& t, double A, double B, double C, double D,
-                                double E) {
-  for (int base = 0; base < kLsSteps; base += kWave) {
-    const int p = base + (int)threadIdx.x;
-    bool rise = false;
-    if (p < kLsSteps) {
-      const double f_old = quartic_at(t, p, A, B, C, D, E);
-      const double f_new = quartic_at(t, p + 1, A, B, C, D, E);
-      rise = f_new > f_old;
-    }
-    const unsigned long long m = __ballot(rise);
-    if (m) return base + __ffsll((long long)m) - 1;
-  }
-  return kLsSteps - 1;
-}
-
-// ---------------------------------------------------------------------------
 // P kernel.
 template <class T, int NC>
 struct PRaw {
@@ -663,22 +630,62 @@ __global__ __launch_bounds__(kBlock) void k_finish_grad(FinishArgs f) {
   }
 }
 
-// After P: the quartic A..E and the line search (lineSearch).
+// After P: the quartic A..E and the line search (lineSearch).  One
+// 1024-thread workgroup per image: wavefront 0 reduces the partials while
+// every thread loads its scan candidates p and p + 1024, so the scan takes
+// at most two rounds; the first rise is found with an LDS minimum.
+constexpr int kFinishPolyBlock = 1024;
+
 template <bool COMP>
-__global__ __launch_bounds__(kBlock) void k_finish_poly(FinishArgs f) {
-  int n;
-  const DevString& s = string_of_image(f, n);
-  double fin[5];
-  reduce_partials<5, COMP>(s.part_p, n, s.tiles_x * s.tiles_y, fin);
-  const int first = line_search_wave(*f.ls, fin[0], fin[1], fin[2], fin[3], fin[4]);
-  if (threadIdx.x == 0) {
+__global__ __launch_bounds__(kFinishPolyBlock) void k_finish_poly(FinishArgs f) {
+  __shared__ double s_q[5];
+  __shared__ int s_first;
+  const int2 si = f.img[blockIdx.x];
+  const DevString& s = f.strings[si.x];
+  const LineSearchTable& t = *f.ls;
+  const int tid = threadIdx.x;
+  double x[2][2], x4[2][2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int p = min(tid + r * kFinishPolyBlock, kLsSteps - 1);
+    x[r][0] = t.x[p];
+    x[r][1] = t.x[p + 1];
+    x4[r][0] = t.x4[p];
+    x4[r][1] = t.x4[p + 1];
+  }
+  if (tid == 0) s_first = kLsSteps;
+  if (tid < kWave) {
+    double fin[5];
+    reduce_partials<5, COMP>(s.part_p, si.y, s.tiles_x * s.tiles_y, fin);
+    if (tid == 0)
+#pragma unroll
+      for (int k = 0; k < 5; ++k) s_q[k] = fin[k];
+  }
+  __syncthreads();
+  const double A = s_q[0], B = s_q[1], C = s_q[2], D = s_q[3], E = s_q[4];
+  // f evaluated without contraction, in the reference's operation order.
+  auto fq = [&](double xv, double x4v) {
+    double v = __dmul_rn(E, x4v);
+    v = __dadd_rn(v, __dmul_rn(__dmul_rn(__dmul_rn(D, xv), xv), xv));
+    v = __dadd_rn(v, __dmul_rn(__dmul_rn(C, xv), xv));
+    v = __dadd_rn(v, __dmul_rn(B, xv));
+    return __dadd_rn(v, A);
+  };
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int p = tid + r * kFinishPolyBlock;
+    if (p < kLsSteps && fq(x[r][1], x4[r][1]) > fq(x[r][0], x4[r][0])) atomicMin(&s_first, p);
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const int first = s_first < kLsSteps ? s_first : kLsSteps - 1;
     ImageScalars& sc = f.sc[blockIdx.x];
-    sc.fenergy = fin[0];
-    sc.poly[0] = fin[1];
-    sc.poly[1] = fin[2];
-    sc.poly[2] = fin[3];
-    sc.poly[3] = fin[4];
-    sc.alpha = f.ls->alpha[first];
+    sc.fenergy = A;
+    sc.poly[0] = B;
+    sc.poly[1] = C;
+    sc.poly[2] = D;
+    sc.poly[3] = E;
+    sc.alpha = t.alpha[first];
     sc.ls_index = first;
   }
 }
@@ -891,44 +898,63 @@ __global__ __launch_bounds__(kBlock) void k_arclen(AuxArgs a) {
 // generateNewFramesLinear (interpolation.cu:200-219) and of
 // generateNewFramesCubic (interpolation.cu:373-390), and the Thomas factors
 // of the spline system built by buildHMatrix (interpolation.cu:252-290).
-// One thread per string.
-__global__ void k_arc_finalize(AuxArgs a, int nstrings) {
-  const int si = blockIdx.x * blockDim.x + threadIdx.x;
+// One workgroup per string.  The sequential recurrences (arclength prefix,
+// the spline's running sums, the Thomas factors) run in one thread over LDS
+// copies in the reference's order; the segment searches run one new image
+// per thread: for a non-decreasing s the reference's incremental search
+// stops at the first node k >= 1 with s[k] >= s_new, which is what each
+// thread finds directly.
+__global__ __launch_bounds__(kArcBlock) void k_arc_finalize(AuxArgs a, int nstrings) {
+  __shared__ double ls[kMaxImages], lsj[kMaxImages], cum[kMaxImages];
+  const int si = blockIdx.x;
   if (si >= nstrings) return;
   const DevString& s = a.strings[si];
   ImageScalars* sc = a.sc + s.img0;
   const int F = s.F;
-  if (F < 2) return;
-  sc[0].s = 0;
-  sc[0].s_jb = 0;
-  for (int n = 1; n < F; ++n) {
-    sc[n].s = sc[n - 1].s + sqrt(sc[n].ds2);
-    sc[n].s_jb = sc[n - 1].s_jb + sqrt(sc[n].ds2_jb);
+  const int tid = threadIdx.x;
+  if (F < 2 || F > kMaxImages) return;
+  for (int n = tid; n < F; n += kArcBlock) {
+    ls[n] = sc[n].ds2;
+    lsj[n] = sc[n].ds2_jb;
   }
-  for (int n = 0; n < F; ++n) {
-    sc[n].s /= sc[F - 1].s;
-    sc[n].s_jb /= sc[F - 1].s_jb;
-  }
-  // Linear plan (F_new = F).
-  {
-    int k = 1;
-    double seg_start = 0, seg_end = sc[1].s_jb;
-    for (int n = 0; n < F; ++n) {
-      const double s_new = (double)n / ((double)F - 1);
-      while (seg_end < s_new && k + 1 < F) {
-        ++k;
-        seg_start = sc[k - 1].s_jb;
-        seg_end = sc[k].s_jb;
-      }
-      s.plan_k[n] = k;
-      s.plan_w[n] = (s_new - seg_start) / (sc[k].s_jb - sc[k - 1].s_jb);
+  __syncthreads();
+  if (tid == 0) {
+    ls[0] = 0;
+    lsj[0] = 0;
+    for (int n = 1; n < F; ++n) {
+      ls[n] = ls[n - 1] + sqrt(ls[n]);
+      lsj[n] = lsj[n - 1] + sqrt(lsj[n]);
     }
   }
-  // Cubic plan and spline factors (need F >= 4).
-  if (F >= 4) {
-    double* cb = s.cubic;  // [F][4]: c', m, sub-diagonal, h
-    for (int n = 0; n < F; ++n) cb[4 * n + 3] = n == 0 ? 0.0 : sc[n].s_jb - sc[n - 1].s_jb;
-    auto h = [&](int n) { return cb[4 * n + 3]; };
+  __syncthreads();
+  const double tot = ls[F - 1], totj = lsj[F - 1];
+  __syncthreads();
+  for (int n = tid; n < F; n += kArcBlock) {
+    ls[n] /= tot;
+    lsj[n] /= totj;
+    sc[n].s = ls[n];
+    sc[n].s_jb = lsj[n];
+  }
+  __syncthreads();
+  // Linear plan (F_new = F).
+  for (int n = tid; n < F; n += kArcBlock) {
+    const double s_new = (double)n / ((double)F - 1);
+    int k = 1;
+    while (k + 1 < F && lsj[k] < s_new) ++k;
+    const double seg_start = k == 1 ? 0.0 : lsj[k - 1];
+    s.plan_k[n] = k;
+    s.plan_w[n] = (s_new - seg_start) / (lsj[k] - lsj[k - 1]);
+  }
+  if (F < 4) return;
+  // Cubic: h, the running sums of h, the Thomas factors, then the plan.
+  double* cb = s.cubic;  // [F][4]: c', m, sub-diagonal, h
+  for (int n = tid; n < F; n += kArcBlock) cb[4 * n + 3] = n == 0 ? 0.0 : lsj[n] - lsj[n - 1];
+  __syncthreads();
+  if (tid == 0) {
+    auto h = [&](int n) { return n == 0 ? 0.0 : lsj[n] - lsj[n - 1]; };
+    cum[0] = 0;
+    for (int n = 1; n < F; ++n) cum[n] = cum[n - 1] + h(n);
+    double cprev = 0;
     for (int n = 0; n < F; ++n) {
       double dl, d, du;
       if (n == 0) {
@@ -938,23 +964,20 @@ __global__ void k_arc_finalize(AuxArgs a, int nstrings) {
       } else {
         dl = h(n); d = 2 * (h(n) + h(n + 1)); du = h(n + 1);
       }
-      const double m = n == 0 ? d : d - dl * cb[4 * (n - 1) + 0];
-      cb[4 * n + 0] = du / m;
+      const double m = n == 0 ? d : d - dl * cprev;
+      cprev = du / m;
+      cb[4 * n + 0] = cprev;
       cb[4 * n + 1] = m;
       cb[4 * n + 2] = dl;
     }
+  }
+  __syncthreads();
+  for (int n = tid; n < F; n += kArcBlock) {
+    const double s_new = (double)n / ((double)F - 1);
     int k = 1;
-    double seg_start = 0, seg_end = h(1);
-    for (int n = 0; n < F; ++n) {
-      const double s_new = (double)n / ((double)F - 1);
-      while (seg_end < s_new && k + 1 < F) {
-        ++k;
-        seg_start += h(k - 1);
-        seg_end += h(k);
-      }
-      s.cplan_k[n] = k;
-      s.cplan_w[n] = (s_new - seg_start) / h(k);
-    }
+    while (k + 1 < F && cum[k] < s_new) ++k;
+    s.cplan_k[n] = k;
+    s.cplan_w[n] = (s_new - cum[k - 1]) / (lsj[k] - lsj[k - 1]);
   }
 }
 
@@ -1047,6 +1070,93 @@ __global__ __launch_bounds__(kBlock) void k_redistribute_cubic(AuxArgs a, int te
         O[(long long)n * s.image_stride] = (T)v;
       }
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Constant-phase alignment of neighbouring images (no reference function; the
+// reference relies on the gauge-invariant s_jb metric).  Partials of
+// sum conj(psi_n) psi_{n-1} over sites and components, image n >= 1.
+template <class T, int NC>
+__device__ void align_body(const AuxArgs& a, const DevString& s, const WorkItem& it) {
+  const Strip t = strip_of(s, it);
+  double acc_s[2] = {0, 0}, acc_c[2] = {0, 0};
+  if (t.n > 0 && t.i < s.N) {
+    const T* X1 = (const T*)s.x[a.xin] + (long long)t.n * s.image_stride;
+    const T* X0 = X1 - s.image_stride;
+    for (int j = t.j0; j < t.j1; ++j) {
+      const long long off = (long long)j * s.pitch + t.i;
+      double re = 0, im = 0;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        const long long pu = (PU1 + 2 * c) * s.plane_stride + off;
+        const long long pv = (PV1 + 2 * c) * s.plane_stride + off;
+        const double u1 = X1[pu], v1 = X1[pv], u0 = X0[pu], v0 = X0[pv];
+        re += u1 * u0 + v1 * v0;
+        im += u1 * v0 - v1 * u0;
+      }
+      two_sum_acc(acc_s[0], acc_c[0], re);
+      two_sum_acc(acc_s[1], acc_c[1], im);
+    }
+  }
+  publish_partial<2, true>(acc_s, acc_c, s.part_a, t.n, t.tile, s.tiles_x * s.tiles_y);
+}
+
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_align(AuxArgs a) {
+  const WorkItem it = a.items[xcd_item(blockIdx.x, a.nitems)];
+  const DevString& s = a.strings[it.string];
+  if (s.nc == 2)
+    align_body<T, 2>(a, s, it);
+  else
+    align_body<T, 1>(a, s, it);
+}
+
+// Theta_n = sum_{m <= n} arg(z_m), one thread per string.
+__global__ void k_align_finalize(AuxArgs a, int nstrings) {
+  const int si = blockIdx.x * blockDim.x + threadIdx.x;
+  if (si >= nstrings) return;
+  const DevString& s = a.strings[si];
+  ImageScalars* sc = a.sc + s.img0;
+  sc[0].theta = 0;
+  for (int n = 1; n < s.F; ++n) sc[n].theta = sc[n - 1].theta + atan2(sc[n].align_im, sc[n].align_re);
+}
+
+// Rotates psi of image n by theta_n in the fields, the direction and both
+// gradient buffers (the energy and the NLCG inner products are invariant).
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_align_apply(AuxArgs a) {
+  const WorkItem it = a.items[xcd_item(blockIdx.x, a.nitems)];
+  const DevString& s = a.strings[it.string];
+  const Strip t = strip_of(s, it);
+  if (t.n == 0 || t.i >= s.N) return;
+  const double th = a.sc[s.img0 + t.n].theta;
+  const T cs = (T)cos(th), sn = (T)sin(th);
+  T* bufs[4] = {(T*)s.x[a.xin], (T*)s.d[a.dcur], (T*)s.g[0], (T*)s.g[1]};
+  for (int b = 0; b < 4; ++b) {
+    T* B = bufs[b] + (long long)t.n * s.image_stride;
+    for (int j = t.j0; j < t.j1; ++j) {
+      const long long off = (long long)j * s.pitch + t.i;
+      for (int c = 0; c < s.nc; ++c) {
+        const long long pu = (PU1 + 2 * c) * s.plane_stride + off;
+        const long long pv = (PV1 + 2 * c) * s.plane_stride + off;
+        const T u = B[pu], v = B[pv];
+        B[pu] = u * cs - v * sn;
+        B[pv] = u * sn + v * cs;
+      }
+    }
+  }
+}
+
+// After the alignment partials.
+__global__ __launch_bounds__(kBlock) void k_finish_align(FinishArgs f) {
+  int n;
+  const DevString& s = string_of_image(f, n);
+  double fin[2];
+  reduce_partials<2, true>(s.part_a, n, s.tiles_x * s.tiles_y, fin);
+  if (threadIdx.x == 0) {
+    f.sc[blockIdx.x].align_re = fin[0];
+    f.sc[blockIdx.x].align_im = fin[1];
   }
 }
 
@@ -1145,7 +1255,7 @@ static hipError_t launch_aux_impl(const AuxArgs& a, int which, int arg, hipStrea
       if (a.nitems > 0) hipLaunchKernelGGL(k_arclen<T>, g, b, 0, st, a);
       break;
     case kKernelArcFinalize:
-      hipLaunchKernelGGL(k_arc_finalize, dim3((arg + 63) / 64), dim3(64), 0, st, a, arg);
+      hipLaunchKernelGGL(k_arc_finalize, dim3(arg), dim3(kArcBlock), 0, st, a, arg);
       break;
     case kKernelLinear:
       if (a.nitems > 0) hipLaunchKernelGGL(k_redistribute_linear<T>, g, b, 0, st, a);
@@ -1158,6 +1268,15 @@ static hipError_t launch_aux_impl(const AuxArgs& a, int which, int arg, hipStrea
       break;
     case kKernelResetNorms:
       hipLaunchKernelGGL(k_reset_norms, dim3(a.nstrings), dim3(64), 0, st, a, a.nstrings, arg);
+      break;
+    case kKernelAlign:
+      if (a.nitems > 0) hipLaunchKernelGGL(k_align<T>, g, b, 0, st, a);
+      break;
+    case kKernelAlignFinalize:
+      hipLaunchKernelGGL(k_align_finalize, dim3((arg + 63) / 64), dim3(64), 0, st, a, arg);
+      break;
+    case kKernelAlignApply:
+      if (a.nitems > 0) hipLaunchKernelGGL(k_align_apply<T>, g, b, 0, st, a);
       break;
     default:
       return hipErrorInvalidValue;
@@ -1182,15 +1301,18 @@ hipError_t launch_finish(const FinishArgs& f, int which, bool fp32, hipStream_t 
       break;
     case kKernelPoly:
       if (fp32)
-        hipLaunchKernelGGL(k_finish_poly<true>, g, b, 0, st, f);
+        hipLaunchKernelGGL(k_finish_poly<true>, g, dim3(kFinishPolyBlock), 0, st, f);
       else
-        hipLaunchKernelGGL(k_finish_poly<false>, g, b, 0, st, f);
+        hipLaunchKernelGGL(k_finish_poly<false>, g, dim3(kFinishPolyBlock), 0, st, f);
       break;
     case kKernelEnergy:
       hipLaunchKernelGGL(k_finish_energy, g, b, 0, st, f);
       break;
     case kKernelArclen:
       hipLaunchKernelGGL(k_finish_arclen, g, b, 0, st, f);
+      break;
+    case kKernelAlign:
+      hipLaunchKernelGGL(k_finish_align, g, b, 0, st, f);
       break;
     default:
       return hipErrorInvalidValue;

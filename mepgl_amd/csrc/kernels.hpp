@@ -19,6 +19,9 @@ enum KernelId {
   kKernelCubic,
   kKernelCopyA,
   kKernelResetNorms,
+  kKernelAlign,        // alignment partials
+  kKernelAlignFinalize,
+  kKernelAlignApply,
   kKernelCount
 };
 
@@ -49,6 +52,7 @@ struct AuxArgs {
   int nstrings;
   ImageScalars* sc;
   int xin, xout;
+  int dcur;             // current direction buffer
   double* out;          // per global image (energy)
   double* const* jb;    // per string JB planes (device array of pointers)
 };

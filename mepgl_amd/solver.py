@@ -34,14 +34,16 @@ class Solver:
 
     def __init__(self, strings: Sequence[GLString], precision: str = "fp64",
                  poly_exact: bool = False, cubic_textbook: bool = False,
-                 use_graph: bool = True, device: int = 0, segment_rows: int = 0):
+                 use_graph: bool = True, device: int = 0, segment_rows: int = 0,
+                 gauge_align: bool = False):
         self._L = _lib.load()
         self.strings: List[GLString] = [s.validate() for s in strings]
         self.precision = precision
         prec = {"fp64": _lib.FP64, "mixed": _lib.MIXED}[precision]
         flags = (_lib.POLY_EXACT if poly_exact else 0) | \
                 (_lib.CUBIC_TEXTBOOK if cubic_textbook else 0) | \
-                (0 if use_graph else _lib.NO_GRAPH)
+                (0 if use_graph else _lib.NO_GRAPH) | \
+                (_lib.GAUGE_ALIGN if gauge_align else 0)
         self._keep = []
         descs = (_lib.StringDesc * len(self.strings))()
         for d, s in zip(descs, self.strings):
@@ -178,6 +180,12 @@ class Solver:
         """End-of-run resampling to F_new images (mepgl.cu:699-766)."""
         self._check(self._L.mepgl_resample(self._ctx, i, int(F_new)))
         self.F[i] = int(F_new)
+
+    def gauge_align(self):
+        """Constant-phase alignment of neighbouring images; returns the angles
+        applied to each string's images."""
+        self._check(self._L.mepgl_gauge_align(self._ctx))
+        return [self.scalars(i, "theta") for i in range(len(self.strings))]
 
     def reset_interior_beta(self):
         """mepgl.cu:231-234."""

@@ -32,6 +32,8 @@
  *   or_arc_length(_jb)     affine strings give s_n = n/(F-1)
  *   or_redistribute_*      exactness on linear / cubic data, node
  *                          interpolation, banded solve of the spline system
+ *   or_gauge_align         removes a known constant phase between images;
+ *                          energy, s_jb unchanged; no neighbour distance grows
  */
 #include "mepgl_oracle.h"
 
@@ -881,4 +883,47 @@ int or_redistribute_cubic(int N, int F, const double *s, const double *f, int F_
     free(d);
     free(du);
     return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Constant-phase alignment of neighbouring images (no reference function:
+ * the reference relies on the gauge-invariant s_jb metric).  Image n is
+ * rotated by Theta_n = sum_{m<=n} arg(sum conj(psi_m) psi_{m-1}) over all
+ * sites and components; both components rotate by the same angle, which is
+ * an exact symmetry of the discrete energy.  theta receives Theta_n
+ * (F values). */
+void or_gauge_align(int N, int F, int multicomponent, double *psi_1, double *psi_2,
+                    double *theta)
+{
+    const size_t NN = (size_t)N * N;
+    double *phi = (double *)calloc((size_t)F, sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int n = 1; n < F; n++) {
+        double re = 0, im = 0;
+        for (int c = 0; c < 1 + (multicomponent ? 1 : 0); c++) {
+            const double *p = c == 0 ? psi_1 : psi_2;
+            for (size_t k = 0; k < NN; k++) {
+                const size_t i = 2 * (k + NN * n), j = i - 2 * NN;
+                re += p[i] * p[j] + p[i + 1] * p[j + 1];
+                im += p[i] * p[j + 1] - p[i + 1] * p[j];
+            }
+        }
+        phi[n] = atan2(im, re);
+    }
+    theta[0] = 0;
+    for (int n = 1; n < F; n++) theta[n] = theta[n - 1] + phi[n];
+    free(phi);
+#pragma omp parallel for schedule(static)
+    for (int n = 1; n < F; n++) {
+        const double cs = cos(theta[n]), sn = sin(theta[n]);
+        for (int c = 0; c < 1 + (multicomponent ? 1 : 0); c++) {
+            double *p = c == 0 ? psi_1 : psi_2;
+            for (size_t k = 0; k < NN; k++) {
+                const size_t i = 2 * (k + NN * n);
+                const double u = p[i], v = p[i + 1];
+                p[i] = u * cs - v * sn;
+                p[i + 1] = u * sn + v * cs;
+            }
+        }
+    }
 }

@@ -88,6 +88,9 @@ void or_redistribute_linear(int N, int F, const double *s, const double *f,
 int or_redistribute_cubic(int N, int F, const double *s, const double *f,
                           int F_new, double *f_new, int textbook);
 
+void or_gauge_align(int N, int F, int multicomponent, double *psi_1, double *psi_2,
+                    double *theta);
+
 void or_thomas(int n, const double *dl, const double *d, const double *du,
                double *rhs);
 

@@ -226,6 +226,16 @@ def redistribute_cubic(N, F, s, f, F_new, textbook=False):
     return out
 
 
+def gauge_align(N, F, multicomponent, p1, p2):
+    """Constant-phase alignment; returns (psi_1, psi_2, theta)."""
+    p1 = _f64(p1).copy()
+    p2 = _f64(p2).copy() if p2 is not None else np.zeros_like(p1)
+    th = np.zeros(F)
+    lib().or_gauge_align(C.c_int(N), C.c_int(F), C.c_int(int(multicomponent)), _p(p1), _p(p2),
+                         _p(th))
+    return p1, p2, th
+
+
 def thomas(dl, d, du, rhs):
     x = _f64(rhs).copy()
     lib().or_thomas(C.c_int(len(x)), _p(_f64(dl)), _p(_f64(d)), _p(_f64(du)), _p(x))
@@ -322,19 +332,41 @@ class OracleString:
             new.append(self.psi_2)
         self.a, self.psi_1, self.psi_2 = new
 
+    def gauge_align(self):
+        """Constant-phase alignment of neighbouring images, applied to the
+        fields, the direction and both gradient buffers."""
+        p1, p2, th = gauge_align(self.N, self.F, self.multicomponent, self.psi_1,
+                                 self.psi_2 if self.multicomponent else None)
+        self.psi_1 = p1
+        if self.multicomponent:
+            self.psi_2 = p2
+        rot = np.exp(1j * th)[:, None, None]
+
+        def rotate(arr):
+            z = (arr[..., 0] + 1j * arr[..., 1]) * rot
+            arr[..., 0], arr[..., 1] = z.real, z.imag
+
+        for arr in [self.d_p1, self.g_p1[0], self.g_p1[1]] + \
+                ([self.d_p2, self.g_p2[0], self.g_p2[1]] if self.multicomponent else []):
+            rotate(arr)
+        return th
+
     def reset_interior_beta(self):
         """mepgl.cu:231-234."""
         self.g_norm2[0][1:-1] = np.inf
         self.g_norm2[1][1:-1] = np.inf
 
-    def loop(self, iterations, relaxation_steps, mode="L", nlcg=True):
-        """The outer loop of mepgl.cu:148-360 (without printing)."""
+    def loop(self, iterations, relaxation_steps, mode="L", nlcg=True, align=False):
+        """The outer loop of mepgl.cu:148-360 (without printing); align adds
+        the optional constant-phase alignment before each arclength."""
         for _ in range(iterations):
             if mode == "S":
                 self.nlcg_steps(relaxation_steps, nlcg, False)
             elif mode == "M":
                 self.nlcg_steps(relaxation_steps, nlcg, True)
             elif mode in ("L", "C"):
+                if align:
+                    self.gauge_align()
                 self.arc_lengths()
                 self.redistribute(cubic=(mode == "C"))
                 self.reset_interior_beta()

@@ -216,3 +216,35 @@ def test_resample(oracle):
     assert normwise(p1, ref.psi_1) < 1e-12
     assert normwise(p2, ref.psi_2) < 1e-12
     assert normwise(a, ref.a) < 1e-12
+
+
+@pytest.mark.parametrize("mc", [False, True])
+def test_gauge_alignment(mc, oracle):
+    s = small_string(N=20, F=6, seed=44, multicomponent=mc)
+    say(f"[gpu] gauge alignment mc={mc}")
+    ref = oracle.OracleString(s)
+    ref.nlcg_steps(2)
+    th_ref = ref.gauge_align()
+    ref.nlcg_steps(2)
+    with solver_for([s]) as sv:
+        sv.nlcgSteps(2)
+        th, = sv.gauge_align()
+        sv.nlcgSteps(2)
+        a, p1, p2 = sv.download(0)
+    np.testing.assert_allclose(th, th_ref, rtol=1e-11, atol=1e-13)
+    assert normwise(p1, ref.psi_1) < 1e-10
+    assert normwise(a, ref.a) < 1e-10
+    if mc:
+        assert normwise(p2, ref.psi_2) < 1e-10
+
+
+def test_loop_with_alignment(oracle):
+    s = small_string(N=20, F=6, seed=45, multicomponent=True)
+    say("[gpu] loop with alignment")
+    ref = oracle.OracleString(s)
+    ref.loop(2, 4, "L", align=True)
+    with solver_for([s], gauge_align=True) as sv:
+        sv.loop(2, 4, "L")
+        a, p1, p2 = sv.download(0)
+    assert normwise(p1, ref.psi_1) < 1e-10
+    assert normwise(p2, ref.psi_2) < 1e-10
