@@ -147,8 +147,9 @@ int mepgl_reset_interior_beta(mepgl_ctx *ctx);
 /* loop() (mepgl.cu:148-306): `iterations` outer iterations of mode
  * 'S' (relaxation), 'M' (Maxwell: psi frozen), 'L' (linear
  * reparametrisation then relaxation) or 'C' (cubic).  Each outer iteration
- * is one hipGraph replay unless MEPGL_NO_GRAPH.  Stops early when the stop
- * control is set. */
+ * is one hipGraph replay unless MEPGL_NO_GRAPH.  As in the reference
+ * (mepgl.cu:357-359), the stop control is checked after each iteration, so
+ * a call runs at least one; the control stays set until cleared. */
 int mepgl_loop(mepgl_ctx *ctx, int iterations, int relaxation_steps, char mode, int nlcg);
 
 /* Per-image scalars of one string (F values, 4F for MEPGL_SCALAR_POLY). */

@@ -1002,7 +1002,7 @@ int mepgl_loop(mepgl_ctx* c, int iterations, int relax, char mode, int nlcg) {
   if (mode == 'C' && (rc = ensure_cubic_rhs(c))) return rc;
   const bool use_graph = !(c->flags & MEPGL_NO_GRAPH) && !c->timing.on;
   for (int it = 0; it < iterations; ++it) {
-    if (c->stop.load()) break;
+    if (it > 0 && c->stop.load()) break;
     if (!use_graph) {
       if ((rc = outer_iteration(c, relax, mode, nlcg ? 1 : 0))) return rc;
       continue;

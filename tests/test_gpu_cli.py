@@ -1,0 +1,148 @@
+"""The mepgl program end to end on the GPU against the oracle: the files it
+writes after a run of the relaxation loop (io.cu:128-291), energy.csv
+(mepgl.cu:348-355), the resampling to --Fout images (mepgl.cu:699-766),
+the stdin stop key (mepgl.cu:647-651) and the staged launcher
+(launcher_utils.py:151-191).
+
+Tolerance: fields per-image norm-wise relative 1e-10 (FP64), energies
+relative 1e-11 after --iterations outer iterations.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import normwise, rel_err, small_string
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "mepgl_amd", "mepgl")
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def cli(root, *args, stdin=subprocess.DEVNULL, timeout=120):
+    if not os.path.exists(EXE):
+        raise FileNotFoundError(f"{EXE} is not built")
+    r = subprocess.run([EXE, *map(str, args)], cwd=root, capture_output=True, text=True,
+                       stdin=stdin, timeout=timeout)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    return r
+
+
+@pytest.mark.parametrize("mode", ["L", "C", "S"])
+def test_cli_run_matches_oracle(tmp_path, oracle, mode):
+    from mepgl_amd import npyio
+
+    s = small_string(N=20, F=6, seed=21, multicomponent=True)
+    say(f"[gpu] cli mode={mode} N=20 F=6")
+    npyio.write_input(s, "t", str(tmp_path))
+    r = cli(str(tmp_path), "--simname", "t", "--mode", mode, "--Fin", 6, "--Fout", 6,
+            "--iterations", 3, "--relaxation-steps", 4)
+    assert r.stdout.count("Iteration:") == 3
+    assert "Saving results...  done." in r.stdout
+    out, extra = npyio.read_dir(os.path.join(tmp_path, "simulations", "t", "output_data"))
+
+    ref = oracle.OracleString(s)
+    ref.loop(3, 4, mode)
+    assert normwise(out.psi_1, ref.psi_1) < 1e-10
+    assert normwise(out.psi_2, ref.psi_2) < 1e-10
+    assert normwise(out.a, ref.a) < 1e-10
+    assert rel_err(extra["fenergy"], ref.free_energy()) < 1e-11
+    b, j1, j2 = ref.compute_jb()
+    assert normwise(extra["b"], b) < 1e-10
+    assert normwise(extra["j_1"], j1) < 1e-10
+    assert normwise(extra["j_2"], j2) < 1e-10
+    if mode != "S":
+        # s and s_jb are those of the last reparametrisation
+        assert rel_err(extra["s_jb"], ref.s_jb) < 1e-12
+        assert rel_err(extra["s"], ref.s) < 1e-12
+    # energy.csv holds the last iteration's line-search energies
+    with open(os.path.join(tmp_path, "energy.csv")) as f:
+        csv = np.array([float(v) for v in f.read().split(",")])
+    assert csv.shape == (6,)
+    np.testing.assert_allclose(csv, ref.fenergy, rtol=1e-5)
+
+
+def test_cli_resample_to_fout(tmp_path, oracle):
+    from mepgl_amd import npyio
+
+    s = small_string(N=20, F=5, seed=22, multicomponent=False)
+    say("[gpu] cli resample 5 -> 9")
+    npyio.write_input(s, "t", str(tmp_path))
+    cli(str(tmp_path), "--simname", "t", "--mode", "L", "--Fin", 5, "--Fout", 9,
+        "--iterations", 2, "--relaxation-steps", 3)
+    out, extra = npyio.read_dir(os.path.join(tmp_path, "simulations", "t", "output_data"))
+    assert out.F == 9 and extra["fenergy"].shape == (9,)
+    ref = oracle.OracleString(s)
+    ref.loop(2, 3, "L")
+    ref.arc_lengths()
+    ref.redistribute(F_new=9)
+    assert normwise(out.psi_1, ref.psi_1) < 1e-10
+    assert normwise(out.a, ref.a) < 1e-10
+    assert rel_err(extra["fenergy"], ref.free_energy()) < 1e-11
+
+
+def test_cli_mixed_and_float32_input(tmp_path, oracle):
+    from mepgl_amd import npyio
+
+    s = small_string(N=20, F=5, seed=23, multicomponent=True)
+    say("[gpu] cli mixed precision, float32 files")
+    npyio.write_input(s, "t", str(tmp_path), double_precision=False)
+    cli(str(tmp_path), "--simname", "t", "--mode", "L", "--Fin", 5, "--iterations", 2,
+        "--relaxation-steps", 3, "--precision", "mixed")
+    d = os.path.join(tmp_path, "simulations", "t", "output_data")
+    assert np.load(os.path.join(d, "0", "u_1.npy")).dtype == np.float32
+    out, extra = npyio.read_dir(d)
+    s32, _ = npyio.read_dir(os.path.join(tmp_path, "simulations", "t", "input_data"))
+    ref = oracle.OracleString(s32)
+    ref.loop(2, 3, "L")
+    assert rel_err(extra["fenergy"], ref.free_energy()) < 1e-4
+
+
+def test_cli_stop_key(tmp_path):
+    from mepgl_amd import npyio
+
+    s = small_string(N=16, F=4, seed=24)
+    say("[gpu] cli stop key")
+    npyio.write_input(s, "t", str(tmp_path))
+    # The key arrives while the first iterations run; the loop ends after
+    # the iteration in progress and the results are still written.
+    p = subprocess.run([EXE, "--simname", "t", "--mode", "S", "--Fin", "4", "--iterations",
+                        "100000", "--relaxation-steps", "2"], cwd=str(tmp_path), input="q",
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert "I am quitting! Please wait." in p.stdout
+    n = p.stdout.count("Iteration:")
+    assert 1 <= n < 100000
+    assert os.path.exists(os.path.join(tmp_path, "simulations", "t", "output_data", "3", "b.npy"))
+
+
+def test_launcher_stages(tmp_path, oracle):
+    from mepgl_amd import launcher, npyio
+
+    s = small_string(N=16, F=4, seed=25)
+    say("[gpu] launcher stages 4 -> 7")
+    npyio.write_input(s, "t", str(tmp_path))
+    launcher.launch_simulation("t", [4, 7], [2, 2], ["L", "L"], root=str(tmp_path),
+                               extra_args=["--relaxation-steps", "3", "--no-stdin"], capture=True)
+    out, extra = npyio.read_dir(os.path.join(tmp_path, "simulations", "t", "output_data"))
+    assert out.F == 7
+    ref = oracle.OracleString(s)
+    ref.loop(2, 3, "L")
+    ref.arc_lengths()
+    ref.redistribute(F_new=7)
+    # The reloaded input is what the program wrote: planes and couplings
+    # round-trip through float64 files exactly.
+    ref2 = oracle.OracleString(npyio.read_dir(os.path.join(
+        tmp_path, "simulations", "t", "input_data"))[0])
+    np.testing.assert_array_equal(ref2.psi_1.shape, ref.psi_1.shape)
+    assert normwise(ref2.psi_1, ref.psi_1) < 1e-10
+    ref2.loop(2, 3, "L")
+    assert normwise(out.psi_1, ref2.psi_1) < 1e-10
