@@ -191,6 +191,20 @@ int mepgl_profile_read(mepgl_ctx *ctx, int kernel, double *avg_ms, int *count);
 /* Waits for all work of the context. */
 int mepgl_synchronize(mepgl_ctx *ctx);
 
+/* Schedule.  The reference launches one thread per cell with a fixed block
+ * size (config.cuh:25-31, blockSizeX); here one wavefront marches a
+ * 64-column strip down `rows` rows.  mepgl_set_segment_rows sets one
+ * string's height (clamped to [min(N, 4), N]).  mepgl_autotune times
+ * `steps` NLCG steps per candidate height (4 .. 128) for each class of
+ * strings with equal N, components and fixed-A setting, keeps the fastest,
+ * and restores the solver state, so tuning leaves fields and scalars as it
+ * found them; it needs device memory for one copy of the fields, gradients
+ * and directions.  rows_out (n_strings entries) may be NULL.  Sums over
+ * cells are taken per strip, so a schedule change can move results in the
+ * last bits (well inside 1e-12 relative). */
+int mepgl_set_segment_rows(mepgl_ctx *ctx, int string, int rows);
+int mepgl_autotune(mepgl_ctx *ctx, int steps, int *rows_out);
+
 /* Geometry and memory facts: work items (wavefronts) per NLCG launch,
  * bytes of device memory held, and the segment height of a string. */
 int mepgl_info(mepgl_ctx *ctx, int string, int *tile_rows, int *work_items, long long *bytes);

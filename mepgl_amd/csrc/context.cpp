@@ -116,11 +116,21 @@ void free_string(mepgl_ctx* c, StringHost& s) {
   s.owned.clear();
 }
 
+// Shortest segment a schedule uses; the per-tile partial buffers are sized
+// for it, so a schedule change never reallocates.
+int min_rows(int N) { return std::min(N, 4); }
+
+int clamp_rows(int rows, int N) { return std::max(std::min(rows, N), min_rows(N)); }
+
+size_t max_tiles(const DevString& d) {
+  return (size_t)d.tiles_x * (size_t)((d.N + min_rows(d.N) - 1) / min_rows(d.N));
+}
+
 // Rows marched by one wavefront.  Taller segments re-read fewer halo rows;
 // shorter ones give more wavefronts per launch.  Pick the tallest segment
 // that still yields about eight wavefronts per SIMD (1024 SIMDs).
 int choose_rows(const mepgl_string_desc& d, long long total_images) {
-  if (d.segment_rows > 0) return std::min(d.segment_rows, d.N);
+  if (d.segment_rows > 0) return clamp_rows(d.segment_rows, d.N);
   const long long cols = (d.N + kTileX - 1) / kTileX;
   for (int h : {64, 32, 16, 8}) {
     const long long items = cols * ((d.N + h - 1) / h) * total_images;
@@ -398,7 +408,7 @@ int alloc_images(mepgl_ctx* c, StringHost& s, int F) {
   d.gs = nullptr;
   d.cubic_rhs = nullptr;
   s.jb = nullptr;
-  const size_t nt = (size_t)F * s.ntiles;
+  const size_t nt = (size_t)F * max_tiles(d);
   void* p;
   int rc;
   if ((rc = dev_alloc(c, &s, &p, nt * kPartG * sizeof(double)))) return rc;
@@ -603,6 +613,103 @@ int outer_iteration(mepgl_ctx* c, int relax, char mode, int nlcg) {
   const bool maxwell = mode == 'M';
   for (int k = 0; k < relax; ++k)
     if ((rc = nlcg_step(c, k, nlcg, maxwell ? 1 : 0))) return rc;
+  return MEPGL_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Autotuning of the schedule: the segment height of each class of strings
+// (same N, components and fixed-A setting), timed on the live problem.
+
+// Makes the descriptors, work list and graphs follow the strings' heights.
+int apply_schedule(mepgl_ctx* c) {
+  for (auto& s : c->strings) {
+    DevString& d = s.dev;
+    d.tiles_y = (d.N + d.seg_rows - 1) / d.seg_rows;
+    s.ntiles = d.tiles_x * d.tiles_y;
+  }
+  destroy_graphs(c);
+  int rc;
+  if ((rc = upload_descriptors(c)) || (rc = build_items(c))) return rc;
+  return MEPGL_OK;
+}
+
+// Device copy of everything a step reads or writes, so that timed trial
+// steps leave no trace.
+struct Snapshot {
+  std::vector<std::pair<void*, void*>> buf;  // (copy, live)
+  std::vector<size_t> bytes;
+  void* sc = nullptr;
+  int xp = 0, dp = 0;
+  bool pending = false;
+};
+
+void drop_snapshot(mepgl_ctx* c, Snapshot& sn) {
+  hipStreamSynchronize(c->stream);
+  for (auto& b : sn.buf) hipFree(b.first);
+  if (sn.sc) hipFree(sn.sc);
+  sn.buf.clear();
+  sn.sc = nullptr;
+}
+
+int take_snapshot(mepgl_ctx* c, Snapshot& sn) {
+  auto save = [&](void* live, size_t bytes) -> int {
+    void* p = nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess)
+      return fail(c, MEPGL_ERR_NOMEM, "autotune needs a copy of the solver state (" +
+                                          std::to_string(bytes) + " bytes)");
+    sn.buf.push_back({p, live});
+    sn.bytes.push_back(bytes);
+    HIP_TRY(c, hipMemcpyAsync(p, live, bytes, hipMemcpyDeviceToDevice, c->stream));
+    return MEPGL_OK;
+  };
+  int rc;
+  for (auto& s : c->strings) {
+    const size_t fs = (size_t)s.dev.F * s.dev.image_stride * s.elem;
+    for (int b = 0; b < 2; ++b)
+      if ((rc = save(s.dev.x[b], fs)) || (rc = save(s.dev.d[b], fs)) || (rc = save(s.dev.g[b], fs)))
+        return rc;
+  }
+  const size_t scb = sizeof(ImageScalars) * c->total_images;
+  if (hipMalloc(&sn.sc, scb) != hipSuccess) return fail(c, MEPGL_ERR_NOMEM, "autotune scalars");
+  HIP_TRY(c, hipMemcpyAsync(sn.sc, c->d_sc, scb, hipMemcpyDeviceToDevice, c->stream));
+  sn.xp = c->xp;
+  sn.dp = c->dp;
+  sn.pending = c->pending;
+  return MEPGL_OK;
+}
+
+int restore_snapshot(mepgl_ctx* c, const Snapshot& sn) {
+  for (size_t k = 0; k < sn.buf.size(); ++k)
+    HIP_TRY(c, hipMemcpyAsync(sn.buf[k].second, sn.buf[k].first, sn.bytes[k],
+                              hipMemcpyDeviceToDevice, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(c->d_sc, sn.sc, sizeof(ImageScalars) * c->total_images,
+                            hipMemcpyDeviceToDevice, c->stream));
+  c->xp = sn.xp;
+  c->dp = sn.dp;
+  c->pending = sn.pending;
+  return MEPGL_OK;
+}
+
+// Milliseconds per NLCG step of the current schedule, from `steps` steps
+// after one untimed step, starting from the snapshot.
+int time_schedule(mepgl_ctx* c, const Snapshot& sn, int steps, double* ms) {
+  int rc;
+  if ((rc = apply_schedule(c)) || (rc = restore_snapshot(c, sn))) return rc;
+  if ((rc = nlcg_step(c, 0, 1, 0))) return rc;
+  hipEvent_t e0, e1;
+  HIP_TRY(c, hipEventCreate(&e0));
+  HIP_TRY(c, hipEventCreate(&e1));
+  hipEventRecord(e0, c->stream);
+  for (int k = 0; k < steps && !rc; ++k) rc = nlcg_step(c, k + 1, 1, 0);
+  hipEventRecord(e1, c->stream);
+  hipError_t e = hipEventSynchronize(e1);
+  float t = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (rc) return rc;
+  if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, std::string("autotune timing: ") + hipGetErrorString(e));
+  *ms = (double)t / std::max(steps, 1);
   return MEPGL_OK;
 }
 
@@ -1144,6 +1251,79 @@ int mepgl_profile_read(mepgl_ctx* c, int kernel, double* avg_ms, int* count) {
 int mepgl_synchronize(mepgl_ctx* c) {
   if (!c) return MEPGL_ERR_ARG;
   hipSetDevice(c->device);
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return MEPGL_OK;
+}
+
+int mepgl_set_segment_rows(mepgl_ctx* c, int si, int rows) {
+  if (!valid_string(c, si) || rows < 1) return MEPGL_ERR_ARG;
+  hipSetDevice(c->device);
+  int rc = flush(c);
+  if (rc) return rc;
+  StringHost& s = c->strings[si];
+  s.dev.seg_rows = clamp_rows(rows, s.dev.N);
+  return apply_schedule(c);
+}
+
+int mepgl_autotune(mepgl_ctx* c, int steps, int* rows_out) {
+  if (!c || steps < 1) return MEPGL_ERR_ARG;
+  hipSetDevice(c->device);
+  int rc = flush(c);
+  if (rc) return rc;
+  // Classes of strings that share a kernel specialisation and a grid size,
+  // tuned one after another, largest total work first.
+  std::map<std::tuple<int, int, int>, std::vector<int>> cls;
+  for (size_t si = 0; si < c->strings.size(); ++si) {
+    const DevString& d = c->strings[si].dev;
+    cls[std::make_tuple(d.N, d.nc, d.nsf)].push_back((int)si);
+  }
+  std::vector<std::pair<double, std::vector<int>>> order;
+  for (auto& kv : cls) {
+    double work = 0;
+    for (int si : kv.second) {
+      const DevString& d = c->strings[si].dev;
+      work += (double)d.F * d.N * d.N * d.nplanes;
+    }
+    order.push_back({work, kv.second});
+  }
+  std::sort(order.begin(), order.end(),
+            [](const auto& a, const auto& b) { return a.first > b.first; });
+  Snapshot sn;
+  if ((rc = take_snapshot(c, sn))) {
+    drop_snapshot(c, sn);
+    return rc;
+  }
+  const bool timing = c->timing.on;
+  c->timing.on = false;
+  double best_ms = 0;
+  if ((rc = time_schedule(c, sn, steps, &best_ms)) == MEPGL_OK) {
+    for (auto& cl : order) {
+      const int N = c->strings[cl.second[0]].dev.N;
+      const int start = c->strings[cl.second[0]].dev.seg_rows;
+      int best = start;
+      for (int h : {4, 8, 16, 32, 64, 128}) {
+        const int r = clamp_rows(h, N);
+        if (r == start || (h > N && h != 4)) continue;
+        for (int si : cl.second) c->strings[si].dev.seg_rows = r;
+        double ms = 0;
+        if ((rc = time_schedule(c, sn, steps, &ms))) break;
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = r;
+        }
+      }
+      for (int si : cl.second) c->strings[si].dev.seg_rows = best;
+      if (rc) break;
+    }
+  }
+  c->timing.on = timing;
+  int rc2 = apply_schedule(c);
+  if (rc2 == MEPGL_OK) rc2 = restore_snapshot(c, sn);
+  drop_snapshot(c, sn);
+  if (rc) return rc;
+  if (rc2) return rc2;
+  if (rows_out)
+    for (size_t si = 0; si < c->strings.size(); ++si) rows_out[si] = c->strings[si].dev.seg_rows;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return MEPGL_OK;
 }

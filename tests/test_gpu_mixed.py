@@ -1,0 +1,81 @@
+"""The MIXED precision policy against the FP64 oracle.
+
+Fields, directions and gradients are stored and updated in float; every
+reduction, NLCG scalar, line search, arclength and reparametrisation is in
+double.  Tolerances (relative to the FP64 oracle on the same inputs):
+  energy, single evaluation                1e-5
+  gradient, per-image norm-wise            1e-4
+  energies after a short relaxation loop   1e-5
+  energy barrier max(F - F_0)              1e-3 of the barrier
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import normwise, rel_err, small_string
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "one_comp": dict(multicomponent=False),
+    "one_comp_planes": dict(multicomponent=False, masked=True, planes=True),
+    "two_comp_masked": dict(multicomponent=True, masked=True),
+    "two_comp_thin_film": dict(multicomponent=True, masked=True, no_self_field=True),
+}
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def rounded(s):
+    """The string with its fields as the float store holds them."""
+    t = s.copy()
+    t.a = t.a.astype(np.float32).astype(np.float64)
+    t.psi_1 = t.psi_1.astype(np.float32).astype(np.float64)
+    if t.psi_2 is not None:
+        t.psi_2 = t.psi_2.astype(np.float32).astype(np.float64)
+    return t
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_mixed_energy_and_gradient(case, oracle):
+    from mepgl_amd.solver import Solver
+
+    s = small_string(N=24, F=4, seed=31, **CASES[case])
+    say(f"[gpu] mixed energy/gradient case={case}")
+    with Solver([s], precision="mixed") as sv:
+        e = sv.computeFreeEnergy()[0]
+        ga, g1, g2 = sv.gradient(0)
+    t = rounded(s)
+    assert rel_err(e, oracle.free_energy(t)) < 1e-5
+    ra, r1, r2 = oracle.gradient(t)
+    assert normwise(g1, r1) < 1e-4
+    if not s.no_self_field:
+        assert normwise(ga, ra) < 1e-4
+    if s.multicomponent:
+        assert normwise(g2, r2) < 1e-4
+
+
+@pytest.mark.parametrize("case", ["one_comp", "two_comp_masked"])
+def test_mixed_loop_energies_and_barrier(case, oracle):
+    from mepgl_amd import configs, observables
+    from mepgl_amd.solver import Solver
+
+    if case == "one_comp":
+        s = configs.vortex_entry(32, 0.25, 8, 0.9, seed=2)
+    else:
+        s = small_string(N=28, F=8, seed=32, **CASES[case])
+    say(f"[gpu] mixed loop case={case}")
+    with Solver([s], precision="mixed") as sv:
+        sv.loop(4, 5, "L")
+        e = sv.computeFreeEnergy()[0]
+    ref = oracle.OracleString(s)
+    ref.loop(4, 5, "L")
+    e_ref = ref.free_energy()
+    assert rel_err(e, e_ref) < 1e-5
+    b, _ = observables.barriers(e)
+    b_ref, _ = observables.barriers(e_ref)
+    assert abs(b - b_ref) <= 1e-3 * max(abs(b_ref), 1e-12) + 1e-5 * np.max(np.abs(e_ref))

@@ -248,3 +248,50 @@ def test_loop_with_alignment(oracle):
         a, p1, p2 = sv.download(0)
     assert normwise(p1, ref.psi_1) < 1e-10
     assert normwise(p2, ref.psi_2) < 1e-10
+
+
+@pytest.mark.parametrize("rows", [4, 7, 16, 64])
+def test_segment_rows_do_not_change_results(rows, oracle):
+    s = small_string(N=40, F=4, seed=46, multicomponent=True, masked=True)
+    say(f"[gpu] segment rows {rows}")
+    ref = oracle.OracleString(s)
+    ref.loop(2, 3, "L")
+    with solver_for([s]) as sv:
+        sv.set_segment_rows(0, rows)
+        assert sv.info(0)["tile_rows"] == rows
+        sv.loop(2, 3, "L")
+        a, p1, p2 = sv.download(0)
+        fen = sv.scalars(0, "fenergy")
+    assert normwise(p1, ref.psi_1) < 1e-10
+    assert normwise(p2, ref.psi_2) < 1e-10
+    assert rel_err(fen, ref.fenergy) < 1e-12
+
+
+def test_autotune_leaves_state_unchanged(oracle):
+    strings = [small_string(N=20, F=4, seed=47, multicomponent=False),
+               small_string(N=36, F=5, seed=48, multicomponent=True)]
+    say("[gpu] autotune on 2 strings")
+    with solver_for(strings) as sv:
+        sv.nlcgSteps(3)
+        before = [sv.download(i) for i in range(2)] + \
+                 [sv.download_state(i, "d") for i in range(2)] + \
+                 [sv.scalars(i, "g_norm2") for i in range(2)]
+        rows = sv.autotune(steps=2)
+        after = [sv.download(i) for i in range(2)] + \
+                [sv.download_state(i, "d") for i in range(2)] + \
+                [sv.scalars(i, "g_norm2") for i in range(2)]
+        for x, y in zip(before, after):
+            if isinstance(x, tuple):
+                for u, v in zip(x, y):
+                    if u is not None:
+                        np.testing.assert_array_equal(u, v)
+            else:
+                np.testing.assert_array_equal(x, y)
+        assert [sv.info(i)["tile_rows"] for i in range(2)] == rows
+        sv.nlcgSteps(3)
+        got = [sv.download(i)[1] for i in range(2)]
+    for i, s in enumerate(strings):
+        ref = oracle.OracleString(s)
+        ref.nlcg_steps(3)
+        ref.nlcg_steps(3)
+        assert normwise(got[i], ref.psi_1) < 1e-10, i
