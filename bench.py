@@ -39,7 +39,7 @@ KERNEL_NAMES = ((0, "grad_update"), (1, "poly"), (2, "grad_first"), (3, "finish_
                 (4, "finish_poly"))
 
 
-def run_path(strings, precision, steps, warmup, profile_iters=3, segment_rows=0):
+def run_path(strings, precision, steps, warmup, profile_iters=3, segment_rows=0, tune=True):
     from mepgl_amd.solver import Solver
 
     sv = Solver(strings, precision=precision, segment_rows=segment_rows)
@@ -47,6 +47,9 @@ def run_path(strings, precision, steps, warmup, profile_iters=3, segment_rows=0)
     w_iters = max(warmup // R, 1)
     sv.loop(w_iters, R, "L")
     sv.synchronize()
+    # The library's autotuner picks each string class's segment height on
+    # the live problem (the state is restored afterwards).
+    rows = sv.autotune(steps=4) if (tune and not segment_rows) else None
     # Re-start from the initial guess so that both precisions time the same
     # trajectory; warm-up only compiled graphs and touched memory.
     for i, s in enumerate(strings):
@@ -70,31 +73,72 @@ def run_path(strings, precision, steps, warmup, profile_iters=3, segment_rows=0)
         print(f"# {precision} {name}: {1e3 * ms:.2f} us x {cnt}", flush=True)
     sv.profile(False)
     info = sv.info(0)
+    info["rows"] = rows
     sv.close()
     return dict(seconds=dt, steps=iters * R, energies=energies, times=times, info=info)
 
 
-def cpu_baseline(strings, sample_steps):
-    """The FP64 CPU oracle on a bounded sample of the same workload."""
+def run_separately(strings, precision, steps, warmup):
+    """The same strings, one context (and so one launch sequence) each, run
+    one after another: the unbatched reference point for a sweep."""
+    from mepgl_amd.solver import Solver
+
+    iters = max(steps // R, 1)
+    total = 0.0
+    for s in strings:
+        sv = Solver([s], precision=precision)
+        sv.loop(max(warmup // R, 1), R, "L")
+        sv.autotune(steps=2)
+        sv.upload(0, s)
+        sv.reset_solver()
+        sv.synchronize()
+        t0 = time.perf_counter()
+        sv.loop(iters, R, "L")
+        sv.synchronize()
+        total += time.perf_counter() - t0
+        sv.close()
+    return dict(seconds=total, steps=iters * R)
+
+
+def cpu_baseline(strings, seconds):
+    """The FP64 CPU oracle on a bounded sample of the same workload: NLCG
+    steps of string 0 in chunks of 10 until about `seconds` have passed."""
     from oracle import oracle as O
 
     O.build()
     s = strings[0]
     ref = O.OracleString(s)
     ref.nlcg_steps(1)  # warm-up (page-in, thread pool)
+    done = 0
     t0 = time.perf_counter()
-    ref.nlcg_steps(sample_steps)
-    dt = time.perf_counter() - t0
+    while True:
+        ref.nlcg_steps(10)
+        done += 10
+        dt = time.perf_counter() - t0
+        if dt >= seconds:
+            break
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {
-        "value": s.image_cells * sample_steps / dt,
+        "value": s.image_cells * done / dt,
         "unit": "image*cells/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{sample_steps} NLCG steps of string 0 ({s.F} images x {s.N}^2), "
+        "sample": f"{done} NLCG steps ({dt:.1f} s) of string 0 ({s.F} images x {s.N}^2), "
                   f"oracle/mepgl_oracle.c FP64, OpenMP {threads} threads, "
                   f"{platform.processor() or platform.machine()}",
     }
+
+
+def measured_traffic(workload, precision, kernel):
+    """HBM bytes per launch of `kernel` from the calibrated rocprofv3 counter
+    runs recorded in profiles/traffic.json (null when not recorded)."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            table = json.load(f)
+        return table[workload][precision][kernel]
+    except (OSError, KeyError, ValueError):
+        return None
 
 
 def main(argv=None):
@@ -103,7 +147,9 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="vortex_entry_256")
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--compare-batching", action="store_true",
+                    help="multi-string workloads: batched vs one string per context")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="both", choices=("both", "fp64", "mixed"))
     ap.add_argument("--segment-rows", type=int, default=0)
@@ -119,6 +165,22 @@ def main(argv=None):
     cells = sum(s.image_cells for s in strings)
     C_f = [4 if s.no_self_field else 2 + 2 * s.ncomp for s in strings]
     cf = C_f[0]
+
+    if args.compare_batching:
+        steps = args.steps
+        bat = run_path(strings, "mixed", steps, args.warmup)
+        bat64 = run_path(strings, "fp64", steps, args.warmup)
+        sep = run_separately(strings, "mixed", steps, args.warmup)
+        v_b = cells * bat["steps"] / bat["seconds"]
+        print(json.dumps({
+            "workload": args.workload, "strings": len(strings), "image_cells": cells,
+            "batched_mixed": v_b,
+            "batched_fp64": cells * bat64["steps"] / bat64["seconds"],
+            "separate_mixed": cells * sep["steps"] / sep["seconds"],
+            "batched_over_separate": v_b / (cells * sep["steps"] / sep["seconds"]),
+            "mixed_over_fp64": v_b / (cells * bat64["steps"] / bat64["seconds"]),
+            "rows_mixed": bat["info"]["rows"], "unit": "image*cells/s"}))
+        return 0
 
     res = {}
     precs = ("fp64", "mixed") if args.precision == "both" else (args.precision,)
@@ -152,9 +214,10 @@ def main(argv=None):
     kname64, kms64, ach64, per64 = roof(fp64, 8)
     b_m = barrier(mixed["energies"][0])
     b_d = barrier(fp64["energies"][0])
-    rel_b = abs(b_m[0] - b_d[0]) / abs(b_d[0]) if b_d[0] else None
+    rel_b = [abs(m - d) / abs(d) if d else None for m, d in zip(b_m, b_d)]
     e_rel = float(np.max(np.abs(mixed["energies"][0] - fp64["energies"][0]) /
                          np.abs(fp64["energies"][0])))
+    traffic = measured_traffic(args.workload, "mixed", kname)
 
     out = {
         "metric": "image*cells/s per NLCG step (mixed precision)",
@@ -172,21 +235,24 @@ def main(argv=None):
         "config": {"workload": args.workload, "strings": len(strings),
                    "images": sum(s.F for s in strings), "N": strings[0].N,
                    "components": strings[0].ncomp, "mode": "L", "reparam_every": R,
-                   "tile_rows": mixed["info"]["tile_rows"]},
+                   "segment_rows": mixed["info"]["tile_rows"],
+                   "segment_rows_fp64": fp64["info"]["tile_rows"]},
         "fp64": {"value": value64, "ms_per_step": 1e3 * fp64["seconds"] / fp64["steps"],
                  "roofline": {"kernel": kname64, "achieved": ach64, "peak": HBM_PEAK_GBS,
                               "unit": "GB/s", "frac": ach64 / HBM_PEAK_GBS,
                               "avg_ms": kms64}},
         "mixed_over_fp64": value / value64,
         "energy_rel_err_mixed_vs_fp64": e_rel,
-        "barrier_rel_err_mixed_vs_fp64": rel_b,
-        "barrier_fp64": b_d[0],
+        "barrier_fp64": {"entry": b_d[0], "exit": b_d[1]},
+        "barrier_mixed": {"entry": b_m[0], "exit": b_m[1]},
+        "barrier_rel_err_mixed_vs_fp64": {"entry": rel_b[0], "exit": rel_b[1]},
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": ach, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes_per_image_cell": per, "avg_ms": kms},
+                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
+                     "algorithmic_bytes_per_image_cell": per,
+                     "algorithmic_bytes_per_launch": per * cells, "avg_ms": kms},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(strings, args.cpu_steps)
+        out["cpu_baseline"] = cpu_baseline(strings, args.cpu_seconds)
     print(json.dumps(out))
     return 0
 

@@ -2,7 +2,8 @@
 //
 // Data layout in HBM (one allocation set per string):
 //   field sets x[2], d[2], g[2]: [image n][plane p][row j][col i], row pitch
-//   `pitch` elements (a multiple of 64), planes p = 0 A_x, 1 A_y, 2 u_1,
+//   `pitch` elements (a multiple of 64), N + kPlanePadRows rows per plane
+//   (the padding row breaks power-of-two strides), planes p = 0 A_x, 1 A_y, 2 u_1,
 //   3 v_1, 4 u_2, 5 v_2 (2 + 2*ncomp planes).  Column i runs along x, so a
 //   64-lane wavefront reads one contiguous row segment.
 //   nn: uint16 per site, low byte = superconducting-domain neighbour bits,
@@ -23,6 +24,7 @@ constexpr int kLsSteps = 2000;     // line-search scan length (nlcg.cu:1421)
 constexpr int kNumXcd = 8;         // MI355X accelerator complex dies
 constexpr int kMaxImages = 2048;   // images per string (reparametrisation in LDS)
 constexpr int kArcBlock = 256;     // threads of the per-string arclength pass
+constexpr int kPlanePadRows = 1;   // unused rows after each field plane
 
 enum Plane { PAX = 0, PAY = 1, PU1 = 2, PV1 = 3, PU2 = 4, PV2 = 5 };
 

@@ -60,10 +60,11 @@ struct mepgl_ctx {
   LineSearchTable* d_ls = nullptr;
   WorkItem* d_items = nullptr;
   int nitems = 0;
-  // Items sorted by class (components, fixed A); G and P launch once per
-  // non-empty class so each launch runs one specialised kernel.
+  // Items sorted by class (components, fixed A, parameter planes); G and P
+  // launch once per non-empty class so each launch runs one specialised
+  // kernel.
   struct Group {
-    int begin, count, nc, nsf;
+    int begin, count, nc, nsf, pl;
   };
   std::vector<Group> groups;
   double** d_jb = nullptr;
@@ -203,6 +204,7 @@ int run_step(mepgl_ctx* c, const StepArgs& a0, int which) {
     a.nitems = g.count;
     a.nc = g.nc;
     a.nsf = g.nsf;
+    a.pl = g.pl;
     int rc = run_step_group(c, a, which);
     if (rc) return rc;
   }
@@ -334,17 +336,19 @@ int build_items(mepgl_ctx* c) {
   std::vector<WorkItem> items;
   c->groups.clear();
   for (int nc = 1; nc <= 2; ++nc)
-    for (int nsf = 0; nsf <= 1; ++nsf) {
-      const int begin = (int)items.size();
-      for (size_t si = 0; si < c->strings.size(); ++si) {
-        const DevString& s = c->strings[si].dev;
-        if (s.nc != nc || s.nsf != nsf) continue;
-        for (int n = 0; n < s.F; ++n)
-          for (int t = 0; t < s.tiles_x * s.tiles_y; ++t) items.push_back({(int)si, n, t, 0});
+    for (int nsf = 0; nsf <= 1; ++nsf)
+      for (int pl = 0; pl <= 1; ++pl) {
+        const int begin = (int)items.size();
+        for (size_t si = 0; si < c->strings.size(); ++si) {
+          const DevString& s = c->strings[si].dev;
+          const int spl = (s.planes || s.h.p) ? 1 : 0;
+          if (s.nc != nc || s.nsf != nsf || spl != pl) continue;
+          for (int n = 0; n < s.F; ++n)
+            for (int t = 0; t < s.tiles_x * s.tiles_y; ++t) items.push_back({(int)si, n, t, 0});
+        }
+        const int count = (int)items.size() - begin;
+        if (count > 0) c->groups.push_back({begin, count, nc, nsf, pl});
       }
-      const int count = (int)items.size() - begin;
-      if (count > 0) c->groups.push_back({begin, count, nc, nsf});
-    }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   if (c->d_items) hipFree(c->d_items);
   c->d_items = nullptr;
@@ -765,7 +769,11 @@ int mepgl_create(const mepgl_string_desc* strings, int n_strings, int precision,
     d.tiles_x = (in.N + kTileX - 1) / kTileX;
     d.tiles_y = (in.N + d.seg_rows - 1) / d.seg_rows;
     s.ntiles = d.tiles_x * d.tiles_y;
-    d.plane_stride = (long long)in.N * d.pitch;
+    // One padding row per plane: with N a power of two the plane and image
+    // strides would be powers of two, and the dozen streams one wavefront
+    // reads at the same (row, column) of different planes would fall on the
+    // same HBM channels.
+    d.plane_stride = (long long)(in.N + kPlanePadRows) * d.pitch;
     d.image_stride = (long long)d.nplanes * d.plane_stride;
     d.dx = in.dx;
     d.inv_dx = 1.0 / in.dx;

@@ -23,6 +23,8 @@
 // two-sum compensation in the mixed-precision path.
 #include <float.h>
 
+#include <type_traits>
+
 #include "common.hpp"
 #include "kernels.hpp"
 #include "stencil.hpp"
@@ -37,6 +39,18 @@ __device__ __forceinline__ int xcd_item(int b, int nb) {
   const int q = nb / kNumXcd, r = nb % kNumXcd;
   const int k = b % kNumXcd, l = b / kNumXcd;
   return k * q + (k < r ? k : r) + l;
+}
+
+// The work item of this workgroup, made wave-uniform so that the descriptor
+// behind it is read with scalar loads.
+__device__ __forceinline__ WorkItem load_item(const WorkItem* items, int k) {
+  const WorkItem w = items[k];
+  WorkItem u;
+  u.string = __builtin_amdgcn_readfirstlane(w.string);
+  u.image = __builtin_amdgcn_readfirstlane(w.image);
+  u.tile = __builtin_amdgcn_readfirstlane(w.tile);
+  u.pad = 0;
+  return u;
 }
 
 // Pointers read from descriptors are generic; the hot kernels cast them to
@@ -187,6 +201,112 @@ __device__ __forceinline__ double hval(const DevString& s, long long off) {
 }
 
 // ---------------------------------------------------------------------------
+// Buffer access for the G and P kernels.  Each field set of one image is one
+// buffer resource (base in scalar registers); an element is addressed by a
+// 32-bit byte offset of (row, column) in a VGPR plus the plane's byte
+// offset in an SGPR, so no 64-bit address arithmetic runs per load.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+constexpr int kRsrcWord3 = 0x00020000;  // raw untyped buffer (CDNA)
+
+__device__ __forceinline__ Rsrc rsrc_of(const void* base, long long bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0,
+                                           (int)(bytes > 0x7fffffffLL ? 0x7fffffff : bytes),
+                                           kRsrcWord3);
+}
+
+__device__ __forceinline__ double bld(double, Rsrc r, unsigned vo, unsigned so) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+}
+__device__ __forceinline__ float bld(float, Rsrc r, unsigned vo, unsigned so) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+}
+template <class T>
+__device__ __forceinline__ T bload(Rsrc r, unsigned vo, unsigned so) {
+  return bld(T(), r, vo, so);
+}
+// Stores are non-temporal (nt): the next reader is another launch, after
+// far more data than the caches hold has streamed through.
+constexpr int kStoreNT = 2;
+__device__ __forceinline__ void bstore(double v, Rsrc r, unsigned vo, unsigned so) {
+  typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, vo, so, kStoreNT);
+}
+__device__ __forceinline__ void bstore(float v, Rsrc r, unsigned vo, unsigned so) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vo, so, kStoreNT);
+}
+__device__ __forceinline__ uint16_t bload_u16(Rsrc r, unsigned vo) {
+  return __builtin_amdgcn_raw_buffer_load_b16(r, vo, 0, 0);
+}
+
+// Per-site parameters of one row when some parameter is a plane: a, b,
+// m_xx, m_yy per component and the applied field at the five stencil
+// points (centre, +x, -x, +y, -y).
+template <class T, int NC, bool PL>
+struct RowParams {
+  __device__ __forceinline__ void load(const ParamView<NC>&, gptr<const double>, long long, int,
+                                       int, int, const Strip&) {}
+};
+template <class T, int NC>
+struct RowParams<T, NC, true> {
+  double prm[NC][4];
+  T h[5];
+  __device__ __forceinline__ void load(const ParamView<NC>& pv, gptr<const double> hp, long long off,
+                                       int j, int N, int P_, const Strip& t) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      prm[c][0] = param_at<T>(pv.a[c], off);
+      prm[c][1] = param_at<T>(pv.b[c], off);
+      prm[c][2] = param_at<T>(pv.mxx[c], off);
+      prm[c][3] = param_at<T>(pv.myy[c], off);
+    }
+    if (hp) {
+      const int ip = min(t.ic + 1, N - 1), im = max(t.ic - 1, 0);
+      h[0] = (T)hp[off];
+      h[1] = (T)hp[(long long)j * P_ + ip];
+      h[2] = (T)hp[(long long)j * P_ + im];
+      h[3] = (T)hp[(long long)clampr(j + 1, N) * P_ + t.ic];
+      h[4] = (T)hp[(long long)clampr(j - 1, N) * P_ + t.ic];
+    }
+  }
+};
+
+template <class T, int NC>
+__device__ __forceinline__ void comp_from_params(const double (&prm)[NC][4], const double (&q)[NC],
+                                                 CompConst<T> (&k)[NC]) {
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const double a = prm[c][0], b = prm[c][1], mx = prm[c][2], my = prm[c][3];
+    k[c].q = (T)q[c];
+    k[c].two_a = (T)(2.0 * a);
+    k[c].two_b = (T)(2.0 * b);
+    k[c].b = (T)b;
+    k[c].ab = (T)(a / b);
+    k[c].hx = (T)(0.5 / mx);
+    k[c].hy = (T)(0.5 / my);
+  }
+}
+
+// Byte offsets of a strip row: own column and halo column.
+struct RowOff {
+  unsigned own, halo;
+};
+template <class T>
+__device__ __forceinline__ RowOff row_off(int row, int P_, const Strip& t) {
+  return {(unsigned)((row * P_ + t.ic) * (int)sizeof(T)), (unsigned)((row * P_ + t.hc) * (int)sizeof(T))};
+}
+
+// Makes a value live in a register at this point of the program: the
+// compiler may not defer its computation (and the wait for the loads it
+// reads) past the point.
+template <class T>
+__device__ __forceinline__ void pin(T& v) {
+  asm volatile("" : "+v"(v));
+}
+// A compiler barrier for memory operations: later stores are not moved above
+// it (pinned computations before it stay before the stores).
+__device__ __forceinline__ void memory_fence_compiler() { asm volatile("" ::: "memory"); }
+
+// ---------------------------------------------------------------------------
 // G kernel.
 template <class T, int NC>
 struct GRaw {
@@ -203,25 +323,41 @@ struct GRow {
   uint16_t nn;
 };
 
+// Buffers of one G work item.
+struct GBufs {
+  Rsrc x, xo, d, g, go, nn;
+  unsigned ps;  // plane stride in bytes
+};
+
+template <class T, int NC, bool UPD, bool NSF>
+__device__ __forceinline__ void g_load(GRaw<T, NC>& r, const GBufs& b, int row, int P_,
+                                       const Strip& t, unsigned nnoff) {
+  const RowOff o = row_off<T>(row, P_, t);
+#pragma unroll
+  for (int p = 0; p < GRaw<T, NC>::P; ++p) {
+    r.x[p] = bload<T>(b.x, o.own, p * b.ps);
+    r.d[p] = (UPD && !(NSF && p < 2)) ? bload<T>(b.d, o.own, p * b.ps) : (T)0;
+  }
+  // Halo columns: only lanes 0 and 63 read the columns outside the strip
+  // (exec-masked loads); the other lanes' halo values are never used.
+#pragma unroll
+  for (int p = 0; p < GRaw<T, NC>::P; ++p) r.hx[p] = r.hd[p] = (T)0;
+  if (t.edge) {
+#pragma unroll
+    for (int p = 0; p < GRaw<T, NC>::P; ++p) {
+      r.hx[p] = bload<T>(b.x, o.halo, p * b.ps);
+      r.hd[p] = (UPD && !(NSF && p < 2)) ? bload<T>(b.d, o.halo, p * b.ps) : (T)0;
+    }
+  }
+  r.nn = bload_u16(b.nn, nnoff);
+}
+
 template <class T, int NC>
-__device__ __forceinline__ void g_load(GRaw<T, NC>& r, gptr<const T> __restrict__ X,
-                                       gptr<const T> __restrict__ D, long long ps, int P_, int row,
-                                       const Strip& t, bool upd, bool nsf,
-                                       gptr<const uint16_t> __restrict__ nn) {
-  const long long off = (long long)row * P_ + t.ic, hoff = (long long)row * P_ + t.hc;
+__device__ __forceinline__ void pin_row(GRow<T, NC>& w) {
+  pin(w.ax), pin(w.ay), pin(w.axl), pin(w.axr), pin(w.ayl), pin(w.ayr);
 #pragma unroll
-  for (int p = 0; p < GRaw<T, NC>::P; ++p) {
-    r.x[p] = X[p * ps + off];
-    r.d[p] = (upd && !(nsf && p < 2)) ? D[p * ps + off] : (T)0;
-  }
-  // Halo columns: lanes 0 and 63 read the columns outside the strip, the
-  // other lanes re-read their own column (an L1 hit) so no lane branches.
-#pragma unroll
-  for (int p = 0; p < GRaw<T, NC>::P; ++p) {
-    r.hx[p] = X[p * ps + hoff];
-    r.hd[p] = (upd && !(nsf && p < 2)) ? D[p * ps + hoff] : (T)0;
-  }
-  r.nn = nn[off];
+  for (int q = 0; q < NC; ++q) pin(w.u[q]), pin(w.v[q]), pin(w.eu[q]), pin(w.ev[q]);
+  pin(w.nn);
 }
 
 template <class T, int NC>
@@ -248,60 +384,106 @@ __device__ __forceinline__ void g_process(GRow<T, NC>& w, const GRaw<T, NC>& r, 
   w.nn = r.nn;
 }
 
-template <class T, int NC, bool UPD, bool NSF>
+// Own-site values of a row that only the row's own computation needs: the
+// old gradient (Polak-Ribiere), the frozen psi gradient (Maxwell mode) and,
+// with parameter planes, the row's parameters.
+template <class T, int NC, bool PL>
+struct GSide {
+  static constexpr int P = 2 + 2 * NC;
+  T go[P];
+  T gst[NC][2];
+  RowParams<T, NC, PL> rp;
+};
+
+// The old gradient is read unconditionally (the diagnostic entry point that
+// does not need it still reads a valid buffer), so the batch has no
+// data-dependent branches.
+template <class T, int NC, bool NSF, bool PL, bool MW>
+__device__ __forceinline__ void g_side_load(GSide<T, NC, PL>& sd, const GBufs& b, int row, int P_,
+                                            const Strip& t, const ParamView<NC>& pv,
+                                            gptr<const double> hp, int N) {
+  const unsigned o = (unsigned)((row * P_ + t.ic) * (int)sizeof(T));
+#pragma unroll
+  for (int p = 0; p < GSide<T, NC, PL>::P; ++p)
+    sd.go[p] = !(NSF && p < 2) ? bload<T>(b.go, o, p * b.ps) : (T)0;
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    sd.gst[q][0] = MW ? bload<T>(b.g, o, (PU1 + 2 * q) * b.ps) : (T)0;
+    sd.gst[q][1] = MW ? bload<T>(b.g, o, (PV1 + 2 * q) * b.ps) : (T)0;
+  }
+  sd.rp.load(pv, hp, (long long)row * P_ + t.ic, row, N, P_, t);
+}
+
+template <class T, int NC, bool PL>
+__device__ __forceinline__ void pin_side(GSide<T, NC, PL>& sd) {
+#pragma unroll
+  for (int p = 0; p < GSide<T, NC, PL>::P; ++p) pin(sd.go[p]);
+#pragma unroll
+  for (int q = 0; q < NC; ++q) pin(sd.gst[q][0]), pin(sd.gst[q][1]);
+  if constexpr (PL) {
+#pragma unroll
+    for (int q = 0; q < NC; ++q)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) pin(sd.rp.prm[q][k]);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) pin(sd.rp.h[k]);
+  }
+}
+
+template <class T, int NC, bool UPD, bool NSF, bool PL, bool MW>
 __device__ void grad_body(const StepArgs& a, const DevString& s, const WorkItem& it) {
   constexpr bool COMP = sizeof(T) == 4;
+  constexpr int NP = GRaw<T, NC>::P;
   const Strip t = strip_of(s, it);
   const int N = s.N, P_ = s.pitch;
-  const long long ps = s.plane_stride;
+  const long long ibytes = s.image_stride * (long long)sizeof(T);
   const long long ib = (long long)t.n * s.image_stride;
-  gptr<const T> __restrict__ X = as_global((const T*)s.x[a.xin] + ib);
-  gptr<T> __restrict__ XO = as_global((T*)s.x[a.xout] + ib);
-  gptr<const T> __restrict__ D = as_global((const T*)s.d[a.din] + ib);
-  gptr<T> __restrict__ G = as_global((T*)(a.scratch ? s.gs : s.g[a.gcur]) + ib);
-  gptr<const T> __restrict__ GO = as_global((const T*)s.g[a.gold] + ib);
-  gptr<const uint16_t> __restrict__ NN = as_global(s.nn);
-  ImageScalars& sc = a.sc[s.img0 + t.n];
-  constexpr bool upd = UPD;
-  const T al = upd ? (T)sc.alpha : (T)0;
-  constexpr bool nsf = NSF;
+  GBufs b;
+  b.x = rsrc_of((const T*)s.x[a.xin] + ib, ibytes);
+  b.xo = rsrc_of((const T*)s.x[a.xout] + ib, ibytes);
+  b.d = rsrc_of((const T*)s.d[a.din] + ib, ibytes);
+  b.g = rsrc_of((const T*)(a.scratch ? s.gs : s.g[a.gcur]) + ib, ibytes);
+  b.go = rsrc_of((const T*)s.g[a.gold] + ib, ibytes);
+  b.nn = rsrc_of(s.nn, (long long)s.N * s.pitch * (long long)sizeof(uint16_t));
+  b.ps = (unsigned)(s.plane_stride * (long long)sizeof(T));
+  const ImageScalars& sc = a.sc[s.img0 + t.n];
+  const T al = UPD ? (T)sc.alpha : (T)0;
   constexpr bool do_a = !NSF;
-  const bool do_psi = !a.maxwell;
+  const bool want_go = !a.scratch;
   const bool store = t.i < N;
   const T idx = (T)s.inv_dx, dx2 = (T)s.dx2;
   const T eta = (T)s.eta, gam = (T)s.gamma, del = (T)s.delta;
   CompConst<T> kconst[NC];
-  if (!s.planes) load_comp_const<T, NC>(s, 0, kconst);
+  if (!PL) load_comp_const<T, NC>(s, 0, kconst);
+  const ParamView<NC> pv = param_view<NC>(s);
+  gptr<const double> __restrict__ hp = as_global(s.h.p);
+  const T hv = (T)s.h.v;
   double acc_s[2] = {0, 0}, acc_c[2] = {0, 0};
+  auto nnoff = [&](int row) { return (unsigned)((row * P_ + t.ic) * (int)sizeof(uint16_t)); };
 
   GRaw<T, NC> raw;
   GRow<T, NC> wp, wc, wn;
-  g_load<T, NC>(raw, X, D, ps, P_, clampr(t.j0 - 1, N), t, upd, nsf, NN);
-  g_process<T, NC>(wp, raw, al, upd);
-  g_load<T, NC>(raw, X, D, ps, P_, clampr(t.j0, N), t, upd, nsf, NN);
-  g_process<T, NC>(wc, raw, al, upd);
-  g_load<T, NC>(raw, X, D, ps, P_, clampr(t.j0 + 1, N), t, upd, nsf, NN);
-  g_process<T, NC>(wn, raw, al, upd);
+  GSide<T, NC, PL> sc_cur, sc_next;
+  g_load<T, NC, UPD, NSF>(raw, b, clampr(t.j0 - 1, N), P_, t, nnoff(clampr(t.j0 - 1, N)));
+  g_process<T, NC>(wp, raw, al, UPD);
+  g_load<T, NC, UPD, NSF>(raw, b, clampr(t.j0, N), P_, t, nnoff(clampr(t.j0, N)));
+  g_process<T, NC>(wc, raw, al, UPD);
+  g_load<T, NC, UPD, NSF>(raw, b, clampr(t.j0 + 1, N), P_, t, nnoff(clampr(t.j0 + 1, N)));
+  g_process<T, NC>(wn, raw, al, UPD);
+  g_side_load<T, NC, NSF, PL, MW>(sc_cur, b, t.j0, P_, t, pv, hp, N);
 
-#pragma clang loop unroll(disable)
-  for (int j = t.j0; j < t.j1; ++j) {
-    // Prefetch row j + 2 while row j is computed.
-    const bool more = j + 1 < t.j1;
-    if (more) g_load<T, NC>(raw, X, D, ps, P_, clampr(j + 2, N), t, upd, nsf, NN);
-    const long long off = (long long)j * P_ + t.ic;
-    T go[GRaw<T, NC>::P];
-    if (!a.scratch) {
-#pragma unroll
-      for (int p = 0; p < GRaw<T, NC>::P; ++p) go[p] = (do_a || p >= 2) ? GO[p * ps + off] : (T)0;
+  // Row j: issue one batch of loads (the fields of row j + 2, the own-site
+  // values of row j + 1), compute row j from registers, turn the batch into
+  // window form, then store row j.  Every load is consumed before this
+  // row's stores are issued, and the last row (no batch) is a separate copy
+  // of the body, so the loads in flight at each point are known exactly.
+  auto row = [&](int j, auto batch_tag) {
+    constexpr bool BATCH = decltype(batch_tag)::value;
+    if constexpr (BATCH) {
+      g_load<T, NC, UPD, NSF>(raw, b, clampr(j + 2, N), P_, t, nnoff(clampr(j + 2, N)));
+      g_side_load<T, NC, NSF, PL, MW>(sc_next, b, j + 1, P_, t, pv, hp, N);
     }
-    T gstale[NC][2];
-    if (!do_psi) {
-#pragma unroll
-      for (int q = 0; q < NC; ++q) {
-        gstale[q][0] = G[(PU1 + 2 * q) * ps + off];
-        gstale[q][1] = G[(PV1 + 2 * q) * ps + off];
-      }
-    }
+    __builtin_amdgcn_sched_barrier(0);
 
     Nbhd<T, NC> nb;
     nb.ax[0][0] = wp.axl; nb.ax[0][1] = wp.ax; nb.ax[0][2] = wp.axr;
@@ -323,95 +505,105 @@ __device__ void grad_body(const StepArgs& a, const DevString& s, const WorkItem&
       nb.u[q][4] = wp.u[q];
       nb.v[q][4] = wp.v[q];
     }
-    if (s.h.p) {
-      const int ip = min(t.ic + 1, N - 1), im = max(t.ic - 1, 0);
-      nb.h[0] = (T)s.h.p[off];
-      nb.h[1] = (T)s.h.p[(long long)j * P_ + ip];
-      nb.h[2] = (T)s.h.p[(long long)j * P_ + im];
-      nb.h[3] = (T)s.h.p[(long long)clampr(j + 1, N) * P_ + t.ic];
-      nb.h[4] = (T)s.h.p[(long long)clampr(j - 1, N) * P_ + t.ic];
+    CompConst<T> k[NC];
+    if constexpr (PL) {
+      comp_from_params<T, NC>(sc_cur.rp.prm, pv.q, k);
+#pragma unroll
+      for (int q = 0; q < 5; ++q) nb.h[q] = hp ? sc_cur.rp.h[q] : hv;
     } else {
 #pragma unroll
-      for (int q = 0; q < 5; ++q) nb.h[q] = (T)s.h.v;
-    }
-    CompConst<T> k[NC];
-    if (s.planes)
-      load_comp_const<T, NC>(s, off, k);
-    else
-#pragma unroll
       for (int q = 0; q < NC; ++q) k[q] = kconst[q];
+#pragma unroll
+      for (int q = 0; q < 5; ++q) nb.h[q] = hv;
+    }
 
     T gp[NC][2], ga[2] = {0, 0};
-    site_gradient<T, NC>(nb, k, eta, gam, del, idx, dx2, wc.nn & 0xff, wc.nn >> 8, do_psi, do_a,
+    site_gradient<T, NC>(nb, k, eta, gam, del, idx, dx2, wc.nn & 0xff, wc.nn >> 8, !MW, do_a,
                          gp, ga);
-    if (!do_psi) {
+    if constexpr (MW) {
 #pragma unroll
       for (int q = 0; q < NC; ++q) {
-        gp[q][0] = gstale[q][0];
-        gp[q][1] = gstale[q][1];
+        gp[q][0] = sc_cur.gst[q][0];
+        gp[q][1] = sc_cur.gst[q][1];
       }
     }
+    if (store && want_go) {
+      // gradientDensity, nlcg.cu:919-949
+      double n2 = 0, dg = 0;
+      if (do_a) {
+#pragma unroll
+        for (int q = 0; q < 2; ++q) {
+          const double gv = (double)ga[q], go_ = (double)sc_cur.go[q];
+          n2 += gv * gv;
+          dg += gv * (gv - go_);
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < NC; ++q) {
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          const double gv = (double)gp[q][r], go_ = (double)sc_cur.go[PU1 + 2 * q + r];
+          n2 += gv * gv;
+          dg += gv * (gv - go_);
+        }
+      }
+      acc<COMP>(acc_s[0], acc_c[0], n2);
+      acc<COMP>(acc_s[1], acc_c[1], dg);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    GRow<T, NC> wnn;
+    if constexpr (BATCH) {
+      g_process<T, NC>(wnn, raw, al, UPD);
+      sc_cur = sc_next;
+      pin_row<T, NC>(wnn);
+      pin_side<T, NC, PL>(sc_cur);
+      memory_fence_compiler();
+    }
+    __builtin_amdgcn_sched_barrier(0);
     if (store) {
-      if (do_psi) {
+      const unsigned o = (unsigned)((j * P_ + t.ic) * (int)sizeof(T));
+      if constexpr (!MW) {
 #pragma unroll
         for (int q = 0; q < NC; ++q) {
-          G[(PU1 + 2 * q) * ps + off] = gp[q][0];
-          G[(PV1 + 2 * q) * ps + off] = gp[q][1];
+          bstore(gp[q][0], b.g, o, (PU1 + 2 * q) * b.ps);
+          bstore(gp[q][1], b.g, o, (PV1 + 2 * q) * b.ps);
         }
       }
       if (do_a) {
-        G[PAX * ps + off] = ga[0];
-        G[PAY * ps + off] = ga[1];
+        bstore(ga[0], b.g, o, PAX * b.ps);
+        bstore(ga[1], b.g, o, PAY * b.ps);
       }
-      if (upd) {
+      if (UPD) {
         if (do_a) {
-          XO[PAX * ps + off] = wc.ax;
-          XO[PAY * ps + off] = wc.ay;
+          bstore(wc.ax, b.xo, o, PAX * b.ps);
+          bstore(wc.ay, b.xo, o, PAY * b.ps);
         }
 #pragma unroll
         for (int q = 0; q < NC; ++q) {
-          XO[(PU1 + 2 * q) * ps + off] = wc.u[q];
-          XO[(PV1 + 2 * q) * ps + off] = wc.v[q];
+          bstore(wc.u[q], b.xo, o, (PU1 + 2 * q) * b.ps);
+          bstore(wc.v[q], b.xo, o, (PV1 + 2 * q) * b.ps);
         }
-      }
-      if (!a.scratch) {
-        // gradientDensity, nlcg.cu:919-949
-        double n2 = 0, dg = 0;
-        if (do_a) {
-#pragma unroll
-          for (int q = 0; q < 2; ++q) {
-            const double gv = (double)ga[q], go_ = (double)go[q];
-            n2 += gv * gv;
-            dg += gv * (gv - go_);
-          }
-        }
-#pragma unroll
-        for (int q = 0; q < NC; ++q) {
-#pragma unroll
-          for (int r = 0; r < 2; ++r) {
-            const double gv = (double)gp[q][r], go_ = (double)go[PU1 + 2 * q + r];
-            n2 += gv * gv;
-            dg += gv * (gv - go_);
-          }
-        }
-        acc<COMP>(acc_s[0], acc_c[0], n2);
-        acc<COMP>(acc_s[1], acc_c[1], dg);
       }
     }
     // Advance the window.
-    wp = wc;
-    wc = wn;
-    if (more) g_process<T, NC>(wn, raw, al, upd);
-  }
+    if constexpr (BATCH) {
+      wp = wc;
+      wc = wn;
+      wn = wnn;
+    }
+  };
+#pragma clang loop unroll(disable)
+  for (int j = t.j0; j < t.j1 - 1; ++j) row(j, std::true_type());
+  row(t.j1 - 1, std::false_type());
   if (a.scratch) return;
 
   publish_partial<2, COMP>(acc_s, acc_c, s.part_g, t.n, t.tile, s.tiles_x * s.tiles_y);
 }
 
-template <class T, int NC, bool UPD, bool NSF>
+template <class T, int NC, bool UPD, bool NSF, bool PL, bool MW>
 __global__ __launch_bounds__(kBlock) void k_grad(StepArgs a) {
-  const WorkItem it = a.items[xcd_item(blockIdx.x, a.nitems)];
-  grad_body<T, NC, UPD, NSF>(a, a.strings[it.string], it);
+  const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
+  grad_body<T, NC, UPD, NSF, PL, MW>(a, a.strings[it.string], it);
 }
 
 // ---------------------------------------------------------------------------
@@ -432,30 +624,48 @@ struct PRow {
   uint16_t nn;
 };
 
-template <class T, int NC>
-__device__ __forceinline__ void p_load(PRaw<T, NC>& r, gptr<const T> __restrict__ X,
-                                       gptr<const T> __restrict__ G, gptr<const T> __restrict__ D,
-                                       long long ps, int P_, int row, const Strip& t, bool nsf,
-                                       gptr<const uint16_t> __restrict__ nn) {
-  const long long off = (long long)row * P_ + t.ic, hoff = (long long)row * P_ + t.hc;
+struct PBufs {
+  Rsrc x, g, d, dout, nn;
+  unsigned ps;
+};
+
+template <class T, int NC, bool NSF>
+__device__ __forceinline__ void p_load(PRaw<T, NC>& r, const PBufs& b, int row, int P_,
+                                       const Strip& t, unsigned nnoff) {
+  const RowOff o = row_off<T>(row, P_, t);
 #pragma unroll
   for (int p = 0; p < PRaw<T, NC>::P; ++p) {
-    const bool dir = !(nsf && p < 2);
-    r.x[p] = X[p * ps + off];
-    r.g[p] = dir ? G[p * ps + off] : (T)0;
-    r.d[p] = dir ? D[p * ps + off] : (T)0;
+    const bool dir = !(NSF && p < 2);
+    r.x[p] = bload<T>(b.x, o.own, p * b.ps);
+    r.g[p] = dir ? bload<T>(b.g, o.own, p * b.ps) : (T)0;
+    r.d[p] = dir ? bload<T>(b.d, o.own, p * b.ps) : (T)0;
   }
 #pragma unroll
-  for (int p = 0; p < PRaw<T, NC>::P; ++p) {
-    const bool dir = !(nsf && p < 2);
-    r.hx[p] = X[p * ps + hoff];
-    r.hg[p] = dir ? G[p * ps + hoff] : (T)0;
-    r.hd[p] = dir ? D[p * ps + hoff] : (T)0;
+  for (int p = 0; p < PRaw<T, NC>::P; ++p) r.hx[p] = r.hg[p] = r.hd[p] = (T)0;
+  if (t.edge) {
+#pragma unroll
+    for (int p = 0; p < PRaw<T, NC>::P; ++p) {
+      const bool dir = !(NSF && p < 2);
+      r.hx[p] = bload<T>(b.x, o.halo, p * b.ps);
+      r.hg[p] = dir ? bload<T>(b.g, o.halo, p * b.ps) : (T)0;
+      r.hd[p] = dir ? bload<T>(b.d, o.halo, p * b.ps) : (T)0;
+    }
   }
-  r.nn = nn[off];
+  r.nn = bload_u16(b.nn, nnoff);
 }
 
 // d = -g + beta d  (computeStepDirection, nlcg.cu:985-993)
+template <class T, int NC>
+__device__ __forceinline__ void pin_row(PRow<T, NC>& w) {
+  pin(w.ax), pin(w.ay), pin(w.dax), pin(w.day), pin(w.eax), pin(w.eay), pin(w.edax), pin(w.eday);
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    pin(w.u[q]), pin(w.v[q]), pin(w.du[q]), pin(w.dv[q]);
+    pin(w.eu[q]), pin(w.ev[q]), pin(w.edu[q]), pin(w.edv[q]);
+  }
+  pin(w.nn);
+}
+
 template <class T, int NC>
 __device__ __forceinline__ void p_process(PRow<T, NC>& w, const PRaw<T, NC>& r, T beta) {
   T dn[PRaw<T, NC>::P], en[PRaw<T, NC>::P];
@@ -486,19 +696,21 @@ __device__ __forceinline__ void p_process(PRow<T, NC>& w, const PRaw<T, NC>& r, 
   w.nn = r.nn;
 }
 
-template <class T, int NC, bool NSF>
+template <class T, int NC, bool NSF, bool PL>
 __device__ void poly_body(const StepArgs& a, const DevString& s, const WorkItem& it) {
   constexpr bool COMP = sizeof(T) == 4;
   const Strip t = strip_of(s, it);
   const int N = s.N, P_ = s.pitch;
-  const long long ps = s.plane_stride;
+  const long long ibytes = s.image_stride * (long long)sizeof(T);
   const long long ib = (long long)t.n * s.image_stride;
-  gptr<const T> __restrict__ X = as_global((const T*)s.x[a.xin] + ib);
-  gptr<const T> __restrict__ G = as_global((const T*)s.g[a.gcur] + ib);
-  gptr<const T> __restrict__ D = as_global((const T*)s.d[a.din] + ib);
-  gptr<T> __restrict__ DO = as_global((T*)s.d[a.dout] + ib);
-  gptr<const uint16_t> __restrict__ NN = as_global(s.nn);
-  ImageScalars& sc = a.sc[s.img0 + t.n];
+  PBufs b;
+  b.x = rsrc_of((const T*)s.x[a.xin] + ib, ibytes);
+  b.g = rsrc_of((const T*)s.g[a.gcur] + ib, ibytes);
+  b.d = rsrc_of((const T*)s.d[a.din] + ib, ibytes);
+  b.dout = rsrc_of((const T*)s.d[a.dout] + ib, ibytes);
+  b.nn = rsrc_of(s.nn, (long long)s.N * s.pitch * (long long)sizeof(uint16_t));
+  b.ps = (unsigned)(s.plane_stride * (long long)sizeof(T));
+  const ImageScalars& sc = a.sc[s.img0 + t.n];
   const T beta = (T)sc.beta;
   constexpr bool nsf = NSF;
   const bool exact = s.poly_exact;
@@ -506,26 +718,38 @@ __device__ void poly_body(const StepArgs& a, const DevString& s, const WorkItem&
   const T idx = (T)s.inv_dx, dx2 = (T)s.dx2;
   const T eta = (T)s.eta, gam = (T)s.gamma, del = (T)s.delta;
   CompConst<T> kconst[NC];
-  if (!s.planes) load_comp_const<T, NC>(s, 0, kconst);
+  if (!PL) load_comp_const<T, NC>(s, 0, kconst);
+  const ParamView<NC> pv = param_view<NC>(s);
+  gptr<const double> __restrict__ hp = as_global(s.h.p);
+  const T hv = (T)s.h.v;
   double acc_s[5] = {0, 0, 0, 0, 0}, acc_c[5] = {0, 0, 0, 0, 0};
+  auto nnoff = [&](int row) { return (unsigned)((row * P_ + t.ic) * (int)sizeof(uint16_t)); };
 
   PRaw<T, NC> raw;
   PRow<T, NC> wc, wn;
+  RowParams<T, NC, PL> rp_cur, rp_next;
   T pax, pdax;  // row j-1: A_x and its direction
-  p_load<T, NC>(raw, X, G, D, ps, P_, clampr(t.j0 - 1, N), t, nsf, NN);
+  p_load<T, NC, NSF>(raw, b, clampr(t.j0 - 1, N), P_, t, nnoff(clampr(t.j0 - 1, N)));
   p_process<T, NC>(wc, raw, beta);
   pax = wc.ax;
   pdax = wc.dax;
-  p_load<T, NC>(raw, X, G, D, ps, P_, clampr(t.j0, N), t, nsf, NN);
+  p_load<T, NC, NSF>(raw, b, clampr(t.j0, N), P_, t, nnoff(clampr(t.j0, N)));
   p_process<T, NC>(wc, raw, beta);
-  p_load<T, NC>(raw, X, G, D, ps, P_, clampr(t.j0 + 1, N), t, nsf, NN);
+  p_load<T, NC, NSF>(raw, b, clampr(t.j0 + 1, N), P_, t, nnoff(clampr(t.j0 + 1, N)));
   p_process<T, NC>(wn, raw, beta);
+  rp_cur.load(pv, hp, (long long)t.j0 * P_ + t.ic, t.j0, N, P_, t);
 
-#pragma clang loop unroll(disable)
-  for (int j = t.j0; j < t.j1; ++j) {
-    const bool more = j + 1 < t.j1;
-    if (more) p_load<T, NC>(raw, X, G, D, ps, P_, clampr(j + 2, N), t, nsf, NN);
-    const long long off = (long long)j * P_ + t.ic;
+  // Row j: one batch of loads (row j + 2; parameters of row j + 1), the
+  // quartic of row j from registers, the batch into window form, then the
+  // stores of row j; the last row, without a batch, is a separate copy of
+  // the body (see the G loop).
+  auto row = [&](int j, auto batch_tag) {
+    constexpr bool BATCH = decltype(batch_tag)::value;
+    if constexpr (BATCH) {
+      p_load<T, NC, NSF>(raw, b, clampr(j + 2, N), P_, t, nnoff(clampr(j + 2, N)));
+      rp_next.load(pv, hp, (long long)(j + 1) * P_ + t.ic, j + 1, N, P_, t);
+    }
+    __builtin_amdgcn_sched_barrier(0);
 
     PolyNbhd<T, NC> nb;
     nb.ax[0] = wc.ax;
@@ -563,42 +787,69 @@ __device__ void poly_body(const StepArgs& a, const DevString& s, const WorkItem&
       nb.du[q][2] = wn.du[q];
       nb.dv[q][2] = wn.dv[q];
     }
-    nb.h0 = (T)(s.h.p ? s.h.p[off] : s.h.v);
     CompConst<T> k[NC];
-    if (s.planes)
-      load_comp_const<T, NC>(s, off, k);
-    else
+    if constexpr (PL) {
+      comp_from_params<T, NC>(rp_cur.prm, pv.q, k);
+      nb.h0 = hp ? rp_cur.h[0] : hv;
+    } else {
 #pragma unroll
       for (int q = 0; q < NC; ++q) k[q] = kconst[q];
+      nb.h0 = hv;
+    }
 
     if (store) {
-      if (!nsf) {
-        DO[PAX * ps + off] = wc.dax;
-        DO[PAY * ps + off] = wc.day;
-      }
-#pragma unroll
-      for (int q = 0; q < NC; ++q) {
-        DO[(PU1 + 2 * q) * ps + off] = wc.du[q];
-        DO[(PV1 + 2 * q) * ps + off] = wc.dv[q];
-      }
       const Quartic<T> qv =
           site_poly<T, NC>(nb, k, eta, gam, del, idx, dx2, wc.nn & 0xff, wc.nn >> 8, nsf, exact);
 #pragma unroll
       for (int r = 0; r < 5; ++r) acc<COMP>(acc_s[r], acc_c[r], (double)qv.c[r]);
     }
-    pax = wc.ax;
-    pdax = wc.dax;
-    wc = wn;
-    if (more) p_process<T, NC>(wn, raw, beta);
-  }
+    __builtin_amdgcn_sched_barrier(0);
+    PRow<T, NC> wnn;
+    if constexpr (BATCH) {
+      p_process<T, NC>(wnn, raw, beta);
+      rp_cur = rp_next;
+      pin_row<T, NC>(wnn);
+      if constexpr (PL) {
+#pragma unroll
+        for (int q = 0; q < NC; ++q)
+#pragma unroll
+          for (int k = 0; k < 4; ++k) pin(rp_cur.prm[q][k]);
+#pragma unroll
+        for (int k = 0; k < 5; ++k) pin(rp_cur.h[k]);
+      }
+      memory_fence_compiler();
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    if (store) {
+      const unsigned o = (unsigned)((j * P_ + t.ic) * (int)sizeof(T));
+      if (!nsf) {
+        bstore(wc.dax, b.dout, o, PAX * b.ps);
+        bstore(wc.day, b.dout, o, PAY * b.ps);
+      }
+#pragma unroll
+      for (int q = 0; q < NC; ++q) {
+        bstore(wc.du[q], b.dout, o, (PU1 + 2 * q) * b.ps);
+        bstore(wc.dv[q], b.dout, o, (PV1 + 2 * q) * b.ps);
+      }
+    }
+    if constexpr (BATCH) {
+      pax = wc.ax;
+      pdax = wc.dax;
+      wc = wn;
+      wn = wnn;
+    }
+  };
+#pragma clang loop unroll(disable)
+  for (int j = t.j0; j < t.j1 - 1; ++j) row(j, std::true_type());
+  row(t.j1 - 1, std::false_type());
 
   publish_partial<5, COMP>(acc_s, acc_c, s.part_p, t.n, t.tile, s.tiles_x * s.tiles_y);
 }
 
-template <class T, int NC, bool NSF>
+template <class T, int NC, bool NSF, bool PL>
 __global__ __launch_bounds__(kBlock) void k_poly(StepArgs a) {
-  const WorkItem it = a.items[xcd_item(blockIdx.x, a.nitems)];
-  poly_body<T, NC, NSF>(a, a.strings[it.string], it);
+  const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
+  poly_body<T, NC, NSF, PL>(a, a.strings[it.string], it);
 }
 
 // ---------------------------------------------------------------------------
@@ -716,7 +967,7 @@ __global__ __launch_bounds__(kBlock) void k_finish_arclen(FinishArgs f) {
 // before anything other than the next gradient reads the fields.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_update(StepArgs a) {
-  const WorkItem it = a.items[xcd_item(blockIdx.x, a.nitems)];
+  const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
   const DevString& s = a.strings[it.string];
   const Strip t = strip_of(s, it);
   if (t.i >= s.N) return;
@@ -797,7 +1048,7 @@ __device__ void energy_body(const AuxArgs& a, const DevString& s, const WorkItem
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_energy(AuxArgs a) {
-  const WorkItem it = a.items[xcd_item(blockIdx.x, a.nitems)];
+  const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
   const DevString& s = a.strings[it.string];
   if (s.nc == 2)
     energy_body<T, 2>(a, s, it);
@@ -834,7 +1085,7 @@ __device__ void jb_body(const AuxArgs& a, const DevString& s, const WorkItem& it
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_jb(AuxArgs a) {
-  const WorkItem it = a.items[xcd_item(blockIdx.x, a.nitems)];
+  const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
   const DevString& s = a.strings[it.string];
   if (s.nc == 2)
     jb_body<T, 2>(a, s, it);
@@ -885,7 +1136,7 @@ __device__ void arclen_body(const AuxArgs& a, const DevString& s, const WorkItem
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_arclen(AuxArgs a) {
-  const WorkItem it = a.items[xcd_item(blockIdx.x, a.nitems)];
+  const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
   const DevString& s = a.strings[it.string];
   if (s.nc == 2)
     arclen_body<T, 2>(a, s, it);
@@ -985,7 +1236,7 @@ __global__ __launch_bounds__(kArcBlock) void k_arc_finalize(AuxArgs a, int nstri
 // string: image n of the output is w x[k] + (1 - w) x[k-1].
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_redistribute_linear(AuxArgs a) {
-  const WorkItem it = a.items[xcd_item(blockIdx.x, a.nitems)];
+  const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
   const DevString& s = a.strings[it.string];
   const Strip t = strip_of(s, it);
   if (t.i >= s.N) return;
@@ -1007,7 +1258,7 @@ __global__ __launch_bounds__(kBlock) void k_redistribute_linear(AuxArgs a) {
 // buffer xin after a redistribution, so both field buffers hold the same A.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_copy_a(AuxArgs a) {
-  const WorkItem it = a.items[xcd_item(blockIdx.x, a.nitems)];
+  const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
   const DevString& s = a.strings[it.string];
   if (!s.nsf) return;
   const Strip t = strip_of(s, it);
@@ -1029,7 +1280,7 @@ __global__ __launch_bounds__(kBlock) void k_copy_a(AuxArgs a) {
 // (w-1)^3-(w-1) weight on sigma_{k-1}; 1 uses (1-w)^3-(1-w).
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_redistribute_cubic(AuxArgs a, int textbook) {
-  const WorkItem it = a.items[xcd_item(blockIdx.x, a.nitems)];
+  const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
   const DevString& s = a.strings[it.string];
   const Strip t = strip_of(s, it);
   // The solve runs along the image axis: only the image-0 items of the
@@ -1104,7 +1355,7 @@ __device__ void align_body(const AuxArgs& a, const DevString& s, const WorkItem&
 
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_align(AuxArgs a) {
-  const WorkItem it = a.items[xcd_item(blockIdx.x, a.nitems)];
+  const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
   const DevString& s = a.strings[it.string];
   if (s.nc == 2)
     align_body<T, 2>(a, s, it);
@@ -1126,7 +1377,7 @@ __global__ void k_align_finalize(AuxArgs a, int nstrings) {
 // gradient buffers (the energy and the NLCG inner products are invariant).
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_align_apply(AuxArgs a) {
-  const WorkItem it = a.items[xcd_item(blockIdx.x, a.nitems)];
+  const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
   const DevString& s = a.strings[it.string];
   const Strip t = strip_of(s, it);
   if (t.n == 0 || t.i >= s.N) return;
@@ -1206,14 +1457,27 @@ __global__ __launch_bounds__(256) void k_fill(T* p, long long n, T v) {
 
 // ---------------------------------------------------------------------------
 // Host launchers.
+template <class T, int NC, bool NSF, bool PL>
+static void launch_gp_pl(const StepArgs& a, int which, const dim3& g, const dim3& b,
+                         hipStream_t st) {
+  if (which == kKernelPoly)
+    hipLaunchKernelGGL((k_poly<T, NC, NSF, PL>), g, b, 0, st, a);
+  else if (a.maxwell && a.upd)
+    hipLaunchKernelGGL((k_grad<T, NC, true, NSF, PL, true>), g, b, 0, st, a);
+  else if (a.maxwell)
+    hipLaunchKernelGGL((k_grad<T, NC, false, NSF, PL, true>), g, b, 0, st, a);
+  else if (a.upd)
+    hipLaunchKernelGGL((k_grad<T, NC, true, NSF, PL, false>), g, b, 0, st, a);
+  else
+    hipLaunchKernelGGL((k_grad<T, NC, false, NSF, PL, false>), g, b, 0, st, a);
+}
+
 template <class T, int NC, bool NSF>
 static void launch_gp(const StepArgs& a, int which, const dim3& g, const dim3& b, hipStream_t st) {
-  if (which == kKernelPoly)
-    hipLaunchKernelGGL((k_poly<T, NC, NSF>), g, b, 0, st, a);
-  else if (a.upd)
-    hipLaunchKernelGGL((k_grad<T, NC, true, NSF>), g, b, 0, st, a);
+  if (a.pl)
+    launch_gp_pl<T, NC, NSF, true>(a, which, g, b, st);
   else
-    hipLaunchKernelGGL((k_grad<T, NC, false, NSF>), g, b, 0, st, a);
+    launch_gp_pl<T, NC, NSF, false>(a, which, g, b, st);
 }
 
 template <class T>

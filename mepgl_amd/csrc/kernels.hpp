@@ -41,6 +41,7 @@ struct StepArgs {
   int nlcg;         // Polak-Ribiere beta (1) or steepest descent (0)
   int scratch;      // G writes the diagnostic gradient buffer only
   int nc, nsf;      // class of every item in `items` (components, fixed A)
+  int pl;           // ... and whether its strings have parameter planes
 };
 
 // Arguments of the auxiliary launches (energy, JB, arclength,

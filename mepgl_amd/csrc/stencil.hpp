@@ -35,7 +35,9 @@ struct CompConst {
 
 template <class T>
 __device__ __forceinline__ double param_at(const DevParam& p, long long off) {
-  return p.p ? p.p[off] : p.v;
+  // Parameter planes live in global memory: a global (not flat) load.
+  using gd = __attribute__((address_space(1))) const double*;
+  return p.p ? ((gd)p.p)[off] : p.v;
 }
 
 template <class T, int NC>
@@ -59,6 +61,49 @@ __device__ __forceinline__ void load_comp_const(const DevString& s, long long of
     const double a = param_at<T>(s.a[c], off), b = param_at<T>(s.b[c], off);
     const double mx = param_at<T>(s.mxx[c], off), my = param_at<T>(s.myy[c], off);
     k[c].q = (T)s.q[c];
+    k[c].two_a = (T)(2.0 * a);
+    k[c].two_b = (T)(2.0 * b);
+    k[c].b = (T)b;
+    k[c].ab = (T)(a / b);
+    k[c].hx = (T)(0.5 / mx);
+    k[c].hy = (T)(0.5 / my);
+  }
+}
+
+// The parameter descriptors of a string, read once into registers before a
+// row loop (reading them from the descriptor inside the loop would be a
+// memory load per row).
+template <int NC>
+struct ParamView {
+  int planes;
+  DevParam a[NC], b[NC], mxx[NC], myy[NC];
+  double q[NC];
+};
+
+template <int NC>
+__device__ __forceinline__ ParamView<NC> param_view(const DevString& s) {
+  ParamView<NC> v;
+  v.planes = s.planes;
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    v.a[c] = s.a[c];
+    v.b[c] = s.b[c];
+    v.mxx[c] = s.mxx[c];
+    v.myy[c] = s.myy[c];
+    v.q[c] = s.q[c];
+  }
+  return v;
+}
+
+// Per-site constants of plane parameters (ParamView::planes != 0).
+template <class T, int NC>
+__device__ __forceinline__ void load_comp_planes(const ParamView<NC>& v, long long off,
+                                                 CompConst<T> (&k)[NC]) {
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const double a = param_at<T>(v.a[c], off), b = param_at<T>(v.b[c], off);
+    const double mx = param_at<T>(v.mxx[c], off), my = param_at<T>(v.myy[c], off);
+    k[c].q = (T)v.q[c];
     k[c].two_a = (T)(2.0 * a);
     k[c].two_b = (T)(2.0 * b);
     k[c].b = (T)b;

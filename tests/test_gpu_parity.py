@@ -258,7 +258,7 @@ def test_segment_rows_do_not_change_results(rows, oracle):
     ref.loop(2, 3, "L")
     with solver_for([s]) as sv:
         sv.set_segment_rows(0, rows)
-        assert sv.info(0)["tile_rows"] == rows
+        assert sv.info(0)["tile_rows"] == min(rows, s.N)
         sv.loop(2, 3, "L")
         a, p1, p2 = sv.download(0)
         fen = sv.scalars(0, "fenergy")
