@@ -696,7 +696,7 @@ __device__ __forceinline__ void p_process(PRow<T, NC>& w, const PRaw<T, NC>& r, 
   w.nn = r.nn;
 }
 
-template <class T, int NC, bool NSF, bool PL>
+template <class T, int NC, bool NSF, bool PL, int DEPTH>
 __device__ void poly_body(const StepArgs& a, const DevString& s, const WorkItem& it) {
   constexpr bool COMP = sizeof(T) == 4;
   const Strip t = strip_of(s, it);
@@ -725,30 +725,38 @@ __device__ void poly_body(const StepArgs& a, const DevString& s, const WorkItem&
   double acc_s[5] = {0, 0, 0, 0, 0}, acc_c[5] = {0, 0, 0, 0, 0};
   auto nnoff = [&](int row) { return (unsigned)((row * P_ + t.ic) * (int)sizeof(uint16_t)); };
 
-  PRaw<T, NC> raw;
+  PRaw<T, NC> raw[DEPTH];
   PRow<T, NC> wc, wn;
-  RowParams<T, NC, PL> rp_cur, rp_next;
+  RowParams<T, NC, PL> rp_cur, rp[DEPTH];
   T pax, pdax;  // row j-1: A_x and its direction
-  p_load<T, NC, NSF>(raw, b, clampr(t.j0 - 1, N), P_, t, nnoff(clampr(t.j0 - 1, N)));
-  p_process<T, NC>(wc, raw, beta);
+  p_load<T, NC, NSF>(raw[0], b, clampr(t.j0 - 1, N), P_, t, nnoff(clampr(t.j0 - 1, N)));
+  p_process<T, NC>(wc, raw[0], beta);
   pax = wc.ax;
   pdax = wc.dax;
-  p_load<T, NC, NSF>(raw, b, clampr(t.j0, N), P_, t, nnoff(clampr(t.j0, N)));
-  p_process<T, NC>(wc, raw, beta);
-  p_load<T, NC, NSF>(raw, b, clampr(t.j0 + 1, N), P_, t, nnoff(clampr(t.j0 + 1, N)));
-  p_process<T, NC>(wn, raw, beta);
+  p_load<T, NC, NSF>(raw[0], b, clampr(t.j0, N), P_, t, nnoff(clampr(t.j0, N)));
+  p_process<T, NC>(wc, raw[0], beta);
+  p_load<T, NC, NSF>(raw[0], b, clampr(t.j0 + 1, N), P_, t, nnoff(clampr(t.j0 + 1, N)));
+  p_process<T, NC>(wn, raw[0], beta);
   rp_cur.load(pv, hp, (long long)t.j0 * P_ + t.ic, t.j0, N, P_, t);
+  // Load batch k: the fields of row r and the parameters of row r - 1.
+  auto issue = [&](int r, auto buf_tag) {
+    constexpr int K = decltype(buf_tag)::value;
+    p_load<T, NC, NSF>(raw[K], b, clampr(r, N), P_, t, nnoff(clampr(r, N)));
+    rp[K].load(pv, hp, (long long)(r - 1) * P_ + t.ic, r - 1, N, P_, t);
+  };
+  if constexpr (DEPTH == 2)
+    if (t.j0 + 2 <= t.j1) issue(t.j0 + 2, std::integral_constant<int, 0>());
 
-  // Row j: one batch of loads (row j + 2; parameters of row j + 1), the
-  // quartic of row j from registers, the batch into window form, then the
-  // stores of row j; the last row, without a batch, is a separate copy of
-  // the body (see the G loop).
-  auto row = [&](int j, auto batch_tag) {
-    constexpr bool BATCH = decltype(batch_tag)::value;
-    if constexpr (BATCH) {
-      p_load<T, NC, NSF>(raw, b, clampr(j + 2, N), P_, t, nnoff(clampr(j + 2, N)));
-      rp_next.load(pv, hp, (long long)(j + 1) * P_ + t.ic, j + 1, N, P_, t);
-    }
+  // Row j: issue a batch (row j + 1 + DEPTH) into one buffer, compute the
+  // quartic of row j from registers, turn the batch of row j + 2 into window
+  // form, then store row j.  With DEPTH 2 each batch has two rows of
+  // arithmetic to arrive.  The row variants (which buffer is issued and
+  // which is consumed) are separate copies of the body, so no load is
+  // conditional and the loads in flight at each point are known exactly.
+  auto row = [&](int j, auto issue_tag, auto proc_tag) {
+    constexpr int IB = decltype(issue_tag)::value;  // buffer issued, or -1
+    constexpr int PB = decltype(proc_tag)::value;   // buffer consumed, or -1
+    if constexpr (IB >= 0) issue(j + 1 + DEPTH, std::integral_constant<int, IB>());
     __builtin_amdgcn_sched_barrier(0);
 
     PolyNbhd<T, NC> nb;
@@ -805,17 +813,17 @@ __device__ void poly_body(const StepArgs& a, const DevString& s, const WorkItem&
     }
     __builtin_amdgcn_sched_barrier(0);
     PRow<T, NC> wnn;
-    if constexpr (BATCH) {
-      p_process<T, NC>(wnn, raw, beta);
-      rp_cur = rp_next;
+    if constexpr (PB >= 0) {
+      p_process<T, NC>(wnn, raw[PB], beta);
+      rp_cur = rp[PB];
       pin_row<T, NC>(wnn);
       if constexpr (PL) {
 #pragma unroll
         for (int q = 0; q < NC; ++q)
 #pragma unroll
-          for (int k = 0; k < 4; ++k) pin(rp_cur.prm[q][k]);
+          for (int k2 = 0; k2 < 4; ++k2) pin(rp_cur.prm[q][k2]);
 #pragma unroll
-        for (int k = 0; k < 5; ++k) pin(rp_cur.h[k]);
+        for (int k2 = 0; k2 < 5; ++k2) pin(rp_cur.h[k2]);
       }
       memory_fence_compiler();
     }
@@ -832,24 +840,48 @@ __device__ void poly_body(const StepArgs& a, const DevString& s, const WorkItem&
         bstore(wc.dv[q], b.dout, o, (PV1 + 2 * q) * b.ps);
       }
     }
-    if constexpr (BATCH) {
+    if constexpr (PB >= 0) {
       pax = wc.ax;
       pdax = wc.dax;
       wc = wn;
       wn = wnn;
     }
   };
+  using I_ = std::integral_constant<int, -1>;
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  if constexpr (DEPTH == 1) {
 #pragma clang loop unroll(disable)
-  for (int j = t.j0; j < t.j1 - 1; ++j) row(j, std::true_type());
-  row(t.j1 - 1, std::false_type());
+    for (int j = t.j0; j < t.j1 - 1; ++j) row(j, I0(), I0());
+    row(t.j1 - 1, I_(), I_());
+  } else {
+    // Rows j <= j1 - 3 issue row j + 3 and consume row j + 2 (buffers
+    // alternate); row j1 - 2 only consumes; row j1 - 1 does neither.
+    int j = t.j0;
+    int cnt = t.j1 - 2 - t.j0;
+#pragma clang loop unroll(disable)
+    for (; cnt >= 2; cnt -= 2, j += 2) {
+      row(j, I1(), I0());
+      row(j + 1, I0(), I1());
+    }
+    if (cnt == 1) {
+      row(j, I1(), I0());
+      row(j + 1, I_(), I1());
+      j += 2;
+    } else if (cnt == 0) {
+      row(j, I_(), I0());
+      j += 1;
+    }
+    row(j, I_(), I_());
+  }
 
   publish_partial<5, COMP>(acc_s, acc_c, s.part_p, t.n, t.tile, s.tiles_x * s.tiles_y);
 }
 
-template <class T, int NC, bool NSF, bool PL>
+template <class T, int NC, bool NSF, bool PL, int DEPTH>
 __global__ __launch_bounds__(kBlock) void k_poly(StepArgs a) {
   const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
-  poly_body<T, NC, NSF, PL>(a, a.strings[it.string], it);
+  poly_body<T, NC, NSF, PL, DEPTH>(a, a.strings[it.string], it);
 }
 
 // ---------------------------------------------------------------------------
@@ -1460,8 +1492,10 @@ __global__ __launch_bounds__(256) void k_fill(T* p, long long n, T v) {
 template <class T, int NC, bool NSF, bool PL>
 static void launch_gp_pl(const StepArgs& a, int which, const dim3& g, const dim3& b,
                          hipStream_t st) {
-  if (which == kKernelPoly)
-    hipLaunchKernelGGL((k_poly<T, NC, NSF, PL>), g, b, 0, st, a);
+  if (which == kKernelPoly && a.depth == 2)
+    hipLaunchKernelGGL((k_poly<T, NC, NSF, PL, 2>), g, b, 0, st, a);
+  else if (which == kKernelPoly)
+    hipLaunchKernelGGL((k_poly<T, NC, NSF, PL, 1>), g, b, 0, st, a);
   else if (a.maxwell && a.upd)
     hipLaunchKernelGGL((k_grad<T, NC, true, NSF, PL, true>), g, b, 0, st, a);
   else if (a.maxwell)

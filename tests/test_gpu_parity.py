@@ -267,6 +267,23 @@ def test_segment_rows_do_not_change_results(rows, oracle):
     assert rel_err(fen, ref.fenergy) < 1e-12
 
 
+@pytest.mark.parametrize("rows", [4, 7, 16])
+def test_load_depth_does_not_change_results(rows):
+    s = small_string(N=40, F=5, seed=49, multicomponent=True, masked=True)
+    say(f"[gpu] load depth 2 vs 1, rows {rows}")
+    out = []
+    for depth in (1, 2):
+        with solver_for([s]) as sv:
+            sv.set_schedule(0, rows, depth)
+            assert sv.info(0)["depth"] == depth
+            sv.loop(2, 5, "L")
+            out.append(sv.download(0) + (sv.scalars(0, "poly"),))
+    # The row variants of the two depths are compiled separately, so the
+    # quartic's rounding may differ in the last bit.
+    for u, v in zip(out[0], out[1]):
+        assert rel_err(u, v) < 1e-13
+
+
 def test_autotune_leaves_state_unchanged(oracle):
     strings = [small_string(N=20, F=4, seed=47, multicomponent=False),
                small_string(N=36, F=5, seed=48, multicomponent=True)]
