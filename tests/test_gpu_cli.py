@@ -63,6 +63,13 @@ def test_cli_run_matches_oracle(tmp_path, oracle, mode):
         # s and s_jb are those of the last reparametrisation
         assert rel_err(extra["s_jb"], ref.s_jb) < 1e-12
         assert rel_err(extra["s"], ref.s) < 1e-12
+    # The post-processing summary of the same output (post.py:125-152).
+    from mepgl_amd import observables
+
+    z = np.load(observables.write_summary("t", str(tmp_path)), allow_pickle=True)
+    np.testing.assert_array_equal(z["fenergy"], extra["fenergy"])
+    np.testing.assert_allclose(z["phi"], extra["b"].reshape(6, -1).sum(axis=1) * s.dx ** 2,
+                               rtol=1e-12)
     # energy.csv holds the last iteration's line-search energies
     with open(os.path.join(tmp_path, "energy.csv")) as f:
         csv = np.array([float(v) for v in f.read().split(",")])
