@@ -104,33 +104,66 @@ def run_separately(strings, precision, steps, warmup):
     return dict(seconds=total, steps=iters * R)
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
 def cpu_baseline(strings, seconds):
-    """The FP64 CPU oracle on a bounded sample of the same workload: NLCG
-    steps of string 0 in chunks of 10 until about `seconds` have passed."""
+    """The FP64 CPU oracle on a bounded sample of the same workload: after
+    2 warm-up steps, NLCG steps of string 0 in chunks of 5 until about
+    `seconds` have passed; the rate is the median over chunks."""
     from oracle import oracle as O
 
     O.build()
     s = strings[0]
     ref = O.OracleString(s)
-    ref.nlcg_steps(1)  # warm-up (page-in, thread pool)
-    done = 0
+    ref.nlcg_steps(2)  # warm-up (page-in, thread pool)
+    rates, done = [], 0
     t0 = time.perf_counter()
     while True:
-        ref.nlcg_steps(10)
-        done += 10
+        t1 = time.perf_counter()
+        ref.nlcg_steps(5)
+        rates.append(s.image_cells * 5 / (time.perf_counter() - t1))
+        done += 5
         dt = time.perf_counter() - t0
-        if dt >= seconds:
+        if dt >= seconds and len(rates) >= 5:
             break
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {
-        "value": s.image_cells * done / dt,
+        "value": float(np.median(rates)),
         "unit": "image*cells/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"{done} NLCG steps ({dt:.1f} s) of string 0 ({s.F} images x {s.N}^2), "
-                  f"oracle/mepgl_oracle.c FP64, OpenMP {threads} threads, "
-                  f"{platform.processor() or platform.machine()}",
+        "sample": f"{done} NLCG steps ({dt:.1f} s, median of {len(rates)} chunks of 5) of "
+                  f"string 0 ({s.F} images x {s.N}^2), oracle/mepgl_oracle.c FP64 -O3 "
+                  f"-fopenmp, {threads} threads, {cpu_model()}",
     }
+
+
+def streaming_ceilings():
+    """Plain copy and 3-read/1-write kernels of 1 GiB on this GPU
+    (mepgl_amd/copy_calib), for the roofline's measured ceilings."""
+    import re
+    import subprocess
+
+    exe = os.path.join(ROOT, "mepgl_amd", "copy_calib")
+    out = {}
+    for mode in ("copy", "triad"):
+        try:
+            r = subprocess.run([exe, "1024", "10", mode], capture_output=True, text=True,
+                               timeout=60)
+            m = re.search(r"([0-9.]+) GB/s", r.stdout)
+            out[mode] = float(m.group(1)) if (r.returncode == 0 and m) else None
+        except (OSError, subprocess.SubprocessError):
+            out[mode] = None
+    return out
 
 
 def measured_traffic(workload, precision, kernel):
@@ -223,6 +256,10 @@ def main(argv=None):
     e_rel = float(np.max(np.abs(mixed["energies"][0] - fp64["energies"][0]) /
                          np.abs(fp64["energies"][0])))
     traffic = measured_traffic(args.workload, "mixed", kname)
+    ceil = streaming_ceilings()
+    # Cells inside the superconducting domain (the active fraction of masked
+    # geometries; the kernels stream every cell).
+    active = sum(s.F * float(np.count_nonzero(s.sc_domain)) for s in strings) / cells
 
     out = {
         "metric": "image*cells/s per NLCG step (mixed precision)",
@@ -255,7 +292,10 @@ def main(argv=None):
         "roofline": {"bound": "hbm", "kernel": kname, "achieved": ach, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
                      "algorithmic_bytes_per_image_cell": per,
-                     "algorithmic_bytes_per_launch": per * cells, "avg_ms": kms},
+                     "algorithmic_bytes_per_launch": per * cells, "avg_ms": kms,
+                     "measured_ceilings_GBs": ceil,
+                     "frac_of_triad_ceiling": ach / ceil["triad"] if ceil.get("triad") else None},
+        "active_cell_fraction": active,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(strings, args.cpu_seconds)

@@ -37,13 +37,21 @@ def _run(cmd):
 
 
 def build_library(verbose=False) -> str:
-    objs = []
+    objs, procs = [], []
     os.makedirs(os.path.join(HERE, "_build"), exist_ok=True)
+    # The translation units compile concurrently.
     for src in SOURCES:
         obj = os.path.join(HERE, "_build", src + ".o")
         lang = [] if src.endswith(".hip") else ["-x", "hip"]
-        _run([HIPCC, *HIP_FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj])
+        cmd = [HIPCC, *HIP_FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj]
+        procs.append((cmd, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                                            text=True)))
         objs.append(obj)
+    for cmd, p in procs:
+        out, err = p.communicate()
+        if p.returncode != 0:
+            sys.stderr.write(out + err)
+            raise RuntimeError("compile failed: " + " ".join(cmd[:3]) + " ...")
     tmp = LIB + ".tmp"
     _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, *objs])
     os.replace(tmp, LIB)

@@ -139,3 +139,27 @@ def test_sweep_tuned_schedule_close_to_default():
         got = [sv.download(i)[1] for i in range(len(strings))]
     for i in range(len(strings)):
         assert normwise(got[i], ref[i]) < 1e-12, i
+
+
+def test_mixed_barrier_at_convergence():
+    """Mixed vs FP64 barriers at convergence (the outer iteration changes
+    the energies by less than 1e-10 relative): within 1e-6 relative."""
+    from mepgl_amd import configs, observables
+
+    s = configs.vortex_entry_small()
+    say("[gpu] vortex_entry_small: mixed vs fp64 barriers at convergence")
+    bars = {}
+    for prec in ("fp64", "mixed"):
+        with solver([s], precision=prec) as sv:
+            e_old = None
+            for it in range(400):
+                sv.loop(1, 10, "L")
+                e = sv.computeFreeEnergy()[0]
+                if e_old is not None and np.max(np.abs(e - e_old) / np.abs(e)) < 1e-10:
+                    break
+                e_old = e
+            bars[prec] = observables.barriers(e)
+    for k in range(2):
+        ref = bars["fp64"][k]
+        if ref:
+            assert abs(bars["mixed"][k] - ref) <= 1e-6 * abs(ref), (k, bars)
