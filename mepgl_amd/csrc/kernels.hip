@@ -547,8 +547,10 @@ __device__ void grad_body(const StepArgs& a, const DevString& s, const WorkItem&
           dg += gv * (gv - go_);
         }
       }
-      acc<COMP>(acc_s[0], acc_c[0], n2);
-      acc<COMP>(acc_s[1], acc_c[1], dg);
+      // Per-lane running sums over the segment's rows in double; the
+      // compensation (mixed) starts at the cross-lane and cross-tile sums.
+      acc_s[0] += n2;
+      acc_s[1] += dg;
     }
     __builtin_amdgcn_sched_barrier(0);
     GRow<T, NC> wnn;
@@ -809,7 +811,7 @@ __device__ void poly_body(const StepArgs& a, const DevString& s, const WorkItem&
       const Quartic<T> qv =
           site_poly<T, NC>(nb, k, eta, gam, del, idx, dx2, wc.nn & 0xff, wc.nn >> 8, nsf, exact);
 #pragma unroll
-      for (int r = 0; r < 5; ++r) acc<COMP>(acc_s[r], acc_c[r], (double)qv.c[r]);
+      for (int r = 0; r < 5; ++r) acc_s[r] += (double)qv.c[r];
     }
     __builtin_amdgcn_sched_barrier(0);
     PRow<T, NC> wnn;
