@@ -39,8 +39,7 @@ KERNEL_NAMES = ((0, "grad_update"), (1, "poly"), (2, "grad_first"), (3, "finish_
                 (4, "finish_poly"))
 
 
-def run_path(strings, precision, steps, warmup, profile_iters=3, segment_rows=0, tune=True,
-             depth=0):
+def run_path(strings, precision, steps, warmup, profile_iters=3, segment_rows=0, tune=True):
     from mepgl_amd.solver import Solver
 
     sv = Solver(strings, precision=precision, segment_rows=segment_rows)
@@ -51,9 +50,6 @@ def run_path(strings, precision, steps, warmup, profile_iters=3, segment_rows=0,
     # The library's autotuner picks each string class's segment height on
     # the live problem (the state is restored afterwards).
     rows = sv.autotune(steps=4) if (tune and not segment_rows) else None
-    if segment_rows and depth:
-        for i in range(len(strings)):
-            sv.set_schedule(i, segment_rows, depth)
     # Re-start from the initial guess so that both precisions time the same
     # trajectory; warm-up only compiled graphs and touched memory.
     for i, s in enumerate(strings):
@@ -190,7 +186,6 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="both", choices=("both", "fp64", "mixed"))
     ap.add_argument("--segment-rows", type=int, default=0)
-    ap.add_argument("--depth", type=int, default=0, help="with --segment-rows: load depth 1 or 2")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: multi-GPU runs are not part of "
@@ -224,13 +219,13 @@ def main(argv=None):
     precs = ("fp64", "mixed") if args.precision == "both" else (args.precision,)
     for prec in precs:
         res[prec] = run_path(strings, prec, args.steps, args.warmup,
-                             segment_rows=args.segment_rows, depth=args.depth)
+                             segment_rows=args.segment_rows)
         print(f"# {prec}: {res[prec]['steps']} steps in {res[prec]['seconds']:.4f} s", flush=True)
     if args.precision != "both":
         r = res[args.precision]
         print(json.dumps({"precision": args.precision, "ms_per_step": 1e3 * r["seconds"] / r["steps"],
                           "value": cells * r["steps"] / r["seconds"],
-                          "segment_rows": r["info"]["tile_rows"], "depth": r["info"]["depth"]}))
+                          "segment_rows": r["info"]["tile_rows"]}))
         return 0
 
     mixed, fp64 = res["mixed"], res["fp64"]
@@ -277,9 +272,8 @@ def main(argv=None):
         "config": {"workload": args.workload, "strings": len(strings),
                    "images": sum(s.F for s in strings), "N": strings[0].N,
                    "components": strings[0].ncomp, "mode": "L", "reparam_every": R,
-                   "segment_rows": mixed["info"]["tile_rows"], "depth": mixed["info"]["depth"],
-                   "segment_rows_fp64": fp64["info"]["tile_rows"],
-                   "depth_fp64": fp64["info"]["depth"]},
+                   "segment_rows": mixed["info"]["tile_rows"],
+                   "segment_rows_fp64": fp64["info"]["tile_rows"]},
         "fp64": {"value": value64, "ms_per_step": 1e3 * fp64["seconds"] / fp64["steps"],
                  "roofline": {"kernel": kname64, "achieved": ach64, "peak": HBM_PEAK_GBS,
                               "unit": "GB/s", "frac": ach64 / HBM_PEAK_GBS,

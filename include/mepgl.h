@@ -193,20 +193,18 @@ int mepgl_synchronize(mepgl_ctx *ctx);
 
 /* Schedule.  The reference launches one thread per cell with a fixed block
  * size (config.cuh:25-31, blockSizeX); here one wavefront marches a
- * 64-column strip down `rows` rows, with `depth` (1 or 2) rows of loads in
- * flight in the direction/quartic kernel.  mepgl_set_schedule sets one
- * string's schedule (rows clamped to [min(N, 4), N]).  mepgl_autotune
- * times `steps` NLCG steps per candidate (heights 4 .. 128, depths 1 and 2)
- * for each class of strings with equal N, components and fixed-A setting,
- * keeps the fastest, and restores the solver state, so tuning leaves fields
- * and scalars as it found them; it needs device memory for one copy of the
- * fields, gradients and directions.  rows_out and depth_out (n_strings
- * entries each) may be NULL.  Sums over cells are taken per strip, so a
- * schedule change can move results in the last bits (well inside 1e-12
+ * 64-column strip down `rows` rows (heights of 3k + 1 rows waste no work).
+ * mepgl_set_segment_rows sets one string's height (clamped to
+ * [min(N, 4), N]).  mepgl_autotune times `steps` NLCG steps per candidate
+ * height (4 .. 127) for each class of strings with equal N, components and
+ * fixed-A setting, keeps the fastest, and restores the solver state, so
+ * tuning leaves fields and scalars as it found them; it needs device memory
+ * for one copy of the fields, gradients and directions.  rows_out
+ * (n_strings entries) may be NULL.  Sums over cells are taken per strip, so
+ * a height change can move results in the last bits (well inside 1e-12
  * relative). */
-int mepgl_set_schedule(mepgl_ctx *ctx, int string, int rows, int depth);
-int mepgl_get_schedule(mepgl_ctx *ctx, int string, int *rows, int *depth);
-int mepgl_autotune(mepgl_ctx *ctx, int steps, int *rows_out, int *depth_out);
+int mepgl_set_segment_rows(mepgl_ctx *ctx, int string, int rows);
+int mepgl_autotune(mepgl_ctx *ctx, int steps, int *rows_out);
 
 /* Geometry and memory facts: work items (wavefronts) per NLCG launch,
  * bytes of device memory held, and the segment height of a string. */

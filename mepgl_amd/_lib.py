@@ -28,7 +28,7 @@ EXPORTS = [
     "mepgl_compute_jb", "mepgl_arc_length", "mepgl_redistribute", "mepgl_resample",
     "mepgl_reset_interior_beta", "mepgl_gauge_align", "mepgl_loop", "mepgl_get_scalars", "mepgl_download_state",
     "mepgl_set_control", "mepgl_get_control", "mepgl_profile", "mepgl_profile_read",
-    "mepgl_synchronize", "mepgl_info", "mepgl_set_schedule", "mepgl_get_schedule", "mepgl_autotune",
+    "mepgl_synchronize", "mepgl_info", "mepgl_set_segment_rows", "mepgl_autotune",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -98,9 +98,8 @@ def load():
     L.mepgl_profile.argtypes = [vp, C.c_int]
     L.mepgl_profile_read.argtypes = [vp, C.c_int, _dp, C.POINTER(C.c_int)]
     L.mepgl_synchronize.argtypes = [vp]
-    L.mepgl_set_schedule.argtypes = [vp, C.c_int, C.c_int, C.c_int]
-    L.mepgl_get_schedule.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
-    L.mepgl_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    L.mepgl_set_segment_rows.argtypes = [vp, C.c_int, C.c_int]
+    L.mepgl_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
     L.mepgl_info.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                              C.POINTER(C.c_longlong)]
     _lib = L

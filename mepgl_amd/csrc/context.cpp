@@ -32,7 +32,6 @@ struct StringHost {
   DevString dev;            // host copy of the device descriptor
   std::vector<Alloc> owned; // device allocations of this string
   int ntiles = 0;
-  int depth = 1;            // rows of loads in flight per wavefront in P
   double* jb = nullptr;     // JB planes (lazy)
   size_t elem = 8;          // bytes per stored real
 };
@@ -65,7 +64,7 @@ struct mepgl_ctx {
   // launch once per non-empty class so each launch runs one specialised
   // kernel.
   struct Group {
-    int begin, count, nc, nsf, pl, depth;
+    int begin, count, nc, nsf, pl;
   };
   std::vector<Group> groups;
   double** d_jb = nullptr;
@@ -128,13 +127,18 @@ size_t max_tiles(const DevString& d) {
   return (size_t)d.tiles_x * (size_t)((d.N + min_rows(d.N) - 1) / min_rows(d.N));
 }
 
+// Segment heights the library picks from.  The G and P kernels march rows
+// in rotations of three after the first, so heights of 3k + 1 rows waste no
+// work (any other height pads its last rotation).
+constexpr int kRowChoices[] = {4, 7, 10, 16, 31, 64, 127};
+
 // Rows marched by one wavefront.  Taller segments re-read fewer halo rows;
 // shorter ones give more wavefronts per launch.  Pick the tallest segment
 // that still yields about eight wavefronts per SIMD (1024 SIMDs).
 int choose_rows(const mepgl_string_desc& d, long long total_images) {
   if (d.segment_rows > 0) return clamp_rows(d.segment_rows, d.N);
   const long long cols = (d.N + kTileX - 1) / kTileX;
-  for (int h : {64, 32, 16, 8}) {
+  for (int h : {64, 31, 16, 10, 7}) {
     const long long items = cols * ((d.N + h - 1) / h) * total_images;
     if (h <= d.N && items >= 8192) return h;
   }
@@ -206,7 +210,6 @@ int run_step(mepgl_ctx* c, const StepArgs& a0, int which) {
     a.nc = g.nc;
     a.nsf = g.nsf;
     a.pl = g.pl;
-    a.depth = g.depth;
     int rc = run_step_group(c, a, which);
     if (rc) return rc;
   }
@@ -339,19 +342,18 @@ int build_items(mepgl_ctx* c) {
   c->groups.clear();
   for (int nc = 1; nc <= 2; ++nc)
     for (int nsf = 0; nsf <= 1; ++nsf)
-      for (int pl = 0; pl <= 1; ++pl)
-        for (int depth = 1; depth <= 2; ++depth) {
-          const int begin = (int)items.size();
-          for (size_t si = 0; si < c->strings.size(); ++si) {
-            const DevString& s = c->strings[si].dev;
-            const int spl = (s.planes || s.h.p) ? 1 : 0;
-            if (s.nc != nc || s.nsf != nsf || spl != pl || c->strings[si].depth != depth) continue;
-            for (int n = 0; n < s.F; ++n)
-              for (int t = 0; t < s.tiles_x * s.tiles_y; ++t) items.push_back({(int)si, n, t, 0});
-          }
-          const int count = (int)items.size() - begin;
-          if (count > 0) c->groups.push_back({begin, count, nc, nsf, pl, depth});
+      for (int pl = 0; pl <= 1; ++pl) {
+        const int begin = (int)items.size();
+        for (size_t si = 0; si < c->strings.size(); ++si) {
+          const DevString& s = c->strings[si].dev;
+          const int spl = (s.planes || s.h.p) ? 1 : 0;
+          if (s.nc != nc || s.nsf != nsf || spl != pl) continue;
+          for (int n = 0; n < s.F; ++n)
+            for (int t = 0; t < s.tiles_x * s.tiles_y; ++t) items.push_back({(int)si, n, t, 0});
         }
+        const int count = (int)items.size() - begin;
+        if (count > 0) c->groups.push_back({begin, count, nc, nsf, pl});
+      }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   if (c->d_items) hipFree(c->d_items);
   c->d_items = nullptr;
@@ -1266,25 +1268,17 @@ int mepgl_synchronize(mepgl_ctx* c) {
   return MEPGL_OK;
 }
 
-int mepgl_set_schedule(mepgl_ctx* c, int si, int rows, int depth) {
-  if (!valid_string(c, si) || rows < 1 || depth < 1 || depth > 2) return MEPGL_ERR_ARG;
+int mepgl_set_segment_rows(mepgl_ctx* c, int si, int rows) {
+  if (!valid_string(c, si) || rows < 1) return MEPGL_ERR_ARG;
   hipSetDevice(c->device);
   int rc = flush(c);
   if (rc) return rc;
   StringHost& s = c->strings[si];
   s.dev.seg_rows = clamp_rows(rows, s.dev.N);
-  s.depth = depth;
   return apply_schedule(c);
 }
 
-int mepgl_get_schedule(mepgl_ctx* c, int si, int* rows, int* depth) {
-  if (!valid_string(c, si)) return MEPGL_ERR_ARG;
-  if (rows) *rows = c->strings[si].dev.seg_rows;
-  if (depth) *depth = c->strings[si].depth;
-  return MEPGL_OK;
-}
-
-int mepgl_autotune(mepgl_ctx* c, int steps, int* rows_out, int* depth_out) {
+int mepgl_autotune(mepgl_ctx* c, int steps, int* rows_out) {
   if (!c || steps < 1) return MEPGL_ERR_ARG;
   hipSetDevice(c->device);
   int rc = flush(c);
@@ -1319,28 +1313,19 @@ int mepgl_autotune(mepgl_ctx* c, int steps, int* rows_out, int* depth_out) {
     for (auto& cl : order) {
       const int N = c->strings[cl.second[0]].dev.N;
       const int start = c->strings[cl.second[0]].dev.seg_rows;
-      const int start_depth = c->strings[cl.second[0]].depth;
-      int best = start, best_depth = start_depth;
-      for (int depth : {1, 2})
-        for (int h : {4, 8, 16, 32, 64, 128}) {
-          const int r = clamp_rows(h, N);
-          if ((r == start && depth == start_depth) || (h > N && h != 4)) continue;
-          for (int si : cl.second) {
-            c->strings[si].dev.seg_rows = r;
-            c->strings[si].depth = depth;
-          }
-          double ms = 0;
-          if ((rc = time_schedule(c, sn, steps, &ms))) break;
-          if (ms < best_ms) {
-            best_ms = ms;
-            best = r;
-            best_depth = depth;
-          }
+      int best = start;
+      for (int h : kRowChoices) {
+        const int r = clamp_rows(h, N);
+        if (r == start || (h > N && h != kRowChoices[0])) continue;
+        for (int si : cl.second) c->strings[si].dev.seg_rows = r;
+        double ms = 0;
+        if ((rc = time_schedule(c, sn, steps, &ms))) break;
+        if (ms < best_ms) {
+          best_ms = ms;
+          best = r;
         }
-      for (int si : cl.second) {
-        c->strings[si].dev.seg_rows = best;
-        c->strings[si].depth = best_depth;
       }
+      for (int si : cl.second) c->strings[si].dev.seg_rows = best;
       if (rc) break;
     }
   }
@@ -1350,10 +1335,8 @@ int mepgl_autotune(mepgl_ctx* c, int steps, int* rows_out, int* depth_out) {
   drop_snapshot(c, sn);
   if (rc) return rc;
   if (rc2) return rc2;
-  for (size_t si = 0; si < c->strings.size(); ++si) {
-    if (rows_out) rows_out[si] = c->strings[si].dev.seg_rows;
-    if (depth_out) depth_out[si] = c->strings[si].depth;
-  }
+  if (rows_out)
+    for (size_t si = 0; si < c->strings.size(); ++si) rows_out[si] = c->strings[si].dev.seg_rows;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return MEPGL_OK;
 }

@@ -461,27 +461,40 @@ __device__ void grad_body(const StepArgs& a, const DevString& s, const WorkItem&
   double acc_s[2] = {0, 0}, acc_c[2] = {0, 0};
   auto nnoff = [&](int row) { return (unsigned)((row * P_ + t.ic) * (int)sizeof(uint16_t)); };
 
+  // The window is three row slots used in rotation: (previous, current,
+  // next) = (w[IP], w[IC], w[IN]), with the side values of each row in the
+  // slot of the same index.  Each row writes the newly loaded row into the
+  // previous row's slot once the current row is computed, and the next row
+  // uses the slots shifted by one, so no register copies advance the window.
   GRaw<T, NC> raw;
-  GRow<T, NC> wp, wc, wn;
-  GSide<T, NC, PL> sc_cur, sc_next;
+  GRow<T, NC> w[3];
+  GSide<T, NC, PL> sd[3];
   g_load<T, NC, UPD, NSF>(raw, b, clampr(t.j0 - 1, N), P_, t, nnoff(clampr(t.j0 - 1, N)));
-  g_process<T, NC>(wp, raw, al, UPD);
+  g_process<T, NC>(w[0], raw, al, UPD);
   g_load<T, NC, UPD, NSF>(raw, b, clampr(t.j0, N), P_, t, nnoff(clampr(t.j0, N)));
-  g_process<T, NC>(wc, raw, al, UPD);
+  g_process<T, NC>(w[1], raw, al, UPD);
   g_load<T, NC, UPD, NSF>(raw, b, clampr(t.j0 + 1, N), P_, t, nnoff(clampr(t.j0 + 1, N)));
-  g_process<T, NC>(wn, raw, al, UPD);
-  g_side_load<T, NC, NSF, PL, MW>(sc_cur, b, t.j0, P_, t, pv, hp, N);
+  g_process<T, NC>(w[2], raw, al, UPD);
+  g_side_load<T, NC, NSF, PL, MW>(sd[1], b, t.j0, P_, t, pv, hp, N);
 
   // Row j: issue one batch of loads (the fields of row j + 2, the own-site
   // values of row j + 1), compute row j from registers, turn the batch into
   // window form, then store row j.  Every load is consumed before this
   // row's stores are issued, and the last row (no batch) is a separate copy
   // of the body, so the loads in flight at each point are known exactly.
-  auto row = [&](int j, auto batch_tag) {
+  // Rows at or past j1 (padding of the last rotation) are computed but
+  // neither stored nor summed.
+  auto row = [&](int j, auto ip_tag, auto ic_tag, auto in_tag, auto batch_tag) {
+    constexpr int IP = decltype(ip_tag)::value, IC = decltype(ic_tag)::value;
+    constexpr int IN = decltype(in_tag)::value;
     constexpr bool BATCH = decltype(batch_tag)::value;
+    GRow<T, NC>& wp = w[IP];
+    GRow<T, NC>& wc = w[IC];
+    GRow<T, NC>& wn = w[IN];
+    const bool live = store && j < t.j1;
     if constexpr (BATCH) {
       g_load<T, NC, UPD, NSF>(raw, b, clampr(j + 2, N), P_, t, nnoff(clampr(j + 2, N)));
-      g_side_load<T, NC, NSF, PL, MW>(sc_next, b, j + 1, P_, t, pv, hp, N);
+      g_side_load<T, NC, NSF, PL, MW>(sd[IN], b, clampr(j + 1, N), P_, t, pv, hp, N);
     }
     __builtin_amdgcn_sched_barrier(0);
 
@@ -505,11 +518,12 @@ __device__ void grad_body(const StepArgs& a, const DevString& s, const WorkItem&
       nb.u[q][4] = wp.u[q];
       nb.v[q][4] = wp.v[q];
     }
+    const GSide<T, NC, PL>& cur = sd[IC];
     CompConst<T> k[NC];
     if constexpr (PL) {
-      comp_from_params<T, NC>(sc_cur.rp.prm, pv.q, k);
+      comp_from_params<T, NC>(cur.rp.prm, pv.q, k);
 #pragma unroll
-      for (int q = 0; q < 5; ++q) nb.h[q] = hp ? sc_cur.rp.h[q] : hv;
+      for (int q = 0; q < 5; ++q) nb.h[q] = hp ? cur.rp.h[q] : hv;
     } else {
 #pragma unroll
       for (int q = 0; q < NC; ++q) k[q] = kconst[q];
@@ -523,17 +537,17 @@ __device__ void grad_body(const StepArgs& a, const DevString& s, const WorkItem&
     if constexpr (MW) {
 #pragma unroll
       for (int q = 0; q < NC; ++q) {
-        gp[q][0] = sc_cur.gst[q][0];
-        gp[q][1] = sc_cur.gst[q][1];
+        gp[q][0] = cur.gst[q][0];
+        gp[q][1] = cur.gst[q][1];
       }
     }
-    if (store && want_go) {
+    if (live && want_go) {
       // gradientDensity, nlcg.cu:919-949
       double n2 = 0, dg = 0;
       if (do_a) {
 #pragma unroll
         for (int q = 0; q < 2; ++q) {
-          const double gv = (double)ga[q], go_ = (double)sc_cur.go[q];
+          const double gv = (double)ga[q], go_ = (double)cur.go[q];
           n2 += gv * gv;
           dg += gv * (gv - go_);
         }
@@ -542,7 +556,7 @@ __device__ void grad_body(const StepArgs& a, const DevString& s, const WorkItem&
       for (int q = 0; q < NC; ++q) {
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
-          const double gv = (double)gp[q][r], go_ = (double)sc_cur.go[PU1 + 2 * q + r];
+          const double gv = (double)gp[q][r], go_ = (double)cur.go[PU1 + 2 * q + r];
           n2 += gv * gv;
           dg += gv * (gv - go_);
         }
@@ -553,16 +567,14 @@ __device__ void grad_body(const StepArgs& a, const DevString& s, const WorkItem&
       acc_s[1] += dg;
     }
     __builtin_amdgcn_sched_barrier(0);
-    GRow<T, NC> wnn;
     if constexpr (BATCH) {
-      g_process<T, NC>(wnn, raw, al, UPD);
-      sc_cur = sc_next;
-      pin_row<T, NC>(wnn);
-      pin_side<T, NC, PL>(sc_cur);
+      g_process<T, NC>(wp, raw, al, UPD);
+      pin_row<T, NC>(wp);
+      pin_side<T, NC, PL>(sd[IN]);
       memory_fence_compiler();
     }
     __builtin_amdgcn_sched_barrier(0);
-    if (store) {
+    if (live) {
       const unsigned o = (unsigned)((j * P_ + t.ic) * (int)sizeof(T));
       if constexpr (!MW) {
 #pragma unroll
@@ -587,16 +599,21 @@ __device__ void grad_body(const StepArgs& a, const DevString& s, const WorkItem&
         }
       }
     }
-    // Advance the window.
-    if constexpr (BATCH) {
-      wp = wc;
-      wc = wn;
-      wn = wnn;
-    }
   };
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, 1>;
+  using S2 = std::integral_constant<int, 2>;
+  // Rows j0 .. j1-2 carry a batch; they run in rotations of three, the last
+  // rotation padded past j1 - 1 when their count is not a multiple of three.
+  const int groups = (t.j1 - 1 - t.j0 + 2) / 3;
+  int j = t.j0;
 #pragma clang loop unroll(disable)
-  for (int j = t.j0; j < t.j1 - 1; ++j) row(j, std::true_type());
-  row(t.j1 - 1, std::false_type());
+  for (int g = 0; g < groups; ++g, j += 3) {
+    row(j, S0(), S1(), S2(), std::true_type());
+    row(j + 1, S1(), S2(), S0(), std::true_type());
+    row(j + 2, S2(), S0(), S1(), std::true_type());
+  }
+  row(j, S0(), S1(), S2(), std::false_type());
   if (a.scratch) return;
 
   publish_partial<2, COMP>(acc_s, acc_c, s.part_g, t.n, t.tile, s.tiles_x * s.tiles_y);
@@ -698,7 +715,7 @@ __device__ __forceinline__ void p_process(PRow<T, NC>& w, const PRaw<T, NC>& r, 
   w.nn = r.nn;
 }
 
-template <class T, int NC, bool NSF, bool PL, int DEPTH>
+template <class T, int NC, bool NSF, bool PL>
 __device__ void poly_body(const StepArgs& a, const DevString& s, const WorkItem& it) {
   constexpr bool COMP = sizeof(T) == 4;
   const Strip t = strip_of(s, it);
@@ -727,38 +744,37 @@ __device__ void poly_body(const StepArgs& a, const DevString& s, const WorkItem&
   double acc_s[5] = {0, 0, 0, 0, 0}, acc_c[5] = {0, 0, 0, 0, 0};
   auto nnoff = [&](int row) { return (unsigned)((row * P_ + t.ic) * (int)sizeof(uint16_t)); };
 
-  PRaw<T, NC> raw[DEPTH];
-  PRow<T, NC> wc, wn;
-  RowParams<T, NC, PL> rp_cur, rp[DEPTH];
-  T pax, pdax;  // row j-1: A_x and its direction
-  p_load<T, NC, NSF>(raw[0], b, clampr(t.j0 - 1, N), P_, t, nnoff(clampr(t.j0 - 1, N)));
-  p_process<T, NC>(wc, raw[0], beta);
-  pax = wc.ax;
-  pdax = wc.dax;
-  p_load<T, NC, NSF>(raw[0], b, clampr(t.j0, N), P_, t, nnoff(clampr(t.j0, N)));
-  p_process<T, NC>(wc, raw[0], beta);
-  p_load<T, NC, NSF>(raw[0], b, clampr(t.j0 + 1, N), P_, t, nnoff(clampr(t.j0 + 1, N)));
-  p_process<T, NC>(wn, raw[0], beta);
-  rp_cur.load(pv, hp, (long long)t.j0 * P_ + t.ic, t.j0, N, P_, t);
-  // Load batch k: the fields of row r and the parameters of row r - 1.
-  auto issue = [&](int r, auto buf_tag) {
-    constexpr int K = decltype(buf_tag)::value;
-    p_load<T, NC, NSF>(raw[K], b, clampr(r, N), P_, t, nnoff(clampr(r, N)));
-    rp[K].load(pv, hp, (long long)(r - 1) * P_ + t.ic, r - 1, N, P_, t);
-  };
-  if constexpr (DEPTH == 2)
-    if (t.j0 + 2 <= t.j1) issue(t.j0 + 2, std::integral_constant<int, 0>());
+  // Three row slots in rotation, as in the G kernel: w[IP] is the previous
+  // row (only its A_x and direction are read), w[IC] the current, w[IN] the
+  // next; the parameters of a row (parameter planes) sit in the slot of the
+  // same index.
+  PRaw<T, NC> raw;
+  PRow<T, NC> w[3];
+  RowParams<T, NC, PL> rp[3];
+  p_load<T, NC, NSF>(raw, b, clampr(t.j0 - 1, N), P_, t, nnoff(clampr(t.j0 - 1, N)));
+  p_process<T, NC>(w[0], raw, beta);
+  p_load<T, NC, NSF>(raw, b, clampr(t.j0, N), P_, t, nnoff(clampr(t.j0, N)));
+  p_process<T, NC>(w[1], raw, beta);
+  p_load<T, NC, NSF>(raw, b, clampr(t.j0 + 1, N), P_, t, nnoff(clampr(t.j0 + 1, N)));
+  p_process<T, NC>(w[2], raw, beta);
+  rp[1].load(pv, hp, (long long)t.j0 * P_ + t.ic, t.j0, N, P_, t);
 
-  // Row j: issue a batch (row j + 1 + DEPTH) into one buffer, compute the
-  // quartic of row j from registers, turn the batch of row j + 2 into window
-  // form, then store row j.  With DEPTH 2 each batch has two rows of
-  // arithmetic to arrive.  The row variants (which buffer is issued and
-  // which is consumed) are separate copies of the body, so no load is
-  // conditional and the loads in flight at each point are known exactly.
-  auto row = [&](int j, auto issue_tag, auto proc_tag) {
-    constexpr int IB = decltype(issue_tag)::value;  // buffer issued, or -1
-    constexpr int PB = decltype(proc_tag)::value;   // buffer consumed, or -1
-    if constexpr (IB >= 0) issue(j + 1 + DEPTH, std::integral_constant<int, IB>());
+  // Row j: one batch of loads (the fields of row j + 2, the parameters of
+  // row j + 1), the quartic of row j from registers, the batch into the
+  // previous row's slot, then the stores of row j (see the G loop).
+  auto row = [&](int j, auto ip_tag, auto ic_tag, auto in_tag, auto batch_tag) {
+    constexpr int IP = decltype(ip_tag)::value, IC = decltype(ic_tag)::value;
+    constexpr int IN = decltype(in_tag)::value;
+    constexpr bool BATCH = decltype(batch_tag)::value;
+    PRow<T, NC>& wp = w[IP];
+    PRow<T, NC>& wc = w[IC];
+    PRow<T, NC>& wn = w[IN];
+    const bool live = store && j < t.j1;
+    if constexpr (BATCH) {
+      p_load<T, NC, NSF>(raw, b, clampr(j + 2, N), P_, t, nnoff(clampr(j + 2, N)));
+      const int r1 = clampr(j + 1, N);
+      rp[IN].load(pv, hp, (long long)r1 * P_ + t.ic, r1, N, P_, t);
+    }
     __builtin_amdgcn_sched_barrier(0);
 
     PolyNbhd<T, NC> nb;
@@ -766,7 +782,7 @@ __device__ void poly_body(const StepArgs& a, const DevString& s, const WorkItem&
     nb.ax[1] = from_right(wc.ax, wc.eax);
     nb.ax[2] = 0;
     nb.ax[3] = wn.ax;
-    nb.ax[4] = pax;
+    nb.ax[4] = wp.ax;
     nb.ay[0] = wc.ay;
     nb.ay[1] = from_right(wc.ay, wc.eay);
     nb.ay[2] = from_left(wc.ay, wc.eay);
@@ -776,7 +792,7 @@ __device__ void poly_body(const StepArgs& a, const DevString& s, const WorkItem&
     nb.dax[1] = from_right(wc.dax, wc.edax);
     nb.dax[2] = 0;
     nb.dax[3] = wn.dax;
-    nb.dax[4] = pdax;
+    nb.dax[4] = wp.dax;
     nb.day[0] = wc.day;
     nb.day[1] = from_right(wc.day, wc.eday);
     nb.day[2] = from_left(wc.day, wc.eday);
@@ -799,38 +815,36 @@ __device__ void poly_body(const StepArgs& a, const DevString& s, const WorkItem&
     }
     CompConst<T> k[NC];
     if constexpr (PL) {
-      comp_from_params<T, NC>(rp_cur.prm, pv.q, k);
-      nb.h0 = hp ? rp_cur.h[0] : hv;
+      comp_from_params<T, NC>(rp[IC].prm, pv.q, k);
+      nb.h0 = hp ? rp[IC].h[0] : hv;
     } else {
 #pragma unroll
       for (int q = 0; q < NC; ++q) k[q] = kconst[q];
       nb.h0 = hv;
     }
 
-    if (store) {
+    if (live) {
       const Quartic<T> qv =
           site_poly<T, NC>(nb, k, eta, gam, del, idx, dx2, wc.nn & 0xff, wc.nn >> 8, nsf, exact);
 #pragma unroll
       for (int r = 0; r < 5; ++r) acc_s[r] += (double)qv.c[r];
     }
     __builtin_amdgcn_sched_barrier(0);
-    PRow<T, NC> wnn;
-    if constexpr (PB >= 0) {
-      p_process<T, NC>(wnn, raw[PB], beta);
-      rp_cur = rp[PB];
-      pin_row<T, NC>(wnn);
+    if constexpr (BATCH) {
+      p_process<T, NC>(wp, raw, beta);
+      pin_row<T, NC>(wp);
       if constexpr (PL) {
 #pragma unroll
         for (int q = 0; q < NC; ++q)
 #pragma unroll
-          for (int k2 = 0; k2 < 4; ++k2) pin(rp_cur.prm[q][k2]);
+          for (int k2 = 0; k2 < 4; ++k2) pin(rp[IN].prm[q][k2]);
 #pragma unroll
-        for (int k2 = 0; k2 < 5; ++k2) pin(rp_cur.h[k2]);
+        for (int k2 = 0; k2 < 5; ++k2) pin(rp[IN].h[k2]);
       }
       memory_fence_compiler();
     }
     __builtin_amdgcn_sched_barrier(0);
-    if (store) {
+    if (live) {
       const unsigned o = (unsigned)((j * P_ + t.ic) * (int)sizeof(T));
       if (!nsf) {
         bstore(wc.dax, b.dout, o, PAX * b.ps);
@@ -842,48 +856,27 @@ __device__ void poly_body(const StepArgs& a, const DevString& s, const WorkItem&
         bstore(wc.dv[q], b.dout, o, (PV1 + 2 * q) * b.ps);
       }
     }
-    if constexpr (PB >= 0) {
-      pax = wc.ax;
-      pdax = wc.dax;
-      wc = wn;
-      wn = wnn;
-    }
   };
-  using I_ = std::integral_constant<int, -1>;
-  using I0 = std::integral_constant<int, 0>;
-  using I1 = std::integral_constant<int, 1>;
-  if constexpr (DEPTH == 1) {
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, 1>;
+  using S2 = std::integral_constant<int, 2>;
+  const int groups = (t.j1 - 1 - t.j0 + 2) / 3;
+  int j = t.j0;
 #pragma clang loop unroll(disable)
-    for (int j = t.j0; j < t.j1 - 1; ++j) row(j, I0(), I0());
-    row(t.j1 - 1, I_(), I_());
-  } else {
-    // Rows j <= j1 - 3 issue row j + 3 and consume row j + 2 (buffers
-    // alternate); row j1 - 2 only consumes; row j1 - 1 does neither.
-    int j = t.j0;
-    int cnt = t.j1 - 2 - t.j0;
-#pragma clang loop unroll(disable)
-    for (; cnt >= 2; cnt -= 2, j += 2) {
-      row(j, I1(), I0());
-      row(j + 1, I0(), I1());
-    }
-    if (cnt == 1) {
-      row(j, I1(), I0());
-      row(j + 1, I_(), I1());
-      j += 2;
-    } else if (cnt == 0) {
-      row(j, I_(), I0());
-      j += 1;
-    }
-    row(j, I_(), I_());
+  for (int g = 0; g < groups; ++g, j += 3) {
+    row(j, S0(), S1(), S2(), std::true_type());
+    row(j + 1, S1(), S2(), S0(), std::true_type());
+    row(j + 2, S2(), S0(), S1(), std::true_type());
   }
+  row(j, S0(), S1(), S2(), std::false_type());
 
   publish_partial<5, COMP>(acc_s, acc_c, s.part_p, t.n, t.tile, s.tiles_x * s.tiles_y);
 }
 
-template <class T, int NC, bool NSF, bool PL, int DEPTH>
+template <class T, int NC, bool NSF, bool PL>
 __global__ __launch_bounds__(kBlock) void k_poly(StepArgs a) {
   const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
-  poly_body<T, NC, NSF, PL, DEPTH>(a, a.strings[it.string], it);
+  poly_body<T, NC, NSF, PL>(a, a.strings[it.string], it);
 }
 
 // ---------------------------------------------------------------------------
@@ -1494,10 +1487,8 @@ __global__ __launch_bounds__(256) void k_fill(T* p, long long n, T v) {
 template <class T, int NC, bool NSF, bool PL>
 static void launch_gp_pl(const StepArgs& a, int which, const dim3& g, const dim3& b,
                          hipStream_t st) {
-  if (which == kKernelPoly && a.depth == 2)
-    hipLaunchKernelGGL((k_poly<T, NC, NSF, PL, 2>), g, b, 0, st, a);
-  else if (which == kKernelPoly)
-    hipLaunchKernelGGL((k_poly<T, NC, NSF, PL, 1>), g, b, 0, st, a);
+  if (which == kKernelPoly)
+    hipLaunchKernelGGL((k_poly<T, NC, NSF, PL>), g, b, 0, st, a);
   else if (a.maxwell && a.upd)
     hipLaunchKernelGGL((k_grad<T, NC, true, NSF, PL, true>), g, b, 0, st, a);
   else if (a.maxwell)

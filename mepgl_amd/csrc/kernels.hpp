@@ -42,7 +42,6 @@ struct StepArgs {
   int scratch;      // G writes the diagnostic gradient buffer only
   int nc, nsf;      // class of every item in `items` (components, fixed A)
   int pl;           // ... and whether its strings have parameter planes
-  int depth;        // rows of loads in flight per wavefront (1 or 2)
 };
 
 // Arguments of the auxiliary launches (energy, JB, arclength,

@@ -215,26 +215,18 @@ class Solver:
         self._check(self._L.mepgl_profile_read(self._ctx, kernel, C.byref(ms), C.byref(cnt)))
         return ms.value, cnt.value
 
-    def set_schedule(self, i: int, rows: int, depth: int = 1):
-        self._check(self._L.mepgl_set_schedule(self._ctx, i, int(rows), int(depth)))
-
     def set_segment_rows(self, i: int, rows: int):
-        self.set_schedule(i, rows, self.info(i)["depth"])
+        self._check(self._L.mepgl_set_segment_rows(self._ctx, i, int(rows)))
 
     def autotune(self, steps: int = 5):
-        """Pick each string class's segment height and load depth by timing
-        `steps` NLCG steps per candidate; the solver state is left
-        unchanged.  Returns the heights (the depths are in info())."""
+        """Pick each string class's segment height by timing `steps` NLCG
+        steps per candidate; the solver state is left unchanged."""
         rows = (C.c_int * len(self.strings))()
-        depth = (C.c_int * len(self.strings))()
-        self._check(self._L.mepgl_autotune(self._ctx, int(steps), rows, depth))
+        self._check(self._L.mepgl_autotune(self._ctx, int(steps), rows))
         return list(rows)
 
     def info(self, i: int = 0):
         rows, items, nbytes = C.c_int(), C.c_int(), C.c_longlong()
         self._check(self._L.mepgl_info(self._ctx, i, C.byref(rows), C.byref(items),
                                        C.byref(nbytes)))
-        r2, depth = C.c_int(), C.c_int()
-        self._check(self._L.mepgl_get_schedule(self._ctx, i, C.byref(r2), C.byref(depth)))
-        return {"tile_rows": rows.value, "work_items": items.value, "bytes": nbytes.value,
-                "depth": depth.value}
+        return {"tile_rows": rows.value, "work_items": items.value, "bytes": nbytes.value}
