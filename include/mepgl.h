@@ -172,6 +172,17 @@ int mepgl_get_scalars(mepgl_ctx *ctx, int string, int which, double *out);
 int mepgl_download_state(mepgl_ctx *ctx, int string, int which, double *a, double *psi_1,
                          double *psi_2);
 
+/* Restores saved solver state, for an exact resume of the relaxation: the
+ * same buffers as mepgl_download_state (0 g, 1 g_old, 2 d), and the
+ * per-image g_norm2, g_norm2_old, s and s_jb.  The reference restarts each
+ * run from zero gradients and directions (mepgl.cu:124-140); a resume from
+ * its reloaded output files is mepgl_upload + mepgl_reset_solver.  A call
+ * sequence download, download_state x3, get_scalars x2 ... upload,
+ * upload_state x3, set_scalars x2 continues bit for bit. */
+int mepgl_upload_state(mepgl_ctx *ctx, int string, int which, const double *a,
+                       const double *psi_1, const double *psi_2);
+int mepgl_set_scalars(mepgl_ctx *ctx, int string, int which, const double *values);
+
 /* Controls of the stdin thread (mepgl.cu:635-671), safe from any thread. */
 #define MEPGL_CONTROL_STOP 0
 #define MEPGL_CONTROL_RELAXATION_STEPS 1

@@ -28,7 +28,7 @@ EXPORTS = [
     "mepgl_compute_jb", "mepgl_arc_length", "mepgl_redistribute", "mepgl_resample",
     "mepgl_reset_interior_beta", "mepgl_gauge_align", "mepgl_loop", "mepgl_get_scalars", "mepgl_download_state",
     "mepgl_set_control", "mepgl_get_control", "mepgl_profile", "mepgl_profile_read",
-    "mepgl_synchronize", "mepgl_info", "mepgl_set_segment_rows", "mepgl_autotune",
+    "mepgl_synchronize", "mepgl_info", "mepgl_set_segment_rows", "mepgl_autotune", "mepgl_upload_state", "mepgl_set_scalars",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -93,6 +93,8 @@ def load():
     L.mepgl_loop.argtypes = [vp, C.c_int, C.c_int, C.c_char, C.c_int]
     L.mepgl_get_scalars.argtypes = [vp, C.c_int, C.c_int, _dp]
     L.mepgl_download_state.argtypes = [vp, C.c_int, C.c_int, _dp, _dp, _dp]
+    L.mepgl_upload_state.argtypes = [vp, C.c_int, C.c_int, _dp, _dp, _dp]
+    L.mepgl_set_scalars.argtypes = [vp, C.c_int, C.c_int, _dp]
     L.mepgl_set_control.argtypes = [vp, C.c_int, C.c_int]
     L.mepgl_get_control.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
     L.mepgl_profile.argtypes = [vp, C.c_int]

@@ -549,6 +549,25 @@ int copy_out_fields(mepgl_ctx* c, int si, const void* dev, double* a, double* p1
   return MEPGL_OK;
 }
 
+// Host fields (F, N, N, 2) into a device field set of one string.
+int copy_in_fields(mepgl_ctx* c, int si, void* dev, const double* a, const double* p1,
+                   const double* p2) {
+  StringHost& s = c->strings[si];
+  const size_t n = (size_t)s.dev.F * s.dev.image_stride;
+  if (c->fp32()) {
+    std::vector<float> h;
+    pack_fields(s, a, p1, s.dev.nc == 2 ? p2 : nullptr, h);
+    HIP_TRY(c, hipMemcpyAsync(dev, h.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<double> h;
+    pack_fields(s, a, p1, s.dev.nc == 2 ? p2 : nullptr, h);
+    HIP_TRY(c, hipMemcpyAsync(dev, h.data(), n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+  }
+  return MEPGL_OK;
+}
+
 int arc_length_all(mepgl_ctx* c) {
   for (auto& s : c->strings)
     if (s.dev.F < 2)
@@ -1205,6 +1224,44 @@ int mepgl_download_state(mepgl_ctx* c, int si, int which, double* a, double* p1,
   const DevString& d = c->strings[si].dev;
   const void* src = which == 0 ? d.g[0] : which == 1 ? d.g[1] : d.d[c->dp];
   return copy_out_fields(c, si, src, a, p1, p2);
+}
+
+int mepgl_upload_state(mepgl_ctx* c, int si, int which, const double* a, const double* p1,
+                       const double* p2) {
+  if (!valid_string(c, si) || !a || !p1 || which < 0 || which > 2)
+    return fail(c, MEPGL_ERR_ARG, "upload_state: bad arguments");
+  if (c->strings[si].dev.nc == 2 && !p2) return fail(c, MEPGL_ERR_ARG, "upload_state: psi_2 required");
+  hipSetDevice(c->device);
+  int rc = flush(c);  // an owed update uses the direction being replaced
+  if (rc) return rc;
+  const DevString& d = c->strings[si].dev;
+  void* dst = which == 0 ? d.g[0] : which == 1 ? d.g[1] : d.d[c->dp];
+  return copy_in_fields(c, si, dst, a, p1, p2);
+}
+
+int mepgl_set_scalars(mepgl_ctx* c, int si, int which, const double* in) {
+  if (!valid_string(c, si) || !in) return fail(c, MEPGL_ERR_ARG, "set_scalars: bad arguments");
+  if (which != MEPGL_SCALAR_G_NORM2 && which != MEPGL_SCALAR_G_NORM2_OLD &&
+      which != MEPGL_SCALAR_S && which != MEPGL_SCALAR_S_JB)
+    return fail(c, MEPGL_ERR_ARG, "set_scalars: only g_norm2, g_norm2_old, s and s_jb are settable");
+  hipSetDevice(c->device);
+  const DevString& d = c->strings[si].dev;
+  std::vector<ImageScalars> h(d.F);
+  HIP_TRY(c, hipMemcpyAsync(h.data(), c->d_sc + d.img0, sizeof(ImageScalars) * d.F,
+                            hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  for (int n = 0; n < d.F; ++n) {
+    switch (which) {
+      case MEPGL_SCALAR_G_NORM2: h[n].g_norm2[0] = in[n]; break;
+      case MEPGL_SCALAR_G_NORM2_OLD: h[n].g_norm2[1] = in[n]; break;
+      case MEPGL_SCALAR_S: h[n].s = in[n]; break;
+      default: h[n].s_jb = in[n]; break;
+    }
+  }
+  HIP_TRY(c, hipMemcpyAsync(c->d_sc + d.img0, h.data(), sizeof(ImageScalars) * d.F,
+                            hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  return MEPGL_OK;
 }
 
 int mepgl_set_control(mepgl_ctx* c, int key, int value) {

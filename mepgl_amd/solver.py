@@ -131,6 +131,66 @@ class Solver:
         self._check(self._L.mepgl_download_state(self._ctx, i, k, _ptr(a), _ptr(p1), _ptr(p2)))
         return a, p1, p2
 
+    def upload_state(self, i: int, which: str, a, p1, p2=None):
+        """Restores a solver buffer ("g", "g_old" or "d") of string i."""
+        k = {"g": 0, "g_old": 1, "d": 2}[which]
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        p1 = np.ascontiguousarray(p1, dtype=np.float64)
+        p2 = None if p2 is None else np.ascontiguousarray(p2, dtype=np.float64)
+        self._check(self._L.mepgl_upload_state(self._ctx, i, k, _ptr(a), _ptr(p1), _ptr(p2)))
+
+    def set_scalars(self, i: int, which: str, values):
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        if v.shape != (self.F[i],):
+            raise ValueError(f"{which} needs {self.F[i]} values")
+        self._check(self._L.mepgl_set_scalars(self._ctx, i, _lib.SCALARS[which], _ptr(v)))
+
+    _STATE_SCALARS = ("g_norm2", "g_norm2_old", "s", "s_jb")
+
+    def checkpoint(self) -> dict:
+        """The complete relaxation state: fields, both gradient buffers, the
+        direction and the per-image scalars they pair with, per string.
+        restore() of it continues the relaxation bit for bit."""
+        out = {}
+        for i, s in enumerate(self.strings):
+            mc = s.multicomponent
+            for key, arrs in (("x", self.download(i)), ("g", self.download_state(i, "g")),
+                              ("g_old", self.download_state(i, "g_old")),
+                              ("d", self.download_state(i, "d"))):
+                out[f"s{i}_{key}_a"], out[f"s{i}_{key}_p1"] = arrs[0], arrs[1]
+                if mc:
+                    out[f"s{i}_{key}_p2"] = arrs[2]
+            for nm in self._STATE_SCALARS:
+                out[f"s{i}_{nm}"] = self.scalars(i, nm)
+        return out
+
+    def restore(self, ckpt: dict):
+        for i, s in enumerate(self.strings):
+            mc = s.multicomponent
+
+            def get(key):
+                return (ckpt[f"s{i}_{key}_a"], ckpt[f"s{i}_{key}_p1"],
+                        ckpt[f"s{i}_{key}_p2"] if mc else None)
+
+            a, p1, p2 = get("x")
+            if a.shape != self._shape(i):
+                raise ValueError(f"checkpoint of string {i} has shape {a.shape}, "
+                                 f"expected {self._shape(i)}")
+            self._check(self._L.mepgl_upload(self._ctx, i, _ptr(np.ascontiguousarray(a)),
+                                             _ptr(np.ascontiguousarray(p1)),
+                                             _ptr(None if p2 is None else np.ascontiguousarray(p2))))
+            for key in ("g", "g_old", "d"):
+                self.upload_state(i, key, *get(key))
+            for nm in self._STATE_SCALARS:
+                self.set_scalars(i, nm, ckpt[f"s{i}_{nm}"])
+
+    def save_checkpoint(self, path: str):
+        np.savez(path, **self.checkpoint())
+
+    def load_checkpoint(self, path: str):
+        with np.load(path, allow_pickle=False) as z:
+            self.restore({k: z[k] for k in z.files})
+
     def scalars(self, i: int, which: str) -> np.ndarray:
         n = self.F[i] * (4 if which == "poly" else 1)
         out = np.empty(n)

@@ -298,3 +298,31 @@ def test_autotune_leaves_state_unchanged(oracle):
         ref.nlcg_steps(3)
         ref.nlcg_steps(3)
         assert normwise(got[i], ref.psi_1) < 1e-10, i
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_checkpoint_resume_is_exact(tmp_path, precision):
+    """Relax, checkpoint, relax more; a fresh context restored from the
+    checkpoint relaxes to the same fields bit for bit."""
+    strings = [small_string(N=24, F=5, seed=50, multicomponent=True),
+               small_string(N=20, F=4, seed=51, multicomponent=False)]
+    say(f"[gpu] checkpoint resume {precision}")
+    path = str(tmp_path / "ckpt.npz")
+    with solver_for(strings, precision=precision) as sv:
+        sv.loop(2, 3, "L")
+        sv.nlcgSteps(3)
+        sv.save_checkpoint(path)
+        sv.loop(2, 3, "L")
+        sv.nlcgSteps(2)
+        want = [sv.download(i) for i in range(2)] + [sv.scalars(i, "g_norm2") for i in range(2)]
+    with solver_for(strings, precision=precision) as sv:
+        sv.load_checkpoint(path)
+        sv.loop(2, 3, "L")
+        sv.nlcgSteps(2)
+        got = [sv.download(i) for i in range(2)] + [sv.scalars(i, "g_norm2") for i in range(2)]
+    for x, y in zip(want, got):
+        if isinstance(x, tuple):
+            for u, v in zip(x, y):
+                np.testing.assert_array_equal(u, v)
+        else:
+            np.testing.assert_array_equal(x, y)
