@@ -326,3 +326,24 @@ def test_checkpoint_resume_is_exact(tmp_path, precision):
                 np.testing.assert_array_equal(u, v)
         else:
             np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("N", [65, 129])
+def test_one_column_last_strip(N, oracle):
+    """N = 64k + 1: the last 64-column strip holds a single column, whose
+    x-neighbours come from clamped lanes."""
+    s = small_string(N=N, F=3, seed=52, multicomponent=True, masked=True)
+    say(f"[gpu] N={N} one-column strip")
+    ref = oracle.OracleString(s)
+    ref.loop(1, 3, "L")
+    with solver_for([s]) as sv:
+        e = sv.computeFreeEnergy()[0]
+        ga, g1, g2 = sv.gradient(0)
+        sv.loop(1, 3, "L")
+        a, p1, p2 = sv.download(0)
+    assert rel_err(e, oracle.free_energy(s)) < 1e-12
+    ra, r1, r2 = oracle.gradient(s)
+    assert normwise(g1, r1) < 1e-12 and normwise(g2, r2) < 1e-12 and normwise(ga, ra) < 1e-12
+    assert normwise(p1, ref.psi_1) < 1e-10
+    assert normwise(p2, ref.psi_2) < 1e-10
+    assert normwise(a, ref.a) < 1e-10
