@@ -100,6 +100,12 @@ int mepgl_download(mepgl_ctx *ctx, int string, double *a, double *psi_1, double 
  * direction, infinite gradient norms. */
 int mepgl_reset_solver(mepgl_ctx *ctx);
 
+/* The boundary phase winding of every image (compute_winding,
+ * gl_utils.py:10-41, as post.py:140-144 reports it: 2 pi times the winding
+ * number, not divided by 2 pi), for both components; read with
+ * MEPGL_SCALAR_WINDING_1 and MEPGL_SCALAR_WINDING_2. */
+int mepgl_winding(mepgl_ctx *ctx);
+
 /* computeFreeEnergy (nlcg.cu:217-240) for every image of every string,
  * evaluated in double; read with MEPGL_SCALAR_ENERGY. */
 int mepgl_free_energy(mepgl_ctx *ctx);
@@ -165,6 +171,8 @@ int mepgl_loop(mepgl_ctx *ctx, int iterations, int relaxation_steps, char mode, 
 #define MEPGL_SCALAR_G_NORM2_OLD 9
 #define MEPGL_SCALAR_LS_INDEX 10 /* scan index where the line search stopped */
 #define MEPGL_SCALAR_THETA 11    /* phase applied by the last alignment     */
+#define MEPGL_SCALAR_WINDING_1 12 /* boundary winding of psi_1 (mepgl_winding) */
+#define MEPGL_SCALAR_WINDING_2 13 /* boundary winding of psi_2                 */
 int mepgl_get_scalars(mepgl_ctx *ctx, int string, int which, double *out);
 
 /* Internal solver buffers of one string, for state checks:

@@ -18,6 +18,7 @@ POLY_EXACT, CUBIC_TEXTBOOK, NO_GRAPH, GAUGE_ALIGN = 1, 2, 4, 8
 SCALARS = {
     "fenergy": 0, "g_norm2": 1, "alpha": 2, "s": 3, "s_jb": 4, "beta": 5, "g_delta_g": 6,
     "poly": 7, "energy": 8, "g_norm2_old": 9, "ls_index": 10, "theta": 11,
+    "winding_1": 12, "winding_2": 13,
 }
 CONTROL_STOP, CONTROL_RELAXATION_STEPS, CONTROL_NLCG = 0, 1, 2
 KERNEL_GRAD, KERNEL_POLY = 0, 1
@@ -28,7 +29,7 @@ EXPORTS = [
     "mepgl_compute_jb", "mepgl_arc_length", "mepgl_redistribute", "mepgl_resample",
     "mepgl_reset_interior_beta", "mepgl_gauge_align", "mepgl_loop", "mepgl_get_scalars", "mepgl_download_state",
     "mepgl_set_control", "mepgl_get_control", "mepgl_profile", "mepgl_profile_read",
-    "mepgl_synchronize", "mepgl_info", "mepgl_set_segment_rows", "mepgl_autotune", "mepgl_upload_state", "mepgl_set_scalars",
+    "mepgl_synchronize", "mepgl_info", "mepgl_set_segment_rows", "mepgl_autotune", "mepgl_upload_state", "mepgl_set_scalars", "mepgl_winding",
 ]
 
 _dp = C.POINTER(C.c_double)
@@ -82,6 +83,7 @@ def load():
     L.mepgl_download.argtypes = [vp, C.c_int, _dp, _dp, _dp]
     L.mepgl_reset_solver.argtypes = [vp]
     L.mepgl_free_energy.argtypes = [vp]
+    L.mepgl_winding.argtypes = [vp]
     L.mepgl_gradient.argtypes = [vp, C.c_int, _dp, _dp, _dp]
     L.mepgl_nlcg_steps.argtypes = [vp, C.c_int, C.c_int, C.c_int]
     L.mepgl_compute_jb.argtypes = [vp, C.c_int, _dp, _dp, _dp]

@@ -45,6 +45,7 @@ struct ImageScalars {
   double ds2, ds2_jb;  // squared distance to the previous image
   double align_re, align_im;  // sum of conj(psi_n) psi_{n-1} (alignment)
   double theta;        // constant phase applied by the last alignment
+  double winding[2];   // boundary phase winding of each component
   int ls_index;        // scan index at which the line search stopped
 };
 

@@ -249,6 +249,7 @@ FinishArgs finish_args(mepgl_ctx* c) {
   f.ls = c->d_ls;
   f.out = c->d_out;
   f.nlcg = 1;
+  f.xin = c->xp;
   return f;
 }
 
@@ -945,6 +946,16 @@ int mepgl_reset_solver(mepgl_ctx* c) {
   return reset_solver(c);
 }
 
+int mepgl_winding(mepgl_ctx* c) {
+  if (!c) return MEPGL_ERR_ARG;
+  hipSetDevice(c->device);
+  for (auto& s : c->strings)
+    if (s.dev.N < 2) return fail(c, MEPGL_ERR_ARG, "winding needs N >= 2");
+  int rc = flush(c);
+  if (rc) return rc;
+  return run_finish(c, finish_args(c), kKernelWinding);
+}
+
 int mepgl_free_energy(mepgl_ctx* c) {
   if (!c) return MEPGL_ERR_ARG;
   hipSetDevice(c->device);
@@ -1206,6 +1217,8 @@ int mepgl_get_scalars(mepgl_ctx* c, int si, int which, double* out) {
       case MEPGL_SCALAR_G_DELTA_G: out[n] = x.g_delta_g; break;
       case MEPGL_SCALAR_LS_INDEX: out[n] = x.ls_index; break;
       case MEPGL_SCALAR_THETA: out[n] = x.theta; break;
+      case MEPGL_SCALAR_WINDING_1: out[n] = x.winding[0]; break;
+      case MEPGL_SCALAR_WINDING_2: out[n] = x.winding[1]; break;
       case MEPGL_SCALAR_ENERGY: out[n] = e[n]; break;
       case MEPGL_SCALAR_POLY:
         for (int q = 0; q < 4; ++q) out[4 * n + q] = x.poly[q];

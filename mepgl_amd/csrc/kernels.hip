@@ -1426,6 +1426,40 @@ __global__ __launch_bounds__(kBlock) void k_align_apply(AuxArgs a) {
   }
 }
 
+// Phase winding around the grid boundary of one image (gl_utils.py:10-41,
+// post.py:140-144): the boundary is walked along row 0 with x increasing,
+// column N-1 with y increasing, row N-1 with x decreasing and column 0 with
+// y decreasing; each edge adds (u dv - v du) / |psi|^2 at its start point.
+// One wavefront per image; lanes take boundary points in turn and the lane
+// sums are combined in lane order.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_winding(FinishArgs f) {
+  int n;
+  const DevString& s = string_of_image(f, n);
+  const int N = s.N, L = N - 1;
+  const T* X = (const T*)s.x[f.xin] + (long long)n * s.image_stride;
+  double acc[2] = {0, 0}, comp[2] = {0, 0};
+  for (int k = threadIdx.x; k < 4 * L; k += kWave) {
+    const int seg = k / L, m = k % L;
+    int i0, j0, i1, j1;
+    if (seg == 0) { i0 = m; j0 = 0; i1 = m + 1; j1 = 0; }
+    else if (seg == 1) { i0 = L; j0 = m; i1 = L; j1 = m + 1; }
+    else if (seg == 2) { i0 = L - m; j0 = L; i1 = L - m - 1; j1 = L; }
+    else { i0 = 0; j0 = L - m; i1 = 0; j1 = L - m - 1; }
+    const long long o0 = (long long)j0 * s.pitch + i0, o1 = (long long)j1 * s.pitch + i1;
+    for (int c = 0; c < s.nc; ++c) {
+      const long long pu = (PU1 + 2 * c) * s.plane_stride, pv = (PV1 + 2 * c) * s.plane_stride;
+      const double u = X[pu + o0], v = X[pv + o0], un = X[pu + o1], vn = X[pv + o1];
+      acc[c] += (u * (vn - v) - v * (un - u)) / (u * u + v * v);
+    }
+  }
+  wave_reduce<2, true>(acc, comp);
+  if (threadIdx.x == 0) {
+    f.sc[blockIdx.x].winding[0] = acc[0] + comp[0];
+    f.sc[blockIdx.x].winding[1] = s.nc == 2 ? acc[1] + comp[1] : 0.0;
+  }
+}
+
 // After the alignment partials.
 __global__ __launch_bounds__(kBlock) void k_finish_align(FinishArgs f) {
   int n;
@@ -1604,6 +1638,12 @@ hipError_t launch_finish(const FinishArgs& f, int which, bool fp32, hipStream_t 
       break;
     case kKernelAlign:
       hipLaunchKernelGGL(k_finish_align, g, b, 0, st, f);
+      break;
+    case kKernelWinding:
+      if (fp32)
+        hipLaunchKernelGGL(k_winding<float>, g, b, 0, st, f);
+      else
+        hipLaunchKernelGGL(k_winding<double>, g, b, 0, st, f);
       break;
     default:
       return hipErrorInvalidValue;

@@ -22,6 +22,7 @@ enum KernelId {
   kKernelAlign,        // alignment partials
   kKernelAlignFinalize,
   kKernelAlignApply,
+  kKernelWinding,      // boundary phase winding per image (finishing-style launch)
   kKernelCount
 };
 
@@ -67,6 +68,7 @@ struct FinishArgs {
   const LineSearchTable* ls;
   double* out;
   int scur, sold, nlcg;
+  int xin;           // current field buffer (winding)
 };
 
 hipError_t launch_step(const StepArgs& a, int which, bool fp32, hipStream_t st);

@@ -203,6 +203,14 @@ class Solver:
         self._check(self._L.mepgl_free_energy(self._ctx))
         return [self.scalars(i, "energy") for i in range(len(self.strings))]
 
+    def winding(self):
+        """Boundary phase windings (compute_winding, gl_utils.py:10-41) of
+        every image: per string (w_1, w_2), w_2 None for one component."""
+        self._check(self._L.mepgl_winding(self._ctx))
+        return [(self.scalars(i, "winding_1"),
+                 self.scalars(i, "winding_2") if s.multicomponent else None)
+                for i, s in enumerate(self.strings)]
+
     def gradient(self, i: int):
         """gradientPsi + gradientA (nlcg.cu:504-888) at the current fields."""
         a, p1, p2 = (np.empty(self._shape(i)) for _ in range(3))

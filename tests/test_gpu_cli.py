@@ -153,3 +153,33 @@ def test_launcher_stages(tmp_path, oracle):
     assert normwise(ref2.psi_1, ref.psi_1) < 1e-10
     ref2.loop(2, 3, "L")
     assert normwise(out.psi_1, ref2.psi_1) < 1e-10
+
+
+def test_cli_stdin_keys(tmp_path):
+    """+ and - change the relaxation steps, C toggles NLCG, q stops
+    (mepgl.cu:647-669); keys are read one at a time as they arrive."""
+    import time
+
+    from mepgl_amd import npyio
+
+    s = small_string(N=16, F=4, seed=26)
+    say("[gpu] cli stdin keys")
+    npyio.write_input(s, "t", str(tmp_path))
+    p = subprocess.Popen([EXE, "--simname", "t", "--mode", "S", "--Fin", "4", "--iterations",
+                          "100000", "--relaxation-steps", "2"], cwd=str(tmp_path),
+                         stdin=subprocess.PIPE, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
+                         text=True)
+    try:
+        for key in ("+", "+", "-", "C", "q"):
+            time.sleep(0.4)
+            p.stdin.write(key)
+            p.stdin.flush()
+        out, err = p.communicate(timeout=120)
+    finally:
+        if p.poll() is None:
+            p.kill()
+    assert p.returncode == 0, err
+    assert "relaxation_steps = 3" in out and "relaxation_steps = 4" in out
+    assert out.index("relaxation_steps = 4") < out.rindex("relaxation_steps = 3")
+    assert "NLCG set OFF" in out
+    assert "I am quitting! Please wait." in out

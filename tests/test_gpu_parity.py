@@ -347,3 +347,30 @@ def test_one_column_last_strip(N, oracle):
     assert normwise(p1, ref.psi_1) < 1e-10
     assert normwise(p2, ref.psi_2) < 1e-10
     assert normwise(a, ref.a) < 1e-10
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_winding_kernel_matches_summary(precision):
+    """mepgl_winding against observables.winding (pinned by closed forms)
+    on fields with a vortex, an antivortex and no vortex."""
+    from mepgl_amd import observables
+
+    s = small_string(N=33, F=3, seed=53, multicomponent=True)
+    c = (np.arange(33) - 16.0)
+    y, x = np.meshgrid(c, c, indexing="ij")
+    z = x + 1j * y + 0.3 + 0.2j
+    for n, p in enumerate([z / np.abs(z), np.conj(z) / np.abs(z), np.full((33, 33), 0.8 + 0.1j)]):
+        s.psi_1[n, ..., 0], s.psi_1[n, ..., 1] = p.real, p.imag
+        q = (z / np.abs(z)) ** 2
+        s.psi_2[n, ..., 0], s.psi_2[n, ..., 1] = q.real, q.imag
+    say(f"[gpu] winding kernel {precision}")
+    with solver_for([s], precision=precision) as sv:
+        (w1, w2), = sv.winding()
+        _, p1, p2 = sv.download(0)
+    # The reference's layout [n][i][j] is the transpose of [n][j][i].
+    r1 = observables.winding(np.swapaxes(p1[..., 0], 1, 2), np.swapaxes(p1[..., 1], 1, 2))
+    r2 = observables.winding(np.swapaxes(p2[..., 0], 1, 2), np.swapaxes(p2[..., 1], 1, 2))
+    np.testing.assert_allclose(w1, r1, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(w2, r2, rtol=0, atol=1e-12)
+    np.testing.assert_allclose(w1, [2 * np.pi, -2 * np.pi, 0.0], atol=0.05)
+    np.testing.assert_allclose(w2, [4 * np.pi] * 3, atol=0.1)
