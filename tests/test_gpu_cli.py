@@ -157,7 +157,9 @@ def test_launcher_stages(tmp_path, oracle):
 
 def test_cli_stdin_keys(tmp_path):
     """+ and - change the relaxation steps, C toggles NLCG, q stops
-    (mepgl.cu:647-669); keys are read one at a time as they arrive."""
+    (mepgl.cu:647-669).  As in the reference, one read takes up to 4 bytes
+    and acts on the first, so the keys are sent one at a time once the loop
+    runs; the output goes to a file so that the program never blocks."""
     import time
 
     from mepgl_amd import npyio
@@ -165,21 +167,30 @@ def test_cli_stdin_keys(tmp_path):
     s = small_string(N=16, F=4, seed=26)
     say("[gpu] cli stdin keys")
     npyio.write_input(s, "t", str(tmp_path))
-    p = subprocess.Popen([EXE, "--simname", "t", "--mode", "S", "--Fin", "4", "--iterations",
-                          "100000", "--relaxation-steps", "2"], cwd=str(tmp_path),
-                         stdin=subprocess.PIPE, stdout=subprocess.PIPE, stderr=subprocess.PIPE,
-                         text=True)
-    try:
-        for key in ("+", "+", "-", "C", "q"):
-            time.sleep(0.4)
-            p.stdin.write(key)
-            p.stdin.flush()
-        out, err = p.communicate(timeout=120)
-    finally:
-        if p.poll() is None:
-            p.kill()
-    assert p.returncode == 0, err
-    assert "relaxation_steps = 3" in out and "relaxation_steps = 4" in out
-    assert out.index("relaxation_steps = 4") < out.rindex("relaxation_steps = 3")
+    log = tmp_path / "out.txt"
+    with open(log, "w") as fo:
+        p = subprocess.Popen([EXE, "--simname", "t", "--mode", "S", "--Fin", "4", "--iterations",
+                              "100000", "--relaxation-steps", "2"], cwd=str(tmp_path),
+                             stdin=subprocess.PIPE, stdout=fo, stderr=subprocess.STDOUT,
+                             text=True)
+        try:
+            t0 = time.time()
+            while "Starting MEP-MGL." not in log.read_text():
+                assert p.poll() is None and time.time() - t0 < 60, log.read_text()[-2000:]
+                time.sleep(0.1)
+            for key in ("+", "+", "-", "C", "q"):
+                time.sleep(0.3)
+                p.stdin.write(key)
+                p.stdin.flush()
+            p.stdin.close()
+            p.wait(timeout=120)
+        finally:
+            if p.poll() is None:
+                p.kill()
+    out = log.read_text()
+    assert p.returncode == 0, out[-2000:]
+    i3 = out.index("relaxation_steps = 3")
+    i4 = out.index("relaxation_steps = 4")
+    assert i3 < i4 < out.rindex("relaxation_steps = 3")
     assert "NLCG set OFF" in out
     assert "I am quitting! Please wait." in out
