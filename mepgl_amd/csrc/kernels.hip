@@ -2,13 +2,15 @@
 //
 // Work decomposition: one wavefront per work item = one 64-column strip of
 // one row segment of one image of one string.  Lane l owns column
-// 64*strip + l; the wavefront marches down the rows of its segment, keeping
-// the rows j-1, j, j+1 it needs in a register window and prefetching row
-// j+2 while it computes row j.  Neighbours along x come from the adjacent
-// lanes through DPP wave shifts; the two columns just outside the strip are
-// loaded by lanes 0 and 63 only.  Every field value is therefore read from
-// HBM once per launch, plus the halo rows and columns the strip shares with
-// its neighbours (served by L2 when adjacent work items run on the same XCD).
+// 64*strip + l; the wavefront marches along the rows of its segment (down in
+// even segments, up in odd ones), keeping the rows j-1, j, j+1 it needs in a
+// register window and prefetching the next-but-one row while it computes
+// row j.  Neighbours along x come from the adjacent lanes through DPP wave
+// shifts; the two columns just outside the strip are loaded by lanes 0 and
+// 63 only.  Every field value is therefore read from HBM once per launch,
+// plus the halo rows and columns the strip shares with its neighbours,
+// which L2 serves: adjacent work items run at the same time on the same
+// XCD, and neighbouring segments reach their shared rows together.
 //
 // One NLCG step (nlcgSteps, nlcg.cu:1478-1583) is two launches:
 //   G  fused pending update x += alpha d (updateFields), gradient
@@ -192,6 +194,15 @@ __device__ __forceinline__ Strip strip_of(const DevString& s, const WorkItem& it
   t.j0 = sy * s.seg_rows;
   t.j1 = min(t.j0 + s.seg_rows, s.N);
   return t;
+}
+
+// Segments alternate their march direction: even segments march down, odd
+// ones up.  Two segments that share a boundary then read the rows around it
+// at about the same time (both at their start or both at their end), and
+// vertically adjacent segments run concurrently on one XCD (xcd_item), so
+// the second read of each shared row is served by that XCD's L2.
+__device__ __forceinline__ bool marches_up(const DevString& s, const WorkItem& it) {
+  return __builtin_amdgcn_readfirstlane((it.tile / s.tiles_x) & 1) != 0;
 }
 
 __device__ __forceinline__ int clampr(int j, int N) { return j < 0 ? 0 : (j >= N ? N - 1 : j); }
@@ -431,9 +442,8 @@ __device__ __forceinline__ void pin_side(GSide<T, NC, PL>& sd) {
 }
 
 template <class T, int NC, bool UPD, bool NSF, bool PL, bool MW>
-__device__ void grad_body(const StepArgs& a, const DevString& s, const WorkItem& it) {
+__device__ __forceinline__ void grad_body(const StepArgs& a, const DevString& s, const WorkItem& it) {
   constexpr bool COMP = sizeof(T) == 4;
-  constexpr int NP = GRaw<T, NC>::P;
   const Strip t = strip_of(s, it);
   const int N = s.N, P_ = s.pitch;
   const long long ibytes = s.image_stride * (long long)sizeof(T);
@@ -461,159 +471,178 @@ __device__ void grad_body(const StepArgs& a, const DevString& s, const WorkItem&
   double acc_s[2] = {0, 0}, acc_c[2] = {0, 0};
   auto nnoff = [&](int row) { return (unsigned)((row * P_ + t.ic) * (int)sizeof(uint16_t)); };
 
-  // The window is three row slots used in rotation: (previous, current,
-  // next) = (w[IP], w[IC], w[IN]), with the side values of each row in the
-  // slot of the same index.  Each row writes the newly loaded row into the
-  // previous row's slot once the current row is computed, and the next row
-  // uses the slots shifted by one, so no register copies advance the window.
-  GRaw<T, NC> raw;
-  GRow<T, NC> w[3];
-  GSide<T, NC, PL> sd[3];
-  g_load<T, NC, UPD, NSF>(raw, b, clampr(t.j0 - 1, N), P_, t, nnoff(clampr(t.j0 - 1, N)));
-  g_process<T, NC>(w[0], raw, al, UPD);
-  g_load<T, NC, UPD, NSF>(raw, b, clampr(t.j0, N), P_, t, nnoff(clampr(t.j0, N)));
-  g_process<T, NC>(w[1], raw, al, UPD);
-  g_load<T, NC, UPD, NSF>(raw, b, clampr(t.j0 + 1, N), P_, t, nnoff(clampr(t.j0 + 1, N)));
-  g_process<T, NC>(w[2], raw, al, UPD);
-  g_side_load<T, NC, NSF, PL, MW>(sd[1], b, t.j0, P_, t, pv, hp, N);
-
-  // Row j: issue one batch of loads (the fields of row j + 2, the own-site
-  // values of row j + 1), compute row j from registers, turn the batch into
-  // window form, then store row j.  Every load is consumed before this
-  // row's stores are issued, and the last row (no batch) is a separate copy
-  // of the body, so the loads in flight at each point are known exactly.
-  // Rows at or past j1 (padding of the last rotation) are computed but
-  // neither stored nor summed.
-  auto row = [&](int j, auto ip_tag, auto ic_tag, auto in_tag, auto batch_tag) {
-    constexpr int IP = decltype(ip_tag)::value, IC = decltype(ic_tag)::value;
-    constexpr int IN = decltype(in_tag)::value;
-    constexpr bool BATCH = decltype(batch_tag)::value;
-    GRow<T, NC>& wp = w[IP];
-    GRow<T, NC>& wc = w[IC];
-    GRow<T, NC>& wn = w[IN];
-    const bool live = store && j < t.j1;
-    if constexpr (BATCH) {
-      g_load<T, NC, UPD, NSF>(raw, b, clampr(j + 2, N), P_, t, nnoff(clampr(j + 2, N)));
-      g_side_load<T, NC, NSF, PL, MW>(sd[IN], b, clampr(j + 1, N), P_, t, pv, hp, N);
-    }
-    __builtin_amdgcn_sched_barrier(0);
-
-    Nbhd<T, NC> nb;
-    nb.ax[0][0] = wp.axl; nb.ax[0][1] = wp.ax; nb.ax[0][2] = wp.axr;
-    nb.ax[1][0] = wc.axl; nb.ax[1][1] = wc.ax; nb.ax[1][2] = wc.axr;
-    nb.ax[2][0] = wn.axl; nb.ax[2][1] = wn.ax; nb.ax[2][2] = wn.axr;
-    nb.ay[0][0] = wp.ayl; nb.ay[0][1] = wp.ay; nb.ay[0][2] = wp.ayr;
-    nb.ay[1][0] = wc.ayl; nb.ay[1][1] = wc.ay; nb.ay[1][2] = wc.ayr;
-    nb.ay[2][0] = wn.ayl; nb.ay[2][1] = wn.ay; nb.ay[2][2] = wn.ayr;
-#pragma unroll
-    for (int q = 0; q < NC; ++q) {
-      nb.u[q][0] = wc.u[q];
-      nb.v[q][0] = wc.v[q];
-      nb.u[q][1] = from_right(wc.u[q], wc.eu[q]);
-      nb.v[q][1] = from_right(wc.v[q], wc.ev[q]);
-      nb.u[q][2] = from_left(wc.u[q], wc.eu[q]);
-      nb.v[q][2] = from_left(wc.v[q], wc.ev[q]);
-      nb.u[q][3] = wn.u[q];
-      nb.v[q][3] = wn.v[q];
-      nb.u[q][4] = wp.u[q];
-      nb.v[q][4] = wp.v[q];
-    }
-    const GSide<T, NC, PL>& cur = sd[IC];
-    CompConst<T> k[NC];
-    if constexpr (PL) {
-      comp_from_params<T, NC>(cur.rp.prm, pv.q, k);
-#pragma unroll
-      for (int q = 0; q < 5; ++q) nb.h[q] = hp ? cur.rp.h[q] : hv;
-    } else {
-#pragma unroll
-      for (int q = 0; q < NC; ++q) k[q] = kconst[q];
-#pragma unroll
-      for (int q = 0; q < 5; ++q) nb.h[q] = hv;
-    }
-
-    T gp[NC][2], ga[2] = {0, 0};
-    site_gradient<T, NC>(nb, k, eta, gam, del, idx, dx2, wc.nn & 0xff, wc.nn >> 8, !MW, do_a,
-                         gp, ga);
-    if constexpr (MW) {
-#pragma unroll
-      for (int q = 0; q < NC; ++q) {
-        gp[q][0] = cur.gst[q][0];
-        gp[q][1] = cur.gst[q][1];
-      }
-    }
-    if (live && want_go) {
-      // gradientDensity, nlcg.cu:919-949
-      double n2 = 0, dg = 0;
-      if (do_a) {
-#pragma unroll
-        for (int q = 0; q < 2; ++q) {
-          const double gv = (double)ga[q], go_ = (double)cur.go[q];
-          n2 += gv * gv;
-          dg += gv * (gv - go_);
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < NC; ++q) {
-#pragma unroll
-        for (int r = 0; r < 2; ++r) {
-          const double gv = (double)gp[q][r], go_ = (double)cur.go[PU1 + 2 * q + r];
-          n2 += gv * gv;
-          dg += gv * (gv - go_);
-        }
-      }
-      // Per-lane running sums over the segment's rows in double; the
-      // compensation (mixed) starts at the cross-lane and cross-tile sums.
-      acc_s[0] += n2;
-      acc_s[1] += dg;
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    if constexpr (BATCH) {
-      g_process<T, NC>(wp, raw, al, UPD);
-      pin_row<T, NC>(wp);
-      pin_side<T, NC, PL>(sd[IN]);
-      memory_fence_compiler();
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    if (live) {
-      const unsigned o = (unsigned)((j * P_ + t.ic) * (int)sizeof(T));
-      if constexpr (!MW) {
-#pragma unroll
-        for (int q = 0; q < NC; ++q) {
-          bstore(gp[q][0], b.g, o, (PU1 + 2 * q) * b.ps);
-          bstore(gp[q][1], b.g, o, (PV1 + 2 * q) * b.ps);
-        }
-      }
-      if (do_a) {
-        bstore(ga[0], b.g, o, PAX * b.ps);
-        bstore(ga[1], b.g, o, PAY * b.ps);
-      }
-      if (UPD) {
-        if (do_a) {
-          bstore(wc.ax, b.xo, o, PAX * b.ps);
-          bstore(wc.ay, b.xo, o, PAY * b.ps);
-        }
-#pragma unroll
-        for (int q = 0; q < NC; ++q) {
-          bstore(wc.u[q], b.xo, o, (PU1 + 2 * q) * b.ps);
-          bstore(wc.v[q], b.xo, o, (PV1 + 2 * q) * b.ps);
-        }
-      }
-    }
-  };
+  // The march is the only part that depends on the direction
+  // (marches_up); everything it needs from the string's descriptor is
+  // read above, before the branch and before any store, so it stays in
+  // scalar registers.
+  const bool up = marches_up(s, it);
   using S0 = std::integral_constant<int, 0>;
   using S1 = std::integral_constant<int, 1>;
   using S2 = std::integral_constant<int, 2>;
-  // Rows j0 .. j1-2 carry a batch; they run in rotations of three, the last
-  // rotation padded past j1 - 1 when their count is not a multiple of three.
-  const int groups = (t.j1 - 1 - t.j0 + 2) / 3;
-  int j = t.j0;
+  auto march = [&](auto up_tag) {
+    constexpr bool UP = decltype(up_tag)::value;
+    constexpr int D = UP ? -1 : 1;  // march direction in rows
+    // The window is three row slots used in rotation: (previous, current,
+    // next) = (w[IP], w[IC], w[IN]) in the order of the march, with the side
+    // values of each row in the slot of the same index.  Each row writes the
+    // newly loaded row into the previous row's slot once the current row is
+    // computed, and the next row uses the slots shifted by one, so no register
+    // copies advance the window.  A segment marches down (rows j0 .. j1-1) or,
+    // when UP, up (rows j1-1 .. j0); the previous row of the march is then
+    // row j + 1.
+    const int jb = UP ? t.j1 - 1 : t.j0;  // first row of the march
+    GRaw<T, NC> raw;
+    GRow<T, NC> w[3];
+    GSide<T, NC, PL> sd[3];
+    g_load<T, NC, UPD, NSF>(raw, b, clampr(jb - D, N), P_, t, nnoff(clampr(jb - D, N)));
+    g_process<T, NC>(w[0], raw, al, UPD);
+    g_load<T, NC, UPD, NSF>(raw, b, clampr(jb, N), P_, t, nnoff(clampr(jb, N)));
+    g_process<T, NC>(w[1], raw, al, UPD);
+    g_load<T, NC, UPD, NSF>(raw, b, clampr(jb + D, N), P_, t, nnoff(clampr(jb + D, N)));
+    g_process<T, NC>(w[2], raw, al, UPD);
+    g_side_load<T, NC, NSF, PL, MW>(sd[1], b, jb, P_, t, pv, hp, N);
+
+    // Row j: issue one batch of loads (the fields of row j + 2D, the own-site
+    // values of row j + D), compute row j from registers, turn the batch into
+    // window form, then store row j.  Every load is consumed before this
+    // row's stores are issued, and the last row (no batch) is a separate copy
+    // of the body, so the loads in flight at each point are known exactly.
+    // Rows past the end of the segment (padding of the last rotation) are
+    // computed but neither stored nor summed.
+    auto row = [&](int j, auto ip_tag, auto ic_tag, auto in_tag, auto batch_tag) {
+      constexpr int IP = decltype(ip_tag)::value, IC = decltype(ic_tag)::value;
+      constexpr int IN = decltype(in_tag)::value;
+      constexpr bool BATCH = decltype(batch_tag)::value;
+      GRow<T, NC>& wm = w[UP ? IN : IP];  // row j - 1
+      GRow<T, NC>& wc = w[IC];
+      GRow<T, NC>& wpl = w[UP ? IP : IN];  // row j + 1
+      const bool live = store && (UP ? j >= t.j0 : j < t.j1);
+      if constexpr (BATCH) {
+        g_load<T, NC, UPD, NSF>(raw, b, clampr(j + 2 * D, N), P_, t,
+                                nnoff(clampr(j + 2 * D, N)));
+        g_side_load<T, NC, NSF, PL, MW>(sd[IN], b, clampr(j + D, N), P_, t, pv, hp, N);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+
+      Nbhd<T, NC> nb;
+      nb.ax[0][0] = wm.axl; nb.ax[0][1] = wm.ax; nb.ax[0][2] = wm.axr;
+      nb.ax[1][0] = wc.axl; nb.ax[1][1] = wc.ax; nb.ax[1][2] = wc.axr;
+      nb.ax[2][0] = wpl.axl; nb.ax[2][1] = wpl.ax; nb.ax[2][2] = wpl.axr;
+      nb.ay[0][0] = wm.ayl; nb.ay[0][1] = wm.ay; nb.ay[0][2] = wm.ayr;
+      nb.ay[1][0] = wc.ayl; nb.ay[1][1] = wc.ay; nb.ay[1][2] = wc.ayr;
+      nb.ay[2][0] = wpl.ayl; nb.ay[2][1] = wpl.ay; nb.ay[2][2] = wpl.ayr;
+#pragma unroll
+      for (int q = 0; q < NC; ++q) {
+        nb.u[q][0] = wc.u[q];
+        nb.v[q][0] = wc.v[q];
+        nb.u[q][1] = from_right(wc.u[q], wc.eu[q]);
+        nb.v[q][1] = from_right(wc.v[q], wc.ev[q]);
+        nb.u[q][2] = from_left(wc.u[q], wc.eu[q]);
+        nb.v[q][2] = from_left(wc.v[q], wc.ev[q]);
+        nb.u[q][3] = wpl.u[q];
+        nb.v[q][3] = wpl.v[q];
+        nb.u[q][4] = wm.u[q];
+        nb.v[q][4] = wm.v[q];
+      }
+      const GSide<T, NC, PL>& cur = sd[IC];
+      CompConst<T> k[NC];
+      if constexpr (PL) {
+        comp_from_params<T, NC>(cur.rp.prm, pv.q, k);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) nb.h[q] = hp ? cur.rp.h[q] : hv;
+      } else {
+#pragma unroll
+        for (int q = 0; q < NC; ++q) k[q] = kconst[q];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) nb.h[q] = hv;
+      }
+
+      T gp[NC][2], ga[2] = {0, 0};
+      site_gradient<T, NC>(nb, k, eta, gam, del, idx, dx2, wc.nn & 0xff, wc.nn >> 8, !MW, do_a,
+                           gp, ga);
+      if constexpr (MW) {
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+          gp[q][0] = cur.gst[q][0];
+          gp[q][1] = cur.gst[q][1];
+        }
+      }
+      if (live && want_go) {
+        // gradientDensity, nlcg.cu:919-949
+        double n2 = 0, dg = 0;
+        if (do_a) {
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const double gv = (double)ga[q], go_ = (double)cur.go[q];
+            n2 += gv * gv;
+            dg += gv * (gv - go_);
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+#pragma unroll
+          for (int r = 0; r < 2; ++r) {
+            const double gv = (double)gp[q][r], go_ = (double)cur.go[PU1 + 2 * q + r];
+            n2 += gv * gv;
+            dg += gv * (gv - go_);
+          }
+        }
+        // Per-lane running sums over the segment's rows in double; the
+        // compensation (mixed) starts at the cross-lane and cross-tile sums.
+        acc_s[0] += n2;
+        acc_s[1] += dg;
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      if constexpr (BATCH) {
+        g_process<T, NC>(w[IP], raw, al, UPD);
+        pin_row<T, NC>(w[IP]);
+        pin_side<T, NC, PL>(sd[IN]);
+        memory_fence_compiler();
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      if (live) {
+        const unsigned o = (unsigned)((j * P_ + t.ic) * (int)sizeof(T));
+        if constexpr (!MW) {
+#pragma unroll
+          for (int q = 0; q < NC; ++q) {
+            bstore(gp[q][0], b.g, o, (PU1 + 2 * q) * b.ps);
+            bstore(gp[q][1], b.g, o, (PV1 + 2 * q) * b.ps);
+          }
+        }
+        if (do_a) {
+          bstore(ga[0], b.g, o, PAX * b.ps);
+          bstore(ga[1], b.g, o, PAY * b.ps);
+        }
+        if (UPD) {
+          if (do_a) {
+            bstore(wc.ax, b.xo, o, PAX * b.ps);
+            bstore(wc.ay, b.xo, o, PAY * b.ps);
+          }
+#pragma unroll
+          for (int q = 0; q < NC; ++q) {
+            bstore(wc.u[q], b.xo, o, (PU1 + 2 * q) * b.ps);
+            bstore(wc.v[q], b.xo, o, (PV1 + 2 * q) * b.ps);
+          }
+        }
+      }
+    };
+    // All rows of the segment but the last carry a batch; they run in
+    // rotations of three, the last rotation padded past the segment's end when
+    // their count is not a multiple of three.
+    const int groups = (t.j1 - 1 - t.j0 + 2) / 3;
+    int j = jb;
 #pragma clang loop unroll(disable)
-  for (int g = 0; g < groups; ++g, j += 3) {
-    row(j, S0(), S1(), S2(), std::true_type());
-    row(j + 1, S1(), S2(), S0(), std::true_type());
-    row(j + 2, S2(), S0(), S1(), std::true_type());
-  }
-  row(j, S0(), S1(), S2(), std::false_type());
+    for (int g = 0; g < groups; ++g, j += 3 * D) {
+      row(j, S0(), S1(), S2(), std::true_type());
+      row(j + D, S1(), S2(), S0(), std::true_type());
+      row(j + 2 * D, S2(), S0(), S1(), std::true_type());
+    }
+    row(j, S0(), S1(), S2(), std::false_type());
+  };
+  if (up)
+    march(std::true_type());
+  else
+    march(std::false_type());
   if (a.scratch) return;
 
   publish_partial<2, COMP>(acc_s, acc_c, s.part_g, t.n, t.tile, s.tiles_x * s.tiles_y);
@@ -716,7 +745,7 @@ __device__ __forceinline__ void p_process(PRow<T, NC>& w, const PRaw<T, NC>& r, 
 }
 
 template <class T, int NC, bool NSF, bool PL>
-__device__ void poly_body(const StepArgs& a, const DevString& s, const WorkItem& it) {
+__device__ __forceinline__ void poly_body(const StepArgs& a, const DevString& s, const WorkItem& it) {
   constexpr bool COMP = sizeof(T) == 4;
   const Strip t = strip_of(s, it);
   const int N = s.N, P_ = s.pitch;
@@ -744,131 +773,145 @@ __device__ void poly_body(const StepArgs& a, const DevString& s, const WorkItem&
   double acc_s[5] = {0, 0, 0, 0, 0}, acc_c[5] = {0, 0, 0, 0, 0};
   auto nnoff = [&](int row) { return (unsigned)((row * P_ + t.ic) * (int)sizeof(uint16_t)); };
 
-  // Three row slots in rotation, as in the G kernel: w[IP] is the previous
-  // row (only its A_x and direction are read), w[IC] the current, w[IN] the
-  // next; the parameters of a row (parameter planes) sit in the slot of the
-  // same index.
-  PRaw<T, NC> raw;
-  PRow<T, NC> w[3];
-  RowParams<T, NC, PL> rp[3];
-  p_load<T, NC, NSF>(raw, b, clampr(t.j0 - 1, N), P_, t, nnoff(clampr(t.j0 - 1, N)));
-  p_process<T, NC>(w[0], raw, beta);
-  p_load<T, NC, NSF>(raw, b, clampr(t.j0, N), P_, t, nnoff(clampr(t.j0, N)));
-  p_process<T, NC>(w[1], raw, beta);
-  p_load<T, NC, NSF>(raw, b, clampr(t.j0 + 1, N), P_, t, nnoff(clampr(t.j0 + 1, N)));
-  p_process<T, NC>(w[2], raw, beta);
-  rp[1].load(pv, hp, (long long)t.j0 * P_ + t.ic, t.j0, N, P_, t);
-
-  // Row j: one batch of loads (the fields of row j + 2, the parameters of
-  // row j + 1), the quartic of row j from registers, the batch into the
-  // previous row's slot, then the stores of row j (see the G loop).
-  auto row = [&](int j, auto ip_tag, auto ic_tag, auto in_tag, auto batch_tag) {
-    constexpr int IP = decltype(ip_tag)::value, IC = decltype(ic_tag)::value;
-    constexpr int IN = decltype(in_tag)::value;
-    constexpr bool BATCH = decltype(batch_tag)::value;
-    PRow<T, NC>& wp = w[IP];
-    PRow<T, NC>& wc = w[IC];
-    PRow<T, NC>& wn = w[IN];
-    const bool live = store && j < t.j1;
-    if constexpr (BATCH) {
-      p_load<T, NC, NSF>(raw, b, clampr(j + 2, N), P_, t, nnoff(clampr(j + 2, N)));
-      const int r1 = clampr(j + 1, N);
-      rp[IN].load(pv, hp, (long long)r1 * P_ + t.ic, r1, N, P_, t);
-    }
-    __builtin_amdgcn_sched_barrier(0);
-
-    PolyNbhd<T, NC> nb;
-    nb.ax[0] = wc.ax;
-    nb.ax[1] = from_right(wc.ax, wc.eax);
-    nb.ax[2] = 0;
-    nb.ax[3] = wn.ax;
-    nb.ax[4] = wp.ax;
-    nb.ay[0] = wc.ay;
-    nb.ay[1] = from_right(wc.ay, wc.eay);
-    nb.ay[2] = from_left(wc.ay, wc.eay);
-    nb.ay[3] = wn.ay;
-    nb.ay[4] = 0;
-    nb.dax[0] = wc.dax;
-    nb.dax[1] = from_right(wc.dax, wc.edax);
-    nb.dax[2] = 0;
-    nb.dax[3] = wn.dax;
-    nb.dax[4] = wp.dax;
-    nb.day[0] = wc.day;
-    nb.day[1] = from_right(wc.day, wc.eday);
-    nb.day[2] = from_left(wc.day, wc.eday);
-    nb.day[3] = wn.day;
-    nb.day[4] = 0;
-#pragma unroll
-    for (int q = 0; q < NC; ++q) {
-      nb.u[q][0] = wc.u[q];
-      nb.v[q][0] = wc.v[q];
-      nb.du[q][0] = wc.du[q];
-      nb.dv[q][0] = wc.dv[q];
-      nb.u[q][1] = from_right(wc.u[q], wc.eu[q]);
-      nb.v[q][1] = from_right(wc.v[q], wc.ev[q]);
-      nb.du[q][1] = from_right(wc.du[q], wc.edu[q]);
-      nb.dv[q][1] = from_right(wc.dv[q], wc.edv[q]);
-      nb.u[q][2] = wn.u[q];
-      nb.v[q][2] = wn.v[q];
-      nb.du[q][2] = wn.du[q];
-      nb.dv[q][2] = wn.dv[q];
-    }
-    CompConst<T> k[NC];
-    if constexpr (PL) {
-      comp_from_params<T, NC>(rp[IC].prm, pv.q, k);
-      nb.h0 = hp ? rp[IC].h[0] : hv;
-    } else {
-#pragma unroll
-      for (int q = 0; q < NC; ++q) k[q] = kconst[q];
-      nb.h0 = hv;
-    }
-
-    if (live) {
-      const Quartic<T> qv =
-          site_poly<T, NC>(nb, k, eta, gam, del, idx, dx2, wc.nn & 0xff, wc.nn >> 8, nsf, exact);
-#pragma unroll
-      for (int r = 0; r < 5; ++r) acc_s[r] += (double)qv.c[r];
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    if constexpr (BATCH) {
-      p_process<T, NC>(wp, raw, beta);
-      pin_row<T, NC>(wp);
-      if constexpr (PL) {
-#pragma unroll
-        for (int q = 0; q < NC; ++q)
-#pragma unroll
-          for (int k2 = 0; k2 < 4; ++k2) pin(rp[IN].prm[q][k2]);
-#pragma unroll
-        for (int k2 = 0; k2 < 5; ++k2) pin(rp[IN].h[k2]);
-      }
-      memory_fence_compiler();
-    }
-    __builtin_amdgcn_sched_barrier(0);
-    if (live) {
-      const unsigned o = (unsigned)((j * P_ + t.ic) * (int)sizeof(T));
-      if (!nsf) {
-        bstore(wc.dax, b.dout, o, PAX * b.ps);
-        bstore(wc.day, b.dout, o, PAY * b.ps);
-      }
-#pragma unroll
-      for (int q = 0; q < NC; ++q) {
-        bstore(wc.du[q], b.dout, o, (PU1 + 2 * q) * b.ps);
-        bstore(wc.dv[q], b.dout, o, (PV1 + 2 * q) * b.ps);
-      }
-    }
-  };
+  // The march is the only part that depends on the direction
+  // (marches_up); everything it needs from the string's descriptor is
+  // read above, before the branch and before any store, so it stays in
+  // scalar registers.
+  const bool up = marches_up(s, it);
   using S0 = std::integral_constant<int, 0>;
   using S1 = std::integral_constant<int, 1>;
   using S2 = std::integral_constant<int, 2>;
-  const int groups = (t.j1 - 1 - t.j0 + 2) / 3;
-  int j = t.j0;
+  auto march = [&](auto up_tag) {
+    constexpr bool UP = decltype(up_tag)::value;
+    constexpr int D = UP ? -1 : 1;  // march direction in rows
+    // Three row slots in rotation, as in the G kernel: w[IP] is the previous
+    // row of the march, w[IC] the current, w[IN] the next; the parameters of a
+    // row (parameter planes) sit in the slot of the same index.  Of row j - 1
+    // only A_x and its direction are read.
+    const int jb = UP ? t.j1 - 1 : t.j0;  // first row of the march
+    PRaw<T, NC> raw;
+    PRow<T, NC> w[3];
+    RowParams<T, NC, PL> rp[3];
+    p_load<T, NC, NSF>(raw, b, clampr(jb - D, N), P_, t, nnoff(clampr(jb - D, N)));
+    p_process<T, NC>(w[0], raw, beta);
+    p_load<T, NC, NSF>(raw, b, clampr(jb, N), P_, t, nnoff(clampr(jb, N)));
+    p_process<T, NC>(w[1], raw, beta);
+    p_load<T, NC, NSF>(raw, b, clampr(jb + D, N), P_, t, nnoff(clampr(jb + D, N)));
+    p_process<T, NC>(w[2], raw, beta);
+    rp[1].load(pv, hp, (long long)jb * P_ + t.ic, jb, N, P_, t);
+
+    // Row j: one batch of loads (the fields of row j + 2D, the parameters of
+    // row j + D), the quartic of row j from registers, the batch into the
+    // previous row's slot, then the stores of row j (see the G loop).
+    auto row = [&](int j, auto ip_tag, auto ic_tag, auto in_tag, auto batch_tag) {
+      constexpr int IP = decltype(ip_tag)::value, IC = decltype(ic_tag)::value;
+      constexpr int IN = decltype(in_tag)::value;
+      constexpr bool BATCH = decltype(batch_tag)::value;
+      PRow<T, NC>& wm = w[UP ? IN : IP];  // row j - 1
+      PRow<T, NC>& wc = w[IC];
+      PRow<T, NC>& wpl = w[UP ? IP : IN];  // row j + 1
+      const bool live = store && (UP ? j >= t.j0 : j < t.j1);
+      if constexpr (BATCH) {
+        p_load<T, NC, NSF>(raw, b, clampr(j + 2 * D, N), P_, t, nnoff(clampr(j + 2 * D, N)));
+        const int r1 = clampr(j + D, N);
+        rp[IN].load(pv, hp, (long long)r1 * P_ + t.ic, r1, N, P_, t);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+
+      PolyNbhd<T, NC> nb;
+      nb.ax[0] = wc.ax;
+      nb.ax[1] = from_right(wc.ax, wc.eax);
+      nb.ax[2] = 0;
+      nb.ax[3] = wpl.ax;
+      nb.ax[4] = wm.ax;
+      nb.ay[0] = wc.ay;
+      nb.ay[1] = from_right(wc.ay, wc.eay);
+      nb.ay[2] = from_left(wc.ay, wc.eay);
+      nb.ay[3] = wpl.ay;
+      nb.ay[4] = 0;
+      nb.dax[0] = wc.dax;
+      nb.dax[1] = from_right(wc.dax, wc.edax);
+      nb.dax[2] = 0;
+      nb.dax[3] = wpl.dax;
+      nb.dax[4] = wm.dax;
+      nb.day[0] = wc.day;
+      nb.day[1] = from_right(wc.day, wc.eday);
+      nb.day[2] = from_left(wc.day, wc.eday);
+      nb.day[3] = wpl.day;
+      nb.day[4] = 0;
+#pragma unroll
+      for (int q = 0; q < NC; ++q) {
+        nb.u[q][0] = wc.u[q];
+        nb.v[q][0] = wc.v[q];
+        nb.du[q][0] = wc.du[q];
+        nb.dv[q][0] = wc.dv[q];
+        nb.u[q][1] = from_right(wc.u[q], wc.eu[q]);
+        nb.v[q][1] = from_right(wc.v[q], wc.ev[q]);
+        nb.du[q][1] = from_right(wc.du[q], wc.edu[q]);
+        nb.dv[q][1] = from_right(wc.dv[q], wc.edv[q]);
+        nb.u[q][2] = wpl.u[q];
+        nb.v[q][2] = wpl.v[q];
+        nb.du[q][2] = wpl.du[q];
+        nb.dv[q][2] = wpl.dv[q];
+      }
+      CompConst<T> k[NC];
+      if constexpr (PL) {
+        comp_from_params<T, NC>(rp[IC].prm, pv.q, k);
+        nb.h0 = hp ? rp[IC].h[0] : hv;
+      } else {
+#pragma unroll
+        for (int q = 0; q < NC; ++q) k[q] = kconst[q];
+        nb.h0 = hv;
+      }
+
+      if (live) {
+        const Quartic<T> qv =
+            site_poly<T, NC>(nb, k, eta, gam, del, idx, dx2, wc.nn & 0xff, wc.nn >> 8, nsf, exact);
+#pragma unroll
+        for (int r = 0; r < 5; ++r) acc_s[r] += (double)qv.c[r];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      if constexpr (BATCH) {
+        p_process<T, NC>(w[IP], raw, beta);
+        pin_row<T, NC>(w[IP]);
+        if constexpr (PL) {
+#pragma unroll
+          for (int q = 0; q < NC; ++q)
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) pin(rp[IN].prm[q][k2]);
+#pragma unroll
+          for (int k2 = 0; k2 < 5; ++k2) pin(rp[IN].h[k2]);
+        }
+        memory_fence_compiler();
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      if (live) {
+        const unsigned o = (unsigned)((j * P_ + t.ic) * (int)sizeof(T));
+        if (!nsf) {
+          bstore(wc.dax, b.dout, o, PAX * b.ps);
+          bstore(wc.day, b.dout, o, PAY * b.ps);
+        }
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+          bstore(wc.du[q], b.dout, o, (PU1 + 2 * q) * b.ps);
+          bstore(wc.dv[q], b.dout, o, (PV1 + 2 * q) * b.ps);
+        }
+      }
+    };
+    const int groups = (t.j1 - 1 - t.j0 + 2) / 3;
+    int j = jb;
 #pragma clang loop unroll(disable)
-  for (int g = 0; g < groups; ++g, j += 3) {
-    row(j, S0(), S1(), S2(), std::true_type());
-    row(j + 1, S1(), S2(), S0(), std::true_type());
-    row(j + 2, S2(), S0(), S1(), std::true_type());
-  }
-  row(j, S0(), S1(), S2(), std::false_type());
+    for (int g = 0; g < groups; ++g, j += 3 * D) {
+      row(j, S0(), S1(), S2(), std::true_type());
+      row(j + D, S1(), S2(), S0(), std::true_type());
+      row(j + 2 * D, S2(), S0(), S1(), std::true_type());
+    }
+    row(j, S0(), S1(), S2(), std::false_type());
+  };
+  if (up)
+    march(std::true_type());
+  else
+    march(std::false_type());
 
   publish_partial<5, COMP>(acc_s, acc_c, s.part_p, t.n, t.tile, s.tiles_x * s.tiles_y);
 }
