@@ -170,36 +170,39 @@ __device__ __forceinline__ void site_gradient(const Nbhd<T, NC>& s, const CompCo
   const T AX0 = s.ax[1][1], AY0 = s.ay[1][1];
   T gax = 0, gay = 0;
 
-  const bool bx0 = bitof(sc, 0), bx4 = bitof(sc, 4), by2 = bitof(sc, 2), by6 = bitof(sc, 6);
+  // Each bond contributes only when its neighbour exists (bits 0, 4, 2, 6).
+  // Its presence is folded into its weight once per site: an absent bond
+  // gets weight 0, and every term it adds is then an exact zero (the fields
+  // are finite), so no lane diverges and no term needs a select.
+  const T wxf0 = bitof(sc, 0) ? wxf : (T)0, wxb0 = bitof(sc, 4) ? wxb : (T)0;
+  const T wyf0 = bitof(sc, 2) ? wyf : (T)0, wyb0 = bitof(sc, 6) ? wyb : (T)0;
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const T u0 = s.u[c][0], v0 = s.v[c][0];
     const T n2 = u0 * u0 + v0 * v0;
     T gu = (k[c].two_a * u0 + k[c].two_b * u0 * n2) * af;
     T gv = (k[c].two_a * v0 + k[c].two_b * v0 * n2) * af;
-    // Each bond contributes only when its neighbour exists; the terms are
-    // always evaluated and selected, so no lane diverges.
     T bu, bv, bA;
     bu = bv = bA = 0;
-    bond_fwd(u0, v0, AX0, s.u[c][1], s.v[c][1], s.ax[1][2], k[c].q, k[c].hx, wxf, idx, bu, bv, bA);
-    gu += bx0 ? bu : (T)0;
-    gv += bx0 ? bv : (T)0;
-    gax += bx0 ? bA : (T)0;
+    bond_fwd(u0, v0, AX0, s.u[c][1], s.v[c][1], s.ax[1][2], k[c].q, k[c].hx, wxf0, idx, bu, bv, bA);
+    gu += bu;
+    gv += bv;
+    gax += bA;
     bu = bv = bA = 0;
-    bond_bwd(u0, v0, AX0, s.u[c][2], s.v[c][2], s.ax[1][0], k[c].q, k[c].hx, wxb, idx, bu, bv, bA);
-    gu += bx4 ? bu : (T)0;
-    gv += bx4 ? bv : (T)0;
-    gax += bx4 ? bA : (T)0;
+    bond_bwd(u0, v0, AX0, s.u[c][2], s.v[c][2], s.ax[1][0], k[c].q, k[c].hx, wxb0, idx, bu, bv, bA);
+    gu += bu;
+    gv += bv;
+    gax += bA;
     bu = bv = bA = 0;
-    bond_fwd(u0, v0, AY0, s.u[c][3], s.v[c][3], s.ay[2][1], k[c].q, k[c].hy, wyf, idx, bu, bv, bA);
-    gu += by2 ? bu : (T)0;
-    gv += by2 ? bv : (T)0;
-    gay += by2 ? bA : (T)0;
+    bond_fwd(u0, v0, AY0, s.u[c][3], s.v[c][3], s.ay[2][1], k[c].q, k[c].hy, wyf0, idx, bu, bv, bA);
+    gu += bu;
+    gv += bv;
+    gay += bA;
     bu = bv = bA = 0;
-    bond_bwd(u0, v0, AY0, s.u[c][4], s.v[c][4], s.ay[0][1], k[c].q, k[c].hy, wyb, idx, bu, bv, bA);
-    gu += by6 ? bu : (T)0;
-    gv += by6 ? bv : (T)0;
-    gay += by6 ? bA : (T)0;
+    bond_bwd(u0, v0, AY0, s.u[c][4], s.v[c][4], s.ay[0][1], k[c].q, k[c].hy, wyb0, idx, bu, bv, bA);
+    gu += bu;
+    gv += bv;
+    gay += bA;
     gpsi[c][0] = gu;
     gpsi[c][1] = gv;
   }
@@ -231,37 +234,38 @@ __device__ __forceinline__ void site_gradient(const Nbhd<T, NC>& s, const CompCo
     const T dxf = (X[2][1] - X[1][1]) * idx;  // forward y-difference of A_x
     const T dxb = (X[1][1] - X[0][1]) * idx;
     const T h0 = s.h[0];
-    const bool q0 = bitof(cm, 0) && bitof(cm, 1) && bitof(cm, 2);
-    const bool q1 = bitof(cm, 2) && bitof(cm, 3) && bitof(cm, 4);
-    const bool q2 = bitof(cm, 4) && bitof(cm, 5) && bitof(cm, 6);
-    const bool q3 = bitof(cm, 6) && bitof(cm, 7) && bitof(cm, 0);
+    // Quadrant presence folded into the quadrant's weight, as for the bonds.
+    const T c0 = bitof(cm, 0) && bitof(cm, 1) && bitof(cm, 2) ? c4 : (T)0;
+    const T c1 = bitof(cm, 2) && bitof(cm, 3) && bitof(cm, 4) ? c4 : (T)0;
+    const T c2 = bitof(cm, 4) && bitof(cm, 5) && bitof(cm, 6) ? c4 : (T)0;
+    const T c3 = bitof(cm, 6) && bitof(cm, 7) && bitof(cm, 0) ? c4 : (T)0;
     {
       const T e = dyf - dxf - h0;
       const T n1 = dyf - (X[2][2] - X[1][2]) * idx - s.h[1];
       const T n2 = (Y[2][2] - Y[2][1]) * idx - dxf - s.h[3];
-      gax += q0 ? c4 * e + c4 * n2 : (T)0;
-      gay += q0 ? -c4 * e - c4 * n1 : (T)0;
+      gax += c0 * e + c0 * n2;
+      gay += -c0 * e - c0 * n1;
     }
     {
       const T e = dyb - dxf - h0;
       const T n1 = (Y[2][1] - Y[2][0]) * idx - dxf - s.h[3];
       const T n2 = dyb - (X[2][0] - X[1][0]) * idx - s.h[2];
-      gax += q1 ? c4 * e + c4 * n1 : (T)0;
-      gay += q1 ? c4 * e + c4 * n2 : (T)0;
+      gax += c1 * e + c1 * n1;
+      gay += c1 * e + c1 * n2;
     }
     {
       const T e = dyb - dxb - h0;
       const T n1 = dyb - (X[1][0] - X[0][0]) * idx - s.h[2];
       const T n2 = (Y[0][1] - Y[0][0]) * idx - dxb - s.h[4];
-      gax += q2 ? -c4 * e - c4 * n2 : (T)0;
-      gay += q2 ? c4 * e + c4 * n1 : (T)0;
+      gax += -c2 * e - c2 * n2;
+      gay += c2 * e + c2 * n1;
     }
     {
       const T e = dyf - dxb - h0;
       const T n1 = (Y[0][2] - Y[0][1]) * idx - dxb - s.h[4];
       const T n2 = dyf - (X[1][2] - X[0][2]) * idx - s.h[1];
-      gax += q3 ? -c4 * e - c4 * n1 : (T)0;
-      gay += q3 ? -c4 * e - c4 * n2 : (T)0;
+      gax += -c3 * e - c3 * n1;
+      gay += -c3 * e - c3 * n2;
     }
     ga[0] = gax * dx2;
     ga[1] = gay * dx2;
@@ -380,28 +384,30 @@ __device__ __forceinline__ Quartic<T> site_poly(const PolyNbhd<T, NC>& s,
     const T xf = (s.ax[3] - s.ax[0]) * idx, xb = (s.ax[0] - s.ax[4]) * idx;
     const T dyf = (s.day[1] - s.day[0]) * idx, dyb = (s.day[0] - s.day[2]) * idx;
     const T dxf = (s.dax[3] - s.dax[0]) * idx, dxb = (s.dax[0] - s.dax[4]) * idx;
-    const bool q0 = bitof(cm, 0) && bitof(cm, 1) && bitof(cm, 2);
-    const bool q1 = bitof(cm, 2) && bitof(cm, 3) && bitof(cm, 4);
-    const bool q2 = bitof(cm, 4) && bitof(cm, 5) && bitof(cm, 6);
-    const bool q3 = bitof(cm, 6) && bitof(cm, 7) && bitof(cm, 0);
+    // Quadrant presence folded into the quadrant's weight (absent: 0).
+    T fq[4];
+    fq[0] = bitof(cm, 0) && bitof(cm, 1) && bitof(cm, 2) ? f : (T)0;
+    fq[1] = bitof(cm, 2) && bitof(cm, 3) && bitof(cm, 4) ? f : (T)0;
+    fq[2] = bitof(cm, 4) && bitof(cm, 5) && bitof(cm, 6) ? f : (T)0;
+    fq[3] = bitof(cm, 6) && bitof(cm, 7) && bitof(cm, 0) ? f : (T)0;
     T e0[4], e1[4];
-    bool on[4] = {q0, q1, q2, q3};
     e0[0] = yf - xf - s.h0;  e1[0] = dyf - dxf;
     e0[1] = yb - xf - s.h0;  e1[1] = dyb - dxf;
     e0[2] = yb - xb - s.h0;  e1[2] = dyb - dxb;
     e0[3] = yf - xb - s.h0;  e1[3] = dyf - dxb;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-      r.c[0] += on[t] ? (T)0.5 * e0[t] * e0[t] * f : (T)0;
-      r.c[1] += on[t] ? e0[t] * e1[t] * f : (T)0;
-      r.c[2] += on[t] ? (T)0.5 * e1[t] * e1[t] * f : (T)0;
+      r.c[0] += (T)0.5 * e0[t] * e0[t] * fq[t];
+      r.c[1] += e0[t] * e1[t] * fq[t];
+      r.c[2] += (T)0.5 * e1[t] * e1[t] * fq[t];
     }
   }
   // Kinetic bonds, nlcg.cu:1160-1366.
   const T cxf = (T)(bitof(sc, 1) * bitof(sc, 2) + bitof(sc, 6) * bitof(sc, 7)) * (T)0.5;
   const T cyf = (T)(bitof(sc, 0) * bitof(sc, 1) + bitof(sc, 3) * bitof(sc, 4)) * (T)0.5;
   const T c2y = exact ? (T)1 : (T)0.5;
-  const bool bx = bitof(sc, 0), by = bitof(sc, 2);
+  // Bond presence (bits 0 and 2) folded into the bond weights (absent: 0).
+  const T bx = bitof(sc, 0) ? (T)1 : (T)0, by = bitof(sc, 2) ? (T)1 : (T)0;
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
     const T q = k[c].q;
@@ -411,12 +417,12 @@ __device__ __forceinline__ Quartic<T> site_poly(const PolyNbhd<T, NC>& s,
     for (int t = 0; t < 5; ++t) rx.c[t] = ry.c[t] = 0;
     poly_bond(s.u[c][0], s.v[c][0], s.u[c][1], s.v[c][1], s.du[c][0], s.dv[c][0], s.du[c][1],
               s.dv[c][1], s.ax[0], s.ax[1], s.dax[0], s.dax[1], q, c1x, (T)1,
-              k[c].hx * cxf * dx2, idx, rx);
+              k[c].hx * cxf * dx2 * bx, idx, rx);
     poly_bond(s.u[c][0], s.v[c][0], s.u[c][2], s.v[c][2], s.du[c][0], s.dv[c][0], s.du[c][2],
               s.dv[c][2], s.ay[0], s.ay[3], s.day[0], s.day[3], q, -q, c2y,
-              k[c].hy * cyf * dx2, idx, ry);
+              k[c].hy * cyf * dx2 * by, idx, ry);
 #pragma unroll
-    for (int t = 0; t < 5; ++t) r.c[t] += (bx ? rx.c[t] : (T)0) + (by ? ry.c[t] : (T)0);
+    for (int t = 0; t < 5; ++t) r.c[t] += rx.c[t] + ry.c[t];
   }
   return r;
 }
