@@ -111,7 +111,7 @@ int dev_alloc(mepgl_ctx* c, StringHost* s, void** p, size_t bytes) {
 
 void free_string(mepgl_ctx* c, StringHost& s) {
   for (auto& a : s.owned) {
-    hipFree(a.p);
+    (void)hipFree(a.p);
     c->bytes -= (long long)a.bytes;
   }
   s.owned.clear();
@@ -188,14 +188,17 @@ AuxArgs aux_args(mepgl_ctx* c) {
   return a;
 }
 
-hipEvent_t take_event(mepgl_ctx* c) {
+// Records the next event of the timing pool on the stream, creating it on
+// first use.
+hipError_t record_timing_event(mepgl_ctx* c) {
   Timing& t = c->timing;
   if (t.used == t.pool.size()) {
     hipEvent_t e;
-    hipEventCreate(&e);
+    const hipError_t err = hipEventCreate(&e);
+    if (err != hipSuccess) return err;
     t.pool.push_back(e);
   }
-  return t.pool[t.used++];
+  return hipEventRecord(t.pool[t.used++], c->stream);
 }
 
 int run_step_group(mepgl_ctx* c, const StepArgs& a, int which);
@@ -219,13 +222,13 @@ int run_step(mepgl_ctx* c, const StepArgs& a0, int which) {
 int run_step_group(mepgl_ctx* c, const StepArgs& a, int which) {
   if (c->timing.on && which != kKernelUpdate) {
     const size_t s0 = c->timing.used;
-    hipEventRecord(take_event(c), c->stream);
-    hipError_t e = launch_step(a, which, c->fp32(), c->stream);
-    hipEventRecord(take_event(c), c->stream);
+    hipError_t e = record_timing_event(c);
+    if (e == hipSuccess) e = launch_step(a, which, c->fp32(), c->stream);
+    if (e == hipSuccess) e = record_timing_event(c);
+    if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, std::string("launch: ") + hipGetErrorString(e));
     // Timing ids: 0 = G with the pending update, 1 = P, 2 = G without it.
     const int tid = (which == kKernelGrad && !a.upd) ? (int)kKernelUpdate : which;
     c->timing.marks.push_back({tid, {s0, s0 + 1}});
-    if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, std::string("launch: ") + hipGetErrorString(e));
     return MEPGL_OK;
   }
   hipError_t e = launch_step(a, which, c->fp32(), c->stream);
@@ -260,13 +263,12 @@ int run_finish(mepgl_ctx* c, const FinishArgs& f, int which) {
   if (c->dry) return MEPGL_OK;
   const bool timed = c->timing.on && (which == kKernelGrad || which == kKernelPoly);
   const size_t s0 = c->timing.used;
-  if (timed) hipEventRecord(take_event(c), c->stream);
-  hipError_t e = launch_finish(f, which, c->fp32(), c->stream);
-  if (timed) {
-    hipEventRecord(take_event(c), c->stream);
-    c->timing.marks.push_back({which == kKernelGrad ? kTimeFinishGrad : kTimeFinishPoly, {s0, s0 + 1}});
-  }
+  hipError_t e = timed ? record_timing_event(c) : hipSuccess;
+  if (e == hipSuccess) e = launch_finish(f, which, c->fp32(), c->stream);
+  if (timed && e == hipSuccess) e = record_timing_event(c);
   if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, std::string("launch: ") + hipGetErrorString(e));
+  if (timed)
+    c->timing.marks.push_back({which == kKernelGrad ? kTimeFinishGrad : kTimeFinishPoly, {s0, s0 + 1}});
   return MEPGL_OK;
 }
 
@@ -356,7 +358,7 @@ int build_items(mepgl_ctx* c) {
         if (count > 0) c->groups.push_back({begin, count, nc, nsf, pl});
       }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  if (c->d_items) hipFree(c->d_items);
+  if (c->d_items) (void)hipFree(c->d_items);
   c->d_items = nullptr;
   c->nitems = (int)items.size();
   HIP_TRY(c, hipMalloc(&c->d_items, sizeof(WorkItem) * std::max<size_t>(items.size(), 1)));
@@ -448,8 +450,8 @@ int alloc_scalars(mepgl_ctx* c) {
   }
   c->total_images = img;
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  if (c->d_sc) hipFree(c->d_sc);
-  if (c->d_out) hipFree(c->d_out);
+  if (c->d_sc) (void)hipFree(c->d_sc);
+  if (c->d_out) (void)hipFree(c->d_out);
   c->d_sc = nullptr;
   c->d_out = nullptr;
   HIP_TRY(c, hipMalloc(&c->d_sc, sizeof(ImageScalars) * img));
@@ -459,7 +461,7 @@ int alloc_scalars(mepgl_ctx* c) {
   std::vector<int2> map;
   for (size_t si = 0; si < c->strings.size(); ++si)
     for (int n = 0; n < c->strings[si].dev.F; ++n) map.push_back(make_int2((int)si, n));
-  if (c->d_img) hipFree(c->d_img);
+  if (c->d_img) (void)hipFree(c->d_img);
   c->d_img = nullptr;
   HIP_TRY(c, hipMalloc(&c->d_img, sizeof(int2) * map.size()));
   HIP_TRY(c, hipMemcpyAsync(c->d_img, map.data(), sizeof(int2) * map.size(),
@@ -469,7 +471,7 @@ int alloc_scalars(mepgl_ctx* c) {
 }
 
 void destroy_graphs(mepgl_ctx* c) {
-  for (auto& g : c->graphs) hipGraphExecDestroy(g.second);
+  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.second);
   c->graphs.clear();
 }
 
@@ -673,9 +675,9 @@ struct Snapshot {
 };
 
 void drop_snapshot(mepgl_ctx* c, Snapshot& sn) {
-  hipStreamSynchronize(c->stream);
-  for (auto& b : sn.buf) hipFree(b.first);
-  if (sn.sc) hipFree(sn.sc);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto& b : sn.buf) (void)hipFree(b.first);
+  if (sn.sc) (void)hipFree(sn.sc);
   sn.buf.clear();
   sn.sc = nullptr;
 }
@@ -727,15 +729,18 @@ int time_schedule(mepgl_ctx* c, const Snapshot& sn, int steps, double* ms) {
   if ((rc = nlcg_step(c, 0, 1, 0))) return rc;
   hipEvent_t e0, e1;
   HIP_TRY(c, hipEventCreate(&e0));
-  HIP_TRY(c, hipEventCreate(&e1));
-  hipEventRecord(e0, c->stream);
-  for (int k = 0; k < steps && !rc; ++k) rc = nlcg_step(c, k + 1, 1, 0);
-  hipEventRecord(e1, c->stream);
-  hipError_t e = hipEventSynchronize(e1);
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(c, MEPGL_ERR_HIP, "autotune timing: event creation failed");
+  }
+  hipError_t e = hipEventRecord(e0, c->stream);
+  for (int k = 0; k < steps && !rc && e == hipSuccess; ++k) rc = nlcg_step(c, k + 1, 1, 0);
+  if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
   float t = 0;
   if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
-  hipEventDestroy(e0);
-  hipEventDestroy(e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   if (rc) return rc;
   if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, std::string("autotune timing: ") + hipGetErrorString(e));
   *ms = (double)t / std::max(steps, 1);
@@ -842,15 +847,21 @@ int mepgl_create(const mepgl_string_desc* strings, int n_strings, int precision,
       *out = c.release();
       return rc;
     }
-    hipMemsetAsync(nn, 0, (size_t)in.N * d.pitch * sizeof(uint16_t), c->stream);
-    hipMemcpyAsync(dsc, in.sc_domain, db, hipMemcpyHostToDevice, c->stream);
-    hipMemcpyAsync(dcm, in.comp_domain, db, hipMemcpyHostToDevice, c->stream);
-    launch_nn_map((const int*)dsc, (const int*)dcm, in.N, d.pitch, (uint16_t*)nn, c->stream);
-    hipStreamSynchronize(c->stream);
-    hipFree(dsc);
-    hipFree(dcm);
+    hipError_t he = hipMemsetAsync(nn, 0, (size_t)in.N * d.pitch * sizeof(uint16_t), c->stream);
+    if (he == hipSuccess) he = hipMemcpyAsync(dsc, in.sc_domain, db, hipMemcpyHostToDevice, c->stream);
+    if (he == hipSuccess)
+      he = hipMemcpyAsync(dcm, in.comp_domain, db, hipMemcpyHostToDevice, c->stream);
+    if (he == hipSuccess)
+      he = launch_nn_map((const int*)dsc, (const int*)dcm, in.N, d.pitch, (uint16_t*)nn, c->stream);
+    if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+    (void)hipFree(dsc);
+    (void)hipFree(dcm);
     c->bytes -= 2 * (long long)db;
     d.nn = (const uint16_t*)nn;
+    if (he != hipSuccess) {
+      *out = c.release();
+      return fail(*out, MEPGL_ERR_HIP, std::string("neighbour map: ") + hipGetErrorString(he));
+    }
     if ((rc = alloc_images(cp, s, in.F))) {
       *out = c.release();
       return rc;
@@ -869,9 +880,14 @@ int mepgl_create(const mepgl_string_desc* strings, int n_strings, int precision,
     *out = c.release();
     return fail(*out, MEPGL_ERR_NOMEM, "line-search table");
   }
-  hipMemcpyAsync(c->d_ls, t, sizeof(LineSearchTable), hipMemcpyHostToDevice, c->stream);
-  hipStreamSynchronize(c->stream);
+  hipError_t he = hipMemcpyAsync(c->d_ls, t, sizeof(LineSearchTable), hipMemcpyHostToDevice,
+                                 c->stream);
+  if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
   delete t;
+  if (he != hipSuccess) {
+    *out = c.release();
+    return fail(*out, MEPGL_ERR_HIP, std::string("line-search table: ") + hipGetErrorString(he));
+  }
   if ((rc = reset_solver(cp))) {
     *out = c.release();
     return rc;
@@ -886,19 +902,19 @@ int mepgl_create(const mepgl_string_desc* strings, int n_strings, int precision,
 
 void mepgl_destroy(mepgl_ctx* c) {
   if (!c) return;
-  hipSetDevice(c->device);
-  if (c->stream) hipStreamSynchronize(c->stream);
+  (void)hipSetDevice(c->device);  // best effort
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   destroy_graphs(c);
   for (auto& s : c->strings) free_string(c, s);
-  hipFree(c->d_strings);
-  hipFree(c->d_sc);
-  hipFree(c->d_img);
-  hipFree(c->d_ls);
-  hipFree(c->d_items);
-  hipFree(c->d_jb);
-  hipFree(c->d_out);
-  for (auto e : c->timing.pool) hipEventDestroy(e);
-  if (c->stream) hipStreamDestroy(c->stream);
+  (void)hipFree(c->d_strings);
+  (void)hipFree(c->d_sc);
+  (void)hipFree(c->d_img);
+  (void)hipFree(c->d_ls);
+  (void)hipFree(c->d_items);
+  (void)hipFree(c->d_jb);
+  (void)hipFree(c->d_out);
+  for (auto e : c->timing.pool) (void)hipEventDestroy(e);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
@@ -908,7 +924,7 @@ int mepgl_upload(mepgl_ctx* c, int si, const double* a, const double* p1, const 
   if (!valid_string(c, si) || !a || !p1) return fail(c, MEPGL_ERR_ARG, "upload: bad arguments");
   StringHost& s = c->strings[si];
   if (s.dev.nc == 2 && !p2) return fail(c, MEPGL_ERR_ARG, "upload: psi_2 required");
-  hipSetDevice(c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
   int rc = flush(c);  // other strings keep their owed update
   if (rc) return rc;
   const size_t n = (size_t)s.dev.F * s.dev.image_stride;
@@ -932,7 +948,7 @@ int mepgl_upload(mepgl_ctx* c, int si, const double* a, const double* p1, const 
 
 int mepgl_download(mepgl_ctx* c, int si, double* a, double* p1, double* p2) {
   if (!valid_string(c, si) || !a || !p1) return fail(c, MEPGL_ERR_ARG, "download: bad arguments");
-  hipSetDevice(c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
   int rc = flush(c);
   if (rc) return rc;
   return copy_out_fields(c, si, c->strings[si].dev.x[c->xp], a, p1, p2);
@@ -940,7 +956,7 @@ int mepgl_download(mepgl_ctx* c, int si, double* a, double* p1, double* p2) {
 
 int mepgl_reset_solver(mepgl_ctx* c) {
   if (!c) return MEPGL_ERR_ARG;
-  hipSetDevice(c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
   int rc = flush(c);
   if (rc) return rc;
   return reset_solver(c);
@@ -948,7 +964,7 @@ int mepgl_reset_solver(mepgl_ctx* c) {
 
 int mepgl_winding(mepgl_ctx* c) {
   if (!c) return MEPGL_ERR_ARG;
-  hipSetDevice(c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
   for (auto& s : c->strings)
     if (s.dev.N < 2) return fail(c, MEPGL_ERR_ARG, "winding needs N >= 2");
   int rc = flush(c);
@@ -958,7 +974,7 @@ int mepgl_winding(mepgl_ctx* c) {
 
 int mepgl_free_energy(mepgl_ctx* c) {
   if (!c) return MEPGL_ERR_ARG;
-  hipSetDevice(c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
   int rc = flush(c);
   if (rc) return rc;
   AuxArgs a = aux_args(c);
@@ -968,7 +984,7 @@ int mepgl_free_energy(mepgl_ctx* c) {
 
 int mepgl_gradient(mepgl_ctx* c, int si, double* a_g, double* p1_g, double* p2_g) {
   if (!valid_string(c, si) || !a_g || !p1_g) return fail(c, MEPGL_ERR_ARG, "gradient: bad arguments");
-  hipSetDevice(c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
   int rc = flush(c);
   if (rc) return rc;
   bool changed = false;
@@ -990,7 +1006,7 @@ int mepgl_gradient(mepgl_ctx* c, int si, double* a_g, double* p1_g, double* p2_g
 
 int mepgl_nlcg_steps(mepgl_ctx* c, int iter_n, int nlcg, int maxwell) {
   if (!c || iter_n < 0) return MEPGL_ERR_ARG;
-  hipSetDevice(c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
   for (int k = 0; k < iter_n; ++k) {
     int rc = nlcg_step(c, k, nlcg ? 1 : 0, maxwell ? 1 : 0);
     if (rc) return rc;
@@ -1000,7 +1016,7 @@ int mepgl_nlcg_steps(mepgl_ctx* c, int iter_n, int nlcg, int maxwell) {
 
 int mepgl_compute_jb(mepgl_ctx* c, int si, double* b, double* j1, double* j2) {
   if (!valid_string(c, si) || !b || !j1) return fail(c, MEPGL_ERR_ARG, "compute_jb: bad arguments");
-  hipSetDevice(c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
   int rc = flush(c);
   if (rc) return rc;
   bool changed = false;
@@ -1043,13 +1059,13 @@ int mepgl_compute_jb(mepgl_ctx* c, int si, double* b, double* j1, double* j2) {
 
 int mepgl_arc_length(mepgl_ctx* c) {
   if (!c) return MEPGL_ERR_ARG;
-  hipSetDevice(c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
   return arc_length_all(c);
 }
 
 int mepgl_redistribute(mepgl_ctx* c, int cubic) {
   if (!c) return MEPGL_ERR_ARG;
-  hipSetDevice(c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
   int rc;
   if (cubic && (rc = ensure_cubic_rhs(c))) return rc;
   return redistribute_all(c, cubic);
@@ -1057,13 +1073,13 @@ int mepgl_redistribute(mepgl_ctx* c, int cubic) {
 
 int mepgl_gauge_align(mepgl_ctx* c) {
   if (!c) return MEPGL_ERR_ARG;
-  hipSetDevice(c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
   return gauge_align_all(c);
 }
 
 int mepgl_reset_interior_beta(mepgl_ctx* c) {
   if (!c) return MEPGL_ERR_ARG;
-  hipSetDevice(c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
   AuxArgs a = aux_args(c);
   return run_aux(c, a, kKernelResetNorms, 1);
 }
@@ -1071,7 +1087,7 @@ int mepgl_reset_interior_beta(mepgl_ctx* c) {
 int mepgl_resample(mepgl_ctx* c, int si, int F_new) {
   if (!valid_string(c, si) || F_new < 2 || F_new > kMaxImages)
     return fail(c, MEPGL_ERR_ARG, "resample: bad arguments");
-  hipSetDevice(c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
   int rc = arc_length_all(c);
   if (rc) return rc;
   StringHost& s = c->strings[si];
@@ -1126,7 +1142,7 @@ int mepgl_resample(mepgl_ctx* c, int si, int F_new) {
     if (shared) {
       keep.push_back(a);
     } else {
-      hipFree(a.p);
+      (void)hipFree(a.p);
       c->bytes -= (long long)a.bytes;
     }
   }
@@ -1147,7 +1163,7 @@ int mepgl_loop(mepgl_ctx* c, int iterations, int relax, char mode, int nlcg) {
   if (!c || iterations < 0 || relax < 0) return MEPGL_ERR_ARG;
   if (mode != 'S' && mode != 'M' && mode != 'L' && mode != 'C')
     return fail(c, MEPGL_ERR_ARG, std::string("unknown mode ") + mode);
-  hipSetDevice(c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
   int rc;
   if (mode == 'C' && (rc = ensure_cubic_rhs(c))) return rc;
   const bool use_graph = !(c->flags & MEPGL_NO_GRAPH) && !c->timing.on;
@@ -1172,13 +1188,13 @@ int mepgl_loop(mepgl_ctx* c, int iterations, int relax, char mode, int nlcg) {
       c->dp = dp0;
       c->pending = pend0;
       if (rc) {
-        if (g) hipGraphDestroy(g);
+        if (g) (void)hipGraphDestroy(g);
         return rc;
       }
       if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, "graph capture failed");
       hipGraphExec_t ex;
       e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-      hipGraphDestroy(g);
+      (void)hipGraphDestroy(g);
       if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, "graph instantiate failed");
       itg = c->graphs.emplace(key, ex).first;
     }
@@ -1194,7 +1210,7 @@ int mepgl_loop(mepgl_ctx* c, int iterations, int relax, char mode, int nlcg) {
 
 int mepgl_get_scalars(mepgl_ctx* c, int si, int which, double* out) {
   if (!valid_string(c, si) || !out) return fail(c, MEPGL_ERR_ARG, "get_scalars: bad arguments");
-  hipSetDevice(c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
   const DevString& d = c->strings[si].dev;
   std::vector<ImageScalars> h(d.F);
   HIP_TRY(c, hipMemcpyAsync(h.data(), c->d_sc + d.img0, sizeof(ImageScalars) * d.F,
@@ -1233,7 +1249,7 @@ int mepgl_get_scalars(mepgl_ctx* c, int si, int which, double* out) {
 int mepgl_download_state(mepgl_ctx* c, int si, int which, double* a, double* p1, double* p2) {
   if (!valid_string(c, si) || !a || !p1 || which < 0 || which > 2)
     return fail(c, MEPGL_ERR_ARG, "download_state: bad arguments");
-  hipSetDevice(c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
   const DevString& d = c->strings[si].dev;
   const void* src = which == 0 ? d.g[0] : which == 1 ? d.g[1] : d.d[c->dp];
   return copy_out_fields(c, si, src, a, p1, p2);
@@ -1244,7 +1260,7 @@ int mepgl_upload_state(mepgl_ctx* c, int si, int which, const double* a, const d
   if (!valid_string(c, si) || !a || !p1 || which < 0 || which > 2)
     return fail(c, MEPGL_ERR_ARG, "upload_state: bad arguments");
   if (c->strings[si].dev.nc == 2 && !p2) return fail(c, MEPGL_ERR_ARG, "upload_state: psi_2 required");
-  hipSetDevice(c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
   int rc = flush(c);  // an owed update uses the direction being replaced
   if (rc) return rc;
   const DevString& d = c->strings[si].dev;
@@ -1257,7 +1273,7 @@ int mepgl_set_scalars(mepgl_ctx* c, int si, int which, const double* in) {
   if (which != MEPGL_SCALAR_G_NORM2 && which != MEPGL_SCALAR_G_NORM2_OLD &&
       which != MEPGL_SCALAR_S && which != MEPGL_SCALAR_S_JB)
     return fail(c, MEPGL_ERR_ARG, "set_scalars: only g_norm2, g_norm2_old, s and s_jb are settable");
-  hipSetDevice(c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
   const DevString& d = c->strings[si].dev;
   std::vector<ImageScalars> h(d.F);
   HIP_TRY(c, hipMemcpyAsync(h.data(), c->d_sc + d.img0, sizeof(ImageScalars) * d.F,
@@ -1301,7 +1317,7 @@ int mepgl_get_control(mepgl_ctx* c, int key, int* value) {
 
 int mepgl_profile(mepgl_ctx* c, int enable) {
   if (!c) return MEPGL_ERR_ARG;
-  hipSetDevice(c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   c->timing.on = enable != 0;
   c->timing.used = 0;
@@ -1315,7 +1331,7 @@ int mepgl_profile(mepgl_ctx* c, int enable) {
 
 int mepgl_profile_read(mepgl_ctx* c, int kernel, double* avg_ms, int* count) {
   if (!c || kernel < 0 || kernel >= kKernelCount || !avg_ms || !count) return MEPGL_ERR_ARG;
-  hipSetDevice(c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   Timing& t = c->timing;
   for (auto& m : t.marks) {
@@ -1333,14 +1349,14 @@ int mepgl_profile_read(mepgl_ctx* c, int kernel, double* avg_ms, int* count) {
 
 int mepgl_synchronize(mepgl_ctx* c) {
   if (!c) return MEPGL_ERR_ARG;
-  hipSetDevice(c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return MEPGL_OK;
 }
 
 int mepgl_set_segment_rows(mepgl_ctx* c, int si, int rows) {
   if (!valid_string(c, si) || rows < 1) return MEPGL_ERR_ARG;
-  hipSetDevice(c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
   int rc = flush(c);
   if (rc) return rc;
   StringHost& s = c->strings[si];
@@ -1350,7 +1366,7 @@ int mepgl_set_segment_rows(mepgl_ctx* c, int si, int rows) {
 
 int mepgl_autotune(mepgl_ctx* c, int steps, int* rows_out) {
   if (!c || steps < 1) return MEPGL_ERR_ARG;
-  hipSetDevice(c->device);
+  HIP_TRY(c, hipSetDevice(c->device));
   int rc = flush(c);
   if (rc) return rc;
   // Classes of strings that share a kernel specialisation and a grid size,
