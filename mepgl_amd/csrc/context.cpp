@@ -606,7 +606,9 @@ int redistribute_all(mepgl_ctx* c, int cubic) {
     for (auto& s : c->strings)
       if (s.dev.F < 4)
         return fail(c, MEPGL_ERR_ARG, "cubic reparametrisation needs at least 4 images");
-    if ((rc = run_aux(c, a, kKernelCubic, (c->flags & MEPGL_CUBIC_TEXTBOOK) ? 1 : 0))) return rc;
+    if ((rc = run_aux(c, a, kKernelCubicPlan, (int)c->strings.size())) ||
+        (rc = run_aux(c, a, kKernelCubic, (c->flags & MEPGL_CUBIC_TEXTBOOK) ? 1 : 0)))
+      return rc;
   } else {
     if ((rc = run_aux(c, a, kKernelLinear, 0))) return rc;
   }

@@ -1215,18 +1215,15 @@ __global__ __launch_bounds__(kBlock) void k_arclen(AuxArgs a) {
 }
 
 // computeArcLength (interpolation.cu:126-140) for both metrics, then the
-// redistribution plan of every new image on s_jb: the segment search of
-// generateNewFramesLinear (interpolation.cu:200-219) and of
-// generateNewFramesCubic (interpolation.cu:373-390), and the Thomas factors
-// of the spline system built by buildHMatrix (interpolation.cu:252-290).
-// One workgroup per string.  The sequential recurrences (arclength prefix,
-// the spline's running sums, the Thomas factors) run in one thread over LDS
-// copies in the reference's order; the segment searches run one new image
-// per thread: for a non-decreasing s the reference's incremental search
-// stops at the first node k >= 1 with s[k] >= s_new, which is what each
-// thread finds directly.
+// linear redistribution plan of every new image on s_jb: the segment search
+// of generateNewFramesLinear (interpolation.cu:200-219).  One workgroup per
+// string.  The square roots run one image per thread; the prefix sums run in
+// one thread in the reference's order (so s and s_jb are the reference's
+// sums bit for bit); the segment searches run one new image per thread: for
+// a non-decreasing s the reference's incremental search stops at the first
+// node k >= 1 with s[k] >= s_new, which is what each thread finds directly.
 __global__ __launch_bounds__(kArcBlock) void k_arc_finalize(AuxArgs a, int nstrings) {
-  __shared__ double ls[kMaxImages], lsj[kMaxImages], cum[kMaxImages];
+  __shared__ double ls[kMaxImages], lsj[kMaxImages];
   const int si = blockIdx.x;
   if (si >= nstrings) return;
   const DevString& s = a.strings[si];
@@ -1235,16 +1232,18 @@ __global__ __launch_bounds__(kArcBlock) void k_arc_finalize(AuxArgs a, int nstri
   const int tid = threadIdx.x;
   if (F < 2 || F > kMaxImages) return;
   for (int n = tid; n < F; n += kArcBlock) {
-    ls[n] = sc[n].ds2;
-    lsj[n] = sc[n].ds2_jb;
+    ls[n] = n == 0 ? 0.0 : sqrt(sc[n].ds2);
+    lsj[n] = n == 0 ? 0.0 : sqrt(sc[n].ds2_jb);
   }
   __syncthreads();
   if (tid == 0) {
-    ls[0] = 0;
-    lsj[0] = 0;
+    double run = 0, runj = 0;
+#pragma unroll 8
     for (int n = 1; n < F; ++n) {
-      ls[n] = ls[n - 1] + sqrt(ls[n]);
-      lsj[n] = lsj[n - 1] + sqrt(lsj[n]);
+      run += ls[n];
+      runj += lsj[n];
+      ls[n] = run;
+      lsj[n] = runj;
     }
   }
   __syncthreads();
@@ -1266,11 +1265,27 @@ __global__ __launch_bounds__(kArcBlock) void k_arc_finalize(AuxArgs a, int nstri
     s.plan_k[n] = k;
     s.plan_w[n] = (s_new - seg_start) / (lsj[k] - lsj[k - 1]);
   }
-  if (F < 4) return;
-  // Cubic: h, the running sums of h, the Thomas factors, then the plan.
+}
+
+// Cubic redistribution plan on the s_jb of the last arclength pass: the
+// running sums of the spacings h, the Thomas factors of the spline system
+// built by buildHMatrix (interpolation.cu:252-290) and the segment search of
+// generateNewFramesCubic (interpolation.cu:373-390).  One workgroup per
+// string, launched only before a cubic redistribution; the recurrences run
+// in one thread in the reference's order.
+__global__ __launch_bounds__(kArcBlock) void k_cubic_plan(AuxArgs a, int nstrings) {
+  __shared__ double lsj[kMaxImages], cum[kMaxImages];
+  const int si = blockIdx.x;
+  if (si >= nstrings) return;
+  const DevString& s = a.strings[si];
+  const ImageScalars* sc = a.sc + s.img0;
+  const int F = s.F;
+  const int tid = threadIdx.x;
+  if (F < 4 || F > kMaxImages) return;
   double* cb = s.cubic;  // [F][4]: c', m, sub-diagonal, h
-  for (int n = tid; n < F; n += kArcBlock) cb[4 * n + 3] = n == 0 ? 0.0 : lsj[n] - lsj[n - 1];
+  for (int n = tid; n < F; n += kArcBlock) lsj[n] = sc[n].s_jb;
   __syncthreads();
+  for (int n = tid; n < F; n += kArcBlock) cb[4 * n + 3] = n == 0 ? 0.0 : lsj[n] - lsj[n - 1];
   if (tid == 0) {
     auto h = [&](int n) { return n == 0 ? 0.0 : lsj[n] - lsj[n - 1]; };
     cum[0] = 0;
@@ -1624,6 +1639,9 @@ static hipError_t launch_aux_impl(const AuxArgs& a, int which, int arg, hipStrea
       break;
     case kKernelArcFinalize:
       hipLaunchKernelGGL(k_arc_finalize, dim3(arg), dim3(kArcBlock), 0, st, a, arg);
+      break;
+    case kKernelCubicPlan:
+      hipLaunchKernelGGL(k_cubic_plan, dim3(arg), dim3(kArcBlock), 0, st, a, arg);
       break;
     case kKernelLinear:
       if (a.nitems > 0) hipLaunchKernelGGL(k_redistribute_linear<T>, g, b, 0, st, a);

@@ -23,6 +23,7 @@ enum KernelId {
   kKernelAlignFinalize,
   kKernelAlignApply,
   kKernelWinding,      // boundary phase winding per image (finishing-style launch)
+  kKernelCubicPlan,    // spline factors and plan of the cubic redistribution
   kKernelCount
 };
 
