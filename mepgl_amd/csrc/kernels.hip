@@ -133,11 +133,34 @@ __device__ __forceinline__ void reduce_partials(const double* part, int n, int n
 #pragma unroll
   for (int k = 0; k < K; ++k) ss[k] = cc[k] = 0.0;
   const double* base = part + (long long)n * ntiles * (2 * K);
-  for (int t = threadIdx.x; t < ntiles; t += kWave) {
+  // Each lane takes tiles lane, lane + 64, ... in that order; the loads of
+  // four of its tiles are issued before any is accumulated, so a lane waits
+  // for memory once per four tiles instead of once per tile.
+  constexpr int U = 4;
+  for (int t0 = threadIdx.x; t0 < ntiles; t0 += U * kWave) {
+    double v[U][K], w[U][K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      acc<COMP>(ss[k], cc[k], base[(long long)t * 2 * K + k]);
-      if (COMP) cc[k] += base[(long long)t * 2 * K + K + k];
+    for (int u = 0; u < U; ++u) {
+      const int t = t0 + u * kWave;
+#pragma unroll
+      for (int k = 0; k < K; ++k) v[u][k] = w[u][k] = 0.0;
+      if (t < ntiles) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          v[u][k] = base[(long long)t * 2 * K + k];
+          if (COMP) w[u][k] = base[(long long)t * 2 * K + K + k];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (t0 + u * kWave < ntiles) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          acc<COMP>(ss[k], cc[k], v[u][k]);
+          if (COMP) cc[k] += w[u][k];
+        }
+      }
     }
   }
   wave_reduce<K, COMP>(ss, cc);
