@@ -24,7 +24,10 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wall", "-Wno-unused-result", "-I", os.path.join(ROOT, "include")]
 
-SOURCES = ["kernels.hip", "context.cpp"]
+# The step kernels are split by storage type and component count so that they
+# compile concurrently.
+SOURCES = ["step_f64_2.hip", "step_f64_1.hip", "step_f32_2.hip", "step_f32_1.hip", "kernels.hip",
+           "context.cpp"]
 CLI_SOURCES = ["cli.cpp", "npyio.cpp"]
 
 

@@ -73,6 +73,10 @@ struct FinishArgs {
 };
 
 hipError_t launch_step(const StepArgs& a, int which, bool fp32, hipStream_t st);
+// The step kernels of one storage type and component count, instantiated in
+// step_{f32,f64}_{1,2}.hip.
+template <class T, int NC>
+hipError_t launch_step_nc(const StepArgs& a, int which, hipStream_t st);
 hipError_t launch_finish(const FinishArgs& f, int which, bool fp32, hipStream_t st);
 hipError_t launch_aux(const AuxArgs& a, int which, int arg, bool fp32, hipStream_t st);
 hipError_t launch_nn_map(const int* sc_dom, const int* comp_dom, int N, int pitch, uint16_t* nn,

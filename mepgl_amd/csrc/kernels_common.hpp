@@ -1,0 +1,324 @@
+// Device helpers shared by the HIP kernels of the relaxation loop: work-item
+// mapping, deterministic reductions, lane shifts, strip geometry and
+// buffer-resource access.
+#pragma once
+
+#include <float.h>
+
+#include <type_traits>
+
+#include "common.hpp"
+#include "kernels.hpp"
+#include "stencil.hpp"
+
+namespace mepgl {
+
+
+// ---------------------------------------------------------------------------
+// Work-item mapping: consecutive items go to the same XCD so that the halo
+// rows and columns a strip shares with its neighbours are served by that
+// XCD's L2.
+__device__ __forceinline__ int xcd_item(int b, int nb) {
+  const int q = nb / kNumXcd, r = nb % kNumXcd;
+  const int k = b % kNumXcd, l = b / kNumXcd;
+  return k * q + (k < r ? k : r) + l;
+}
+
+// The work item of this workgroup, made wave-uniform so that the descriptor
+// behind it is read with scalar loads.
+__device__ __forceinline__ WorkItem load_item(const WorkItem* items, int k) {
+  const WorkItem w = items[k];
+  WorkItem u;
+  u.string = __builtin_amdgcn_readfirstlane(w.string);
+  u.image = __builtin_amdgcn_readfirstlane(w.image);
+  u.tile = __builtin_amdgcn_readfirstlane(w.tile);
+  u.pad = 0;
+  return u;
+}
+
+// Pointers read from descriptors are generic; the hot kernels cast them to
+// the global address space so that loads and stores compile to global_*
+// instructions instead of flat_*.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+
+template <class T>
+__device__ __forceinline__ gptr<T> as_global(T* p) {
+  return (gptr<T>)p;
+}
+
+// Two-sum accumulation (Knuth): exact rounding error kept in c.
+__device__ __forceinline__ void two_sum_acc(double& s, double& c, double v) {
+  const double t = s + v;
+  const double bp = t - s;
+  const double e = (s - (t - bp)) + (v - bp);
+  s = t;
+  c += e;
+}
+
+template <bool COMP>
+__device__ __forceinline__ void acc(double& s, double& c, double v) {
+  if (COMP)
+    two_sum_acc(s, c, v);
+  else
+    s += v;
+}
+
+// Wavefront reduction of K (sum, compensation) pairs; every lane ends with
+// the same result (butterfly, operands combined in lane order).
+template <int K, bool COMP>
+__device__ __forceinline__ void wave_reduce(double (&s)[K], double (&c)[K]) {
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    const bool lo = (threadIdx.x & off) == 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const double so = __shfl_xor(s[k], off);
+      if (COMP) {
+        const double co = __shfl_xor(c[k], off);
+        double x = lo ? s[k] : so, y = lo ? so : s[k];
+        double cx = lo ? c[k] : co, cy = lo ? co : c[k];
+        two_sum_acc(x, cx, y);
+        s[k] = x;
+        c[k] = cx + cy;
+      } else {
+        s[k] = lo ? s[k] + so : so + s[k];
+      }
+    }
+  }
+}
+
+// Publishes this wavefront's K (sum, compensation) pairs as partial `tile`
+// of image n.  Partials are combined by a separate finishing launch: a
+// cross-wavefront handshake inside the kernel would need an agent-scope
+// release per wavefront, which on a multi-XCD device writes back the L2.
+template <int K, bool COMP>
+__device__ __forceinline__ void publish_partial(double (&s)[K], double (&c)[K], double* part,
+                                                int n, int tile, int ntiles) {
+  wave_reduce<K, COMP>(s, c);
+  if (threadIdx.x == 0) {
+    double* mine = part + ((long long)n * ntiles + tile) * (2 * K);
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      mine[k] = s[k];
+      mine[K + k] = c[k];
+    }
+  }
+}
+
+// Fixed-order total of the ntiles partials of image n (every lane).
+template <int K, bool COMP>
+__device__ __forceinline__ void reduce_partials(const double* part, int n, int ntiles,
+                                                double (&fin)[K]) {
+  double ss[K], cc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) ss[k] = cc[k] = 0.0;
+  const double* base = part + (long long)n * ntiles * (2 * K);
+  // Each lane takes tiles lane, lane + 64, ... in that order; the loads of
+  // four of its tiles are issued before any is accumulated, so a lane waits
+  // for memory once per four tiles instead of once per tile.
+  constexpr int U = 4;
+  for (int t0 = threadIdx.x; t0 < ntiles; t0 += U * kWave) {
+    double v[U][K], w[U][K];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int t = t0 + u * kWave;
+#pragma unroll
+      for (int k = 0; k < K; ++k) v[u][k] = w[u][k] = 0.0;
+      if (t < ntiles) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          v[u][k] = base[(long long)t * 2 * K + k];
+          if (COMP) w[u][k] = base[(long long)t * 2 * K + K + k];
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (t0 + u * kWave < ntiles) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          acc<COMP>(ss[k], cc[k], v[u][k]);
+          if (COMP) cc[k] += w[u][k];
+        }
+      }
+    }
+  }
+  wave_reduce<K, COMP>(ss, cc);
+#pragma unroll
+  for (int k = 0; k < K; ++k) fin[k] = ss[k] + cc[k];
+}
+
+// ---------------------------------------------------------------------------
+// Lane shifts across the wavefront (DPP wave_shr:1 / wave_shl:1).  Lanes
+// without a source keep `edge`, which holds the column outside the strip.
+__device__ __forceinline__ float from_left(float v, float edge) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x138, 0xF, 0xF, false));
+}
+__device__ __forceinline__ float from_right(float v, float edge) {
+  return __int_as_float(
+      __builtin_amdgcn_update_dpp(__float_as_int(edge), __float_as_int(v), 0x130, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double from_left(double v, double edge) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(edge), __double2loint(v), 0x138, 0xF,
+                                             0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(edge), __double2hiint(v), 0x138, 0xF,
+                                             0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double from_right(double v, double edge) {
+  const int lo = __builtin_amdgcn_update_dpp(__double2loint(edge), __double2loint(v), 0x130, 0xF,
+                                             0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(__double2hiint(edge), __double2hiint(v), 0x130, 0xF,
+                                             0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// Strip geometry of a work item.
+struct Strip {
+  int n, tile;
+  int lane, i, ic;   // lane, column, column clamped into the grid
+  int hc;            // halo column of lanes 0 and 63 (own column elsewhere)
+  bool edge;         // lane 0 or 63
+  int j0, j1;        // rows [j0, j1) of the segment
+};
+
+__device__ __forceinline__ Strip strip_of(const DevString& s, const WorkItem& it) {
+  Strip t;
+  t.n = it.image;
+  t.tile = it.tile;
+  const int sx = it.tile % s.tiles_x, sy = it.tile / s.tiles_x;
+  t.lane = threadIdx.x;
+  const int x0 = sx * kTileX;
+  t.i = x0 + t.lane;
+  t.ic = min(t.i, s.N - 1);
+  t.edge = t.lane == 0 || t.lane == kWave - 1;
+  t.hc = t.lane == 0 ? max(x0 - 1, 0) : (t.lane == kWave - 1 ? min(x0 + kWave, s.N - 1) : t.ic);
+  t.j0 = sy * s.seg_rows;
+  t.j1 = min(t.j0 + s.seg_rows, s.N);
+  return t;
+}
+
+// Segments alternate their march direction: even segments march down, odd
+// ones up.  Two segments that share a boundary then read the rows around it
+// at about the same time (both at their start or both at their end), and
+// vertically adjacent segments run concurrently on one XCD (xcd_item), so
+// the second read of each shared row is served by that XCD's L2.
+__device__ __forceinline__ bool marches_up(const DevString& s, const WorkItem& it) {
+  return __builtin_amdgcn_readfirstlane((it.tile / s.tiles_x) & 1) != 0;
+}
+
+__device__ __forceinline__ int clampr(int j, int N) { return j < 0 ? 0 : (j >= N ? N - 1 : j); }
+
+__device__ __forceinline__ double hval(const DevString& s, long long off) {
+  return s.h.p ? s.h.p[off] : s.h.v;
+}
+
+// ---------------------------------------------------------------------------
+// Buffer access for the G and P kernels.  Each field set of one image is one
+// buffer resource (base in scalar registers); an element is addressed by a
+// 32-bit byte offset of (row, column) in a VGPR plus the plane's byte
+// offset in an SGPR, so no 64-bit address arithmetic runs per load.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+constexpr int kRsrcWord3 = 0x00020000;  // raw untyped buffer (CDNA)
+
+__device__ __forceinline__ Rsrc rsrc_of(const void* base, long long bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0,
+                                           (int)(bytes > 0x7fffffffLL ? 0x7fffffff : bytes),
+                                           kRsrcWord3);
+}
+
+__device__ __forceinline__ double bld(double, Rsrc r, unsigned vo, unsigned so) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, vo, so, 0));
+}
+__device__ __forceinline__ float bld(float, Rsrc r, unsigned vo, unsigned so) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, vo, so, 0));
+}
+template <class T>
+__device__ __forceinline__ T bload(Rsrc r, unsigned vo, unsigned so) {
+  return bld(T(), r, vo, so);
+}
+// Stores are non-temporal (nt): the next reader is another launch, after
+// far more data than the caches hold has streamed through.
+constexpr int kStoreNT = 2;
+__device__ __forceinline__ void bstore(double v, Rsrc r, unsigned vo, unsigned so) {
+  typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, vo, so, kStoreNT);
+}
+__device__ __forceinline__ void bstore(float v, Rsrc r, unsigned vo, unsigned so) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vo, so, kStoreNT);
+}
+__device__ __forceinline__ uint16_t bload_u16(Rsrc r, unsigned vo) {
+  return __builtin_amdgcn_raw_buffer_load_b16(r, vo, 0, 0);
+}
+
+// Per-site parameters of one row when some parameter is a plane: a, b,
+// m_xx, m_yy per component and the applied field at the five stencil
+// points (centre, +x, -x, +y, -y).
+template <class T, int NC, bool PL>
+struct RowParams {
+  __device__ __forceinline__ void load(const ParamView<NC>&, gptr<const double>, long long, int,
+                                       int, int, const Strip&) {}
+};
+template <class T, int NC>
+struct RowParams<T, NC, true> {
+  double prm[NC][4];
+  T h[5];
+  __device__ __forceinline__ void load(const ParamView<NC>& pv, gptr<const double> hp, long long off,
+                                       int j, int N, int P_, const Strip& t) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+      prm[c][0] = param_at<T>(pv.a[c], off);
+      prm[c][1] = param_at<T>(pv.b[c], off);
+      prm[c][2] = param_at<T>(pv.mxx[c], off);
+      prm[c][3] = param_at<T>(pv.myy[c], off);
+    }
+    if (hp) {
+      const int ip = min(t.ic + 1, N - 1), im = max(t.ic - 1, 0);
+      h[0] = (T)hp[off];
+      h[1] = (T)hp[(long long)j * P_ + ip];
+      h[2] = (T)hp[(long long)j * P_ + im];
+      h[3] = (T)hp[(long long)clampr(j + 1, N) * P_ + t.ic];
+      h[4] = (T)hp[(long long)clampr(j - 1, N) * P_ + t.ic];
+    }
+  }
+};
+
+template <class T, int NC>
+__device__ __forceinline__ void comp_from_params(const double (&prm)[NC][4], const double (&q)[NC],
+                                                 CompConst<T> (&k)[NC]) {
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    const double a = prm[c][0], b = prm[c][1], mx = prm[c][2], my = prm[c][3];
+    k[c].q = (T)q[c];
+    k[c].two_a = (T)(2.0 * a);
+    k[c].two_b = (T)(2.0 * b);
+    k[c].b = (T)b;
+    k[c].ab = (T)(a / b);
+    k[c].hx = (T)(0.5 / mx);
+    k[c].hy = (T)(0.5 / my);
+  }
+}
+
+// Byte offsets of a strip row: own column and halo column.
+struct RowOff {
+  unsigned own, halo;
+};
+template <class T>
+__device__ __forceinline__ RowOff row_off(int row, int P_, const Strip& t) {
+  return {(unsigned)((row * P_ + t.ic) * (int)sizeof(T)), (unsigned)((row * P_ + t.hc) * (int)sizeof(T))};
+}
+
+// Makes a value live in a register at this point of the program: the
+// compiler may not defer its computation (and the wait for the loads it
+// reads) past the point.
+template <class T>
+__device__ __forceinline__ void pin(T& v) {
+  asm volatile("" : "+v"(v));
+}
+// A compiler barrier for memory operations: later stores are not moved above
+// it (pinned computations before it stay before the stores).
+__device__ __forceinline__ void memory_fence_compiler() { asm volatile("" ::: "memory"); }
+
+}  // namespace mepgl

@@ -1,0 +1,710 @@
+// The G and P kernels of one NLCG step of the MI355X gauged-string-method
+// relaxation loop (gfx950) and the in-place update, with their launcher;
+// instantiated once per storage type and component count (step_f32_1.hip,
+// step_f32_2.hip, step_f64_1.hip, step_f64_2.hip), so that the four parts of
+// the kernel code build concurrently.
+//
+// Work decomposition: one wavefront per work item = one 64-column strip of
+// one row segment of one image of one string.  Lane l owns column
+// 64*strip + l; the wavefront marches along the rows of its segment (down in
+// even segments, up in odd ones), keeping the rows j-1, j, j+1 it needs in a
+// register window and prefetching the next-but-one row while it computes
+// row j.  Neighbours along x come from the adjacent lanes through DPP wave
+// shifts; the two columns just outside the strip are loaded by lanes 0 and
+// 63 only.  Every field value is therefore read from HBM once per launch,
+// plus the halo rows and columns the strip shares with its neighbours,
+// which L2 serves: adjacent work items run at the same time on the same
+// XCD, and neighbouring segments reach their shared rows together.
+//
+// One NLCG step (nlcgSteps, nlcg.cu:1478-1583) is two launches:
+//   G  fused pending update x += alpha d (updateFields), gradient
+//      (gradientPsi + gradientA), |g|^2 and g.(g - g_old) partials
+//      (gradientDensity).
+//   P  direction d = -g + beta d (computeStepDirection), evaluated on the
+//      stencil, line-search quartic partials (polynomialExpansion).
+// Each is followed by a finishing launch with one workgroup per image that
+// reduces the partials in a fixed order: beta after G; A..E and the line
+// search (lineSearch, a parallel scan over a host-built table) after P.
+// Reductions are deterministic (fixed partial order, fixed trees), with
+// two-sum compensation in the mixed-precision path.
+#pragma once
+
+#include "kernels_common.hpp"
+
+namespace mepgl {
+
+// ---------------------------------------------------------------------------
+// G kernel.
+template <class T, int NC>
+struct GRaw {
+  static constexpr int P = 2 + 2 * NC;
+  T x[P], d[P], hx[P], hd[P];
+  uint16_t nn;
+};
+
+template <class T, int NC>
+struct GRow {
+  T ax, ay, axl, axr, ayl, ayr;  // A and its x-neighbours
+  T u[NC], v[NC];                // psi
+  T eu[NC], ev[NC];              // psi outside the strip (lanes 0 / 63)
+  uint16_t nn;
+};
+
+// Buffers of one G work item.
+struct GBufs {
+  Rsrc x, xo, d, g, go, nn;
+  unsigned ps;  // plane stride in bytes
+};
+
+template <class T, int NC, bool UPD, bool NSF>
+__device__ __forceinline__ void g_load(GRaw<T, NC>& r, const GBufs& b, int row, int P_,
+                                       const Strip& t, unsigned nnoff) {
+  const RowOff o = row_off<T>(row, P_, t);
+#pragma unroll
+  for (int p = 0; p < GRaw<T, NC>::P; ++p) {
+    r.x[p] = bload<T>(b.x, o.own, p * b.ps);
+    r.d[p] = (UPD && !(NSF && p < 2)) ? bload<T>(b.d, o.own, p * b.ps) : (T)0;
+  }
+  // Halo columns: only lanes 0 and 63 read the columns outside the strip
+  // (exec-masked loads); the other lanes' halo values are never used.
+#pragma unroll
+  for (int p = 0; p < GRaw<T, NC>::P; ++p) r.hx[p] = r.hd[p] = (T)0;
+  if (t.edge) {
+#pragma unroll
+    for (int p = 0; p < GRaw<T, NC>::P; ++p) {
+      r.hx[p] = bload<T>(b.x, o.halo, p * b.ps);
+      r.hd[p] = (UPD && !(NSF && p < 2)) ? bload<T>(b.d, o.halo, p * b.ps) : (T)0;
+    }
+  }
+  r.nn = bload_u16(b.nn, nnoff);
+}
+
+template <class T, int NC>
+__device__ __forceinline__ void pin_row(GRow<T, NC>& w) {
+  pin(w.ax), pin(w.ay), pin(w.axl), pin(w.axr), pin(w.ayl), pin(w.ayr);
+#pragma unroll
+  for (int q = 0; q < NC; ++q) pin(w.u[q]), pin(w.v[q]), pin(w.eu[q]), pin(w.ev[q]);
+  pin(w.nn);
+}
+
+template <class T, int NC>
+__device__ __forceinline__ void g_process(GRow<T, NC>& w, const GRaw<T, NC>& r, T al, bool upd) {
+  T c[GRaw<T, NC>::P], e[GRaw<T, NC>::P];
+#pragma unroll
+  for (int p = 0; p < GRaw<T, NC>::P; ++p) {
+    c[p] = upd ? fma(al, r.d[p], r.x[p]) : r.x[p];
+    e[p] = upd ? fma(al, r.hd[p], r.hx[p]) : r.hx[p];
+  }
+  w.ax = c[PAX];
+  w.ay = c[PAY];
+  w.axl = from_left(c[PAX], e[PAX]);
+  w.axr = from_right(c[PAX], e[PAX]);
+  w.ayl = from_left(c[PAY], e[PAY]);
+  w.ayr = from_right(c[PAY], e[PAY]);
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    w.u[q] = c[PU1 + 2 * q];
+    w.v[q] = c[PV1 + 2 * q];
+    w.eu[q] = e[PU1 + 2 * q];
+    w.ev[q] = e[PV1 + 2 * q];
+  }
+  w.nn = r.nn;
+}
+
+// Own-site values of a row that only the row's own computation needs: the
+// old gradient (Polak-Ribiere), the frozen psi gradient (Maxwell mode) and,
+// with parameter planes, the row's parameters.
+template <class T, int NC, bool PL>
+struct GSide {
+  static constexpr int P = 2 + 2 * NC;
+  T go[P];
+  T gst[NC][2];
+  RowParams<T, NC, PL> rp;
+};
+
+// The old gradient is read unconditionally (the diagnostic entry point that
+// does not need it still reads a valid buffer), so the batch has no
+// data-dependent branches.
+template <class T, int NC, bool NSF, bool PL, bool MW>
+__device__ __forceinline__ void g_side_load(GSide<T, NC, PL>& sd, const GBufs& b, int row, int P_,
+                                            const Strip& t, const ParamView<NC>& pv,
+                                            gptr<const double> hp, int N) {
+  const unsigned o = (unsigned)((row * P_ + t.ic) * (int)sizeof(T));
+#pragma unroll
+  for (int p = 0; p < GSide<T, NC, PL>::P; ++p)
+    sd.go[p] = !(NSF && p < 2) ? bload<T>(b.go, o, p * b.ps) : (T)0;
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    sd.gst[q][0] = MW ? bload<T>(b.g, o, (PU1 + 2 * q) * b.ps) : (T)0;
+    sd.gst[q][1] = MW ? bload<T>(b.g, o, (PV1 + 2 * q) * b.ps) : (T)0;
+  }
+  sd.rp.load(pv, hp, (long long)row * P_ + t.ic, row, N, P_, t);
+}
+
+template <class T, int NC, bool PL>
+__device__ __forceinline__ void pin_side(GSide<T, NC, PL>& sd) {
+#pragma unroll
+  for (int p = 0; p < GSide<T, NC, PL>::P; ++p) pin(sd.go[p]);
+#pragma unroll
+  for (int q = 0; q < NC; ++q) pin(sd.gst[q][0]), pin(sd.gst[q][1]);
+  if constexpr (PL) {
+#pragma unroll
+    for (int q = 0; q < NC; ++q)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) pin(sd.rp.prm[q][k]);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) pin(sd.rp.h[k]);
+  }
+}
+
+template <class T, int NC, bool UPD, bool NSF, bool PL, bool MW>
+__device__ __forceinline__ void grad_body(const StepArgs& a, const DevString& s, const WorkItem& it) {
+  constexpr bool COMP = sizeof(T) == 4;
+  const Strip t = strip_of(s, it);
+  const int N = s.N, P_ = s.pitch;
+  const long long ibytes = s.image_stride * (long long)sizeof(T);
+  const long long ib = (long long)t.n * s.image_stride;
+  GBufs b;
+  b.x = rsrc_of((const T*)s.x[a.xin] + ib, ibytes);
+  b.xo = rsrc_of((const T*)s.x[a.xout] + ib, ibytes);
+  b.d = rsrc_of((const T*)s.d[a.din] + ib, ibytes);
+  b.g = rsrc_of((const T*)(a.scratch ? s.gs : s.g[a.gcur]) + ib, ibytes);
+  b.go = rsrc_of((const T*)s.g[a.gold] + ib, ibytes);
+  b.nn = rsrc_of(s.nn, (long long)s.N * s.pitch * (long long)sizeof(uint16_t));
+  b.ps = (unsigned)(s.plane_stride * (long long)sizeof(T));
+  const ImageScalars& sc = a.sc[s.img0 + t.n];
+  const T al = UPD ? (T)sc.alpha : (T)0;
+  constexpr bool do_a = !NSF;
+  const bool want_go = !a.scratch;
+  const bool store = t.i < N;
+  const T idx = (T)s.inv_dx, dx2 = (T)s.dx2;
+  const T eta = (T)s.eta, gam = (T)s.gamma, del = (T)s.delta;
+  CompConst<T> kconst[NC];
+  if (!PL) load_comp_const<T, NC>(s, 0, kconst);
+  const ParamView<NC> pv = param_view<NC>(s);
+  gptr<const double> __restrict__ hp = as_global(s.h.p);
+  const T hv = (T)s.h.v;
+  double acc_s[2] = {0, 0}, acc_c[2] = {0, 0};
+  auto nnoff = [&](int row) { return (unsigned)((row * P_ + t.ic) * (int)sizeof(uint16_t)); };
+
+  // The march is the only part that depends on the direction
+  // (marches_up); everything it needs from the string's descriptor is
+  // read above, before the branch and before any store, so it stays in
+  // scalar registers.
+  const bool up = marches_up(s, it);
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, 1>;
+  using S2 = std::integral_constant<int, 2>;
+  auto march = [&](auto up_tag) {
+    constexpr bool UP = decltype(up_tag)::value;
+    constexpr int D = UP ? -1 : 1;  // march direction in rows
+    // The window is three row slots used in rotation: (previous, current,
+    // next) = (w[IP], w[IC], w[IN]) in the order of the march, with the side
+    // values of each row in the slot of the same index.  Each row writes the
+    // newly loaded row into the previous row's slot once the current row is
+    // computed, and the next row uses the slots shifted by one, so no register
+    // copies advance the window.  A segment marches down (rows j0 .. j1-1) or,
+    // when UP, up (rows j1-1 .. j0); the previous row of the march is then
+    // row j + 1.
+    const int jb = UP ? t.j1 - 1 : t.j0;  // first row of the march
+    GRaw<T, NC> raw;
+    GRow<T, NC> w[3];
+    GSide<T, NC, PL> sd[3];
+    g_load<T, NC, UPD, NSF>(raw, b, clampr(jb - D, N), P_, t, nnoff(clampr(jb - D, N)));
+    g_process<T, NC>(w[0], raw, al, UPD);
+    g_load<T, NC, UPD, NSF>(raw, b, clampr(jb, N), P_, t, nnoff(clampr(jb, N)));
+    g_process<T, NC>(w[1], raw, al, UPD);
+    g_load<T, NC, UPD, NSF>(raw, b, clampr(jb + D, N), P_, t, nnoff(clampr(jb + D, N)));
+    g_process<T, NC>(w[2], raw, al, UPD);
+    g_side_load<T, NC, NSF, PL, MW>(sd[1], b, jb, P_, t, pv, hp, N);
+
+    // Row j: issue one batch of loads (the fields of row j + 2D, the own-site
+    // values of row j + D), compute row j from registers, turn the batch into
+    // window form, then store row j.  Every load is consumed before this
+    // row's stores are issued, and the last row (no batch) is a separate copy
+    // of the body, so the loads in flight at each point are known exactly.
+    // Rows past the end of the segment (padding of the last rotation) are
+    // computed but neither stored nor summed.
+    auto row = [&](int j, auto ip_tag, auto ic_tag, auto in_tag, auto batch_tag) {
+      constexpr int IP = decltype(ip_tag)::value, IC = decltype(ic_tag)::value;
+      constexpr int IN = decltype(in_tag)::value;
+      constexpr bool BATCH = decltype(batch_tag)::value;
+      GRow<T, NC>& wm = w[UP ? IN : IP];  // row j - 1
+      GRow<T, NC>& wc = w[IC];
+      GRow<T, NC>& wpl = w[UP ? IP : IN];  // row j + 1
+      const bool live = store && (UP ? j >= t.j0 : j < t.j1);
+      if constexpr (BATCH) {
+        g_load<T, NC, UPD, NSF>(raw, b, clampr(j + 2 * D, N), P_, t,
+                                nnoff(clampr(j + 2 * D, N)));
+        g_side_load<T, NC, NSF, PL, MW>(sd[IN], b, clampr(j + D, N), P_, t, pv, hp, N);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+
+      Nbhd<T, NC> nb;
+      nb.ax[0][0] = wm.axl; nb.ax[0][1] = wm.ax; nb.ax[0][2] = wm.axr;
+      nb.ax[1][0] = wc.axl; nb.ax[1][1] = wc.ax; nb.ax[1][2] = wc.axr;
+      nb.ax[2][0] = wpl.axl; nb.ax[2][1] = wpl.ax; nb.ax[2][2] = wpl.axr;
+      nb.ay[0][0] = wm.ayl; nb.ay[0][1] = wm.ay; nb.ay[0][2] = wm.ayr;
+      nb.ay[1][0] = wc.ayl; nb.ay[1][1] = wc.ay; nb.ay[1][2] = wc.ayr;
+      nb.ay[2][0] = wpl.ayl; nb.ay[2][1] = wpl.ay; nb.ay[2][2] = wpl.ayr;
+#pragma unroll
+      for (int q = 0; q < NC; ++q) {
+        nb.u[q][0] = wc.u[q];
+        nb.v[q][0] = wc.v[q];
+        nb.u[q][1] = from_right(wc.u[q], wc.eu[q]);
+        nb.v[q][1] = from_right(wc.v[q], wc.ev[q]);
+        nb.u[q][2] = from_left(wc.u[q], wc.eu[q]);
+        nb.v[q][2] = from_left(wc.v[q], wc.ev[q]);
+        nb.u[q][3] = wpl.u[q];
+        nb.v[q][3] = wpl.v[q];
+        nb.u[q][4] = wm.u[q];
+        nb.v[q][4] = wm.v[q];
+      }
+      const GSide<T, NC, PL>& cur = sd[IC];
+      CompConst<T> k[NC];
+      if constexpr (PL) {
+        comp_from_params<T, NC>(cur.rp.prm, pv.q, k);
+#pragma unroll
+        for (int q = 0; q < 5; ++q) nb.h[q] = hp ? cur.rp.h[q] : hv;
+      } else {
+#pragma unroll
+        for (int q = 0; q < NC; ++q) k[q] = kconst[q];
+#pragma unroll
+        for (int q = 0; q < 5; ++q) nb.h[q] = hv;
+      }
+
+      T gp[NC][2], ga[2] = {0, 0};
+      site_gradient<T, NC>(nb, k, eta, gam, del, idx, dx2, wc.nn & 0xff, wc.nn >> 8, !MW, do_a,
+                           gp, ga);
+      if constexpr (MW) {
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+          gp[q][0] = cur.gst[q][0];
+          gp[q][1] = cur.gst[q][1];
+        }
+      }
+      if (live && want_go) {
+        // gradientDensity, nlcg.cu:919-949
+        double n2 = 0, dg = 0;
+        if (do_a) {
+#pragma unroll
+          for (int q = 0; q < 2; ++q) {
+            const double gv = (double)ga[q], go_ = (double)cur.go[q];
+            n2 += gv * gv;
+            dg += gv * (gv - go_);
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+#pragma unroll
+          for (int r = 0; r < 2; ++r) {
+            const double gv = (double)gp[q][r], go_ = (double)cur.go[PU1 + 2 * q + r];
+            n2 += gv * gv;
+            dg += gv * (gv - go_);
+          }
+        }
+        // Per-lane running sums over the segment's rows in double; the
+        // compensation (mixed) starts at the cross-lane and cross-tile sums.
+        acc_s[0] += n2;
+        acc_s[1] += dg;
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      if constexpr (BATCH) {
+        g_process<T, NC>(w[IP], raw, al, UPD);
+        pin_row<T, NC>(w[IP]);
+        pin_side<T, NC, PL>(sd[IN]);
+        memory_fence_compiler();
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      if (live) {
+        const unsigned o = (unsigned)((j * P_ + t.ic) * (int)sizeof(T));
+        if constexpr (!MW) {
+#pragma unroll
+          for (int q = 0; q < NC; ++q) {
+            bstore(gp[q][0], b.g, o, (PU1 + 2 * q) * b.ps);
+            bstore(gp[q][1], b.g, o, (PV1 + 2 * q) * b.ps);
+          }
+        }
+        if (do_a) {
+          bstore(ga[0], b.g, o, PAX * b.ps);
+          bstore(ga[1], b.g, o, PAY * b.ps);
+        }
+        if (UPD) {
+          if (do_a) {
+            bstore(wc.ax, b.xo, o, PAX * b.ps);
+            bstore(wc.ay, b.xo, o, PAY * b.ps);
+          }
+#pragma unroll
+          for (int q = 0; q < NC; ++q) {
+            bstore(wc.u[q], b.xo, o, (PU1 + 2 * q) * b.ps);
+            bstore(wc.v[q], b.xo, o, (PV1 + 2 * q) * b.ps);
+          }
+        }
+      }
+    };
+    // All rows of the segment but the last carry a batch; they run in
+    // rotations of three, the last rotation padded past the segment's end when
+    // their count is not a multiple of three.
+    const int groups = (t.j1 - 1 - t.j0 + 2) / 3;
+    int j = jb;
+#pragma clang loop unroll(disable)
+    for (int g = 0; g < groups; ++g, j += 3 * D) {
+      row(j, S0(), S1(), S2(), std::true_type());
+      row(j + D, S1(), S2(), S0(), std::true_type());
+      row(j + 2 * D, S2(), S0(), S1(), std::true_type());
+    }
+    row(j, S0(), S1(), S2(), std::false_type());
+  };
+  if (up)
+    march(std::true_type());
+  else
+    march(std::false_type());
+  if (a.scratch) return;
+
+  publish_partial<2, COMP>(acc_s, acc_c, s.part_g, t.n, t.tile, s.tiles_x * s.tiles_y);
+}
+
+template <class T, int NC, bool UPD, bool NSF, bool PL, bool MW>
+__global__ __launch_bounds__(kBlock) void k_grad(StepArgs a) {
+  const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
+  grad_body<T, NC, UPD, NSF, PL, MW>(a, a.strings[it.string], it);
+}
+
+// ---------------------------------------------------------------------------
+// P kernel.
+template <class T, int NC>
+struct PRaw {
+  static constexpr int P = 2 + 2 * NC;
+  T x[P], g[P], d[P], hx[P], hg[P], hd[P];
+  uint16_t nn;
+};
+
+template <class T, int NC>
+struct PRow {
+  T ax, ay, dax, day;              // A and its new direction
+  T eax, eay, edax, eday;          // the same outside the strip (lanes 0 / 63)
+  T u[NC], v[NC], du[NC], dv[NC];  // psi and its new direction
+  T eu[NC], ev[NC], edu[NC], edv[NC];
+  uint16_t nn;
+};
+
+struct PBufs {
+  Rsrc x, g, d, dout, nn;
+  unsigned ps;
+};
+
+template <class T, int NC, bool NSF>
+__device__ __forceinline__ void p_load(PRaw<T, NC>& r, const PBufs& b, int row, int P_,
+                                       const Strip& t, unsigned nnoff) {
+  const RowOff o = row_off<T>(row, P_, t);
+#pragma unroll
+  for (int p = 0; p < PRaw<T, NC>::P; ++p) {
+    const bool dir = !(NSF && p < 2);
+    r.x[p] = bload<T>(b.x, o.own, p * b.ps);
+    r.g[p] = dir ? bload<T>(b.g, o.own, p * b.ps) : (T)0;
+    r.d[p] = dir ? bload<T>(b.d, o.own, p * b.ps) : (T)0;
+  }
+#pragma unroll
+  for (int p = 0; p < PRaw<T, NC>::P; ++p) r.hx[p] = r.hg[p] = r.hd[p] = (T)0;
+  if (t.edge) {
+#pragma unroll
+    for (int p = 0; p < PRaw<T, NC>::P; ++p) {
+      const bool dir = !(NSF && p < 2);
+      r.hx[p] = bload<T>(b.x, o.halo, p * b.ps);
+      r.hg[p] = dir ? bload<T>(b.g, o.halo, p * b.ps) : (T)0;
+      r.hd[p] = dir ? bload<T>(b.d, o.halo, p * b.ps) : (T)0;
+    }
+  }
+  r.nn = bload_u16(b.nn, nnoff);
+}
+
+// d = -g + beta d  (computeStepDirection, nlcg.cu:985-993)
+template <class T, int NC>
+__device__ __forceinline__ void pin_row(PRow<T, NC>& w) {
+  pin(w.ax), pin(w.ay), pin(w.dax), pin(w.day), pin(w.eax), pin(w.eay), pin(w.edax), pin(w.eday);
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    pin(w.u[q]), pin(w.v[q]), pin(w.du[q]), pin(w.dv[q]);
+    pin(w.eu[q]), pin(w.ev[q]), pin(w.edu[q]), pin(w.edv[q]);
+  }
+  pin(w.nn);
+}
+
+template <class T, int NC>
+__device__ __forceinline__ void p_process(PRow<T, NC>& w, const PRaw<T, NC>& r, T beta) {
+  T dn[PRaw<T, NC>::P], en[PRaw<T, NC>::P];
+#pragma unroll
+  for (int p = 0; p < PRaw<T, NC>::P; ++p) {
+    dn[p] = fma(beta, r.d[p], -r.g[p]);
+    en[p] = fma(beta, r.hd[p], -r.hg[p]);
+  }
+  w.ax = r.x[PAX];
+  w.ay = r.x[PAY];
+  w.eax = r.hx[PAX];
+  w.eay = r.hx[PAY];
+  w.dax = dn[PAX];
+  w.day = dn[PAY];
+  w.edax = en[PAX];
+  w.eday = en[PAY];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    w.u[q] = r.x[PU1 + 2 * q];
+    w.v[q] = r.x[PV1 + 2 * q];
+    w.eu[q] = r.hx[PU1 + 2 * q];
+    w.ev[q] = r.hx[PV1 + 2 * q];
+    w.du[q] = dn[PU1 + 2 * q];
+    w.dv[q] = dn[PV1 + 2 * q];
+    w.edu[q] = en[PU1 + 2 * q];
+    w.edv[q] = en[PV1 + 2 * q];
+  }
+  w.nn = r.nn;
+}
+
+template <class T, int NC, bool NSF, bool PL>
+__device__ __forceinline__ void poly_body(const StepArgs& a, const DevString& s, const WorkItem& it) {
+  constexpr bool COMP = sizeof(T) == 4;
+  const Strip t = strip_of(s, it);
+  const int N = s.N, P_ = s.pitch;
+  const long long ibytes = s.image_stride * (long long)sizeof(T);
+  const long long ib = (long long)t.n * s.image_stride;
+  PBufs b;
+  b.x = rsrc_of((const T*)s.x[a.xin] + ib, ibytes);
+  b.g = rsrc_of((const T*)s.g[a.gcur] + ib, ibytes);
+  b.d = rsrc_of((const T*)s.d[a.din] + ib, ibytes);
+  b.dout = rsrc_of((const T*)s.d[a.dout] + ib, ibytes);
+  b.nn = rsrc_of(s.nn, (long long)s.N * s.pitch * (long long)sizeof(uint16_t));
+  b.ps = (unsigned)(s.plane_stride * (long long)sizeof(T));
+  const ImageScalars& sc = a.sc[s.img0 + t.n];
+  const T beta = (T)sc.beta;
+  constexpr bool nsf = NSF;
+  const bool exact = s.poly_exact;
+  const bool store = t.i < N;
+  const T idx = (T)s.inv_dx, dx2 = (T)s.dx2;
+  const T eta = (T)s.eta, gam = (T)s.gamma, del = (T)s.delta;
+  CompConst<T> kconst[NC];
+  if (!PL) load_comp_const<T, NC>(s, 0, kconst);
+  const ParamView<NC> pv = param_view<NC>(s);
+  gptr<const double> __restrict__ hp = as_global(s.h.p);
+  const T hv = (T)s.h.v;
+  double acc_s[5] = {0, 0, 0, 0, 0}, acc_c[5] = {0, 0, 0, 0, 0};
+  auto nnoff = [&](int row) { return (unsigned)((row * P_ + t.ic) * (int)sizeof(uint16_t)); };
+
+  // The march is the only part that depends on the direction
+  // (marches_up); everything it needs from the string's descriptor is
+  // read above, before the branch and before any store, so it stays in
+  // scalar registers.
+  const bool up = marches_up(s, it);
+  using S0 = std::integral_constant<int, 0>;
+  using S1 = std::integral_constant<int, 1>;
+  using S2 = std::integral_constant<int, 2>;
+  auto march = [&](auto up_tag) {
+    constexpr bool UP = decltype(up_tag)::value;
+    constexpr int D = UP ? -1 : 1;  // march direction in rows
+    // Three row slots in rotation, as in the G kernel: w[IP] is the previous
+    // row of the march, w[IC] the current, w[IN] the next; the parameters of a
+    // row (parameter planes) sit in the slot of the same index.  Of row j - 1
+    // only A_x and its direction are read.
+    const int jb = UP ? t.j1 - 1 : t.j0;  // first row of the march
+    PRaw<T, NC> raw;
+    PRow<T, NC> w[3];
+    RowParams<T, NC, PL> rp[3];
+    p_load<T, NC, NSF>(raw, b, clampr(jb - D, N), P_, t, nnoff(clampr(jb - D, N)));
+    p_process<T, NC>(w[0], raw, beta);
+    p_load<T, NC, NSF>(raw, b, clampr(jb, N), P_, t, nnoff(clampr(jb, N)));
+    p_process<T, NC>(w[1], raw, beta);
+    p_load<T, NC, NSF>(raw, b, clampr(jb + D, N), P_, t, nnoff(clampr(jb + D, N)));
+    p_process<T, NC>(w[2], raw, beta);
+    rp[1].load(pv, hp, (long long)jb * P_ + t.ic, jb, N, P_, t);
+
+    // Row j: one batch of loads (the fields of row j + 2D, the parameters of
+    // row j + D), the quartic of row j from registers, the batch into the
+    // previous row's slot, then the stores of row j (see the G loop).
+    auto row = [&](int j, auto ip_tag, auto ic_tag, auto in_tag, auto batch_tag) {
+      constexpr int IP = decltype(ip_tag)::value, IC = decltype(ic_tag)::value;
+      constexpr int IN = decltype(in_tag)::value;
+      constexpr bool BATCH = decltype(batch_tag)::value;
+      PRow<T, NC>& wm = w[UP ? IN : IP];  // row j - 1
+      PRow<T, NC>& wc = w[IC];
+      PRow<T, NC>& wpl = w[UP ? IP : IN];  // row j + 1
+      const bool live = store && (UP ? j >= t.j0 : j < t.j1);
+      if constexpr (BATCH) {
+        p_load<T, NC, NSF>(raw, b, clampr(j + 2 * D, N), P_, t, nnoff(clampr(j + 2 * D, N)));
+        const int r1 = clampr(j + D, N);
+        rp[IN].load(pv, hp, (long long)r1 * P_ + t.ic, r1, N, P_, t);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+
+      PolyNbhd<T, NC> nb;
+      nb.ax[0] = wc.ax;
+      nb.ax[1] = from_right(wc.ax, wc.eax);
+      nb.ax[2] = 0;
+      nb.ax[3] = wpl.ax;
+      nb.ax[4] = wm.ax;
+      nb.ay[0] = wc.ay;
+      nb.ay[1] = from_right(wc.ay, wc.eay);
+      nb.ay[2] = from_left(wc.ay, wc.eay);
+      nb.ay[3] = wpl.ay;
+      nb.ay[4] = 0;
+      nb.dax[0] = wc.dax;
+      nb.dax[1] = from_right(wc.dax, wc.edax);
+      nb.dax[2] = 0;
+      nb.dax[3] = wpl.dax;
+      nb.dax[4] = wm.dax;
+      nb.day[0] = wc.day;
+      nb.day[1] = from_right(wc.day, wc.eday);
+      nb.day[2] = from_left(wc.day, wc.eday);
+      nb.day[3] = wpl.day;
+      nb.day[4] = 0;
+#pragma unroll
+      for (int q = 0; q < NC; ++q) {
+        nb.u[q][0] = wc.u[q];
+        nb.v[q][0] = wc.v[q];
+        nb.du[q][0] = wc.du[q];
+        nb.dv[q][0] = wc.dv[q];
+        nb.u[q][1] = from_right(wc.u[q], wc.eu[q]);
+        nb.v[q][1] = from_right(wc.v[q], wc.ev[q]);
+        nb.du[q][1] = from_right(wc.du[q], wc.edu[q]);
+        nb.dv[q][1] = from_right(wc.dv[q], wc.edv[q]);
+        nb.u[q][2] = wpl.u[q];
+        nb.v[q][2] = wpl.v[q];
+        nb.du[q][2] = wpl.du[q];
+        nb.dv[q][2] = wpl.dv[q];
+      }
+      CompConst<T> k[NC];
+      if constexpr (PL) {
+        comp_from_params<T, NC>(rp[IC].prm, pv.q, k);
+        nb.h0 = hp ? rp[IC].h[0] : hv;
+      } else {
+#pragma unroll
+        for (int q = 0; q < NC; ++q) k[q] = kconst[q];
+        nb.h0 = hv;
+      }
+
+      if (live) {
+        const Quartic<T> qv =
+            site_poly<T, NC>(nb, k, eta, gam, del, idx, dx2, wc.nn & 0xff, wc.nn >> 8, nsf, exact);
+#pragma unroll
+        for (int r = 0; r < 5; ++r) acc_s[r] += (double)qv.c[r];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      if constexpr (BATCH) {
+        p_process<T, NC>(w[IP], raw, beta);
+        pin_row<T, NC>(w[IP]);
+        if constexpr (PL) {
+#pragma unroll
+          for (int q = 0; q < NC; ++q)
+#pragma unroll
+            for (int k2 = 0; k2 < 4; ++k2) pin(rp[IN].prm[q][k2]);
+#pragma unroll
+          for (int k2 = 0; k2 < 5; ++k2) pin(rp[IN].h[k2]);
+        }
+        memory_fence_compiler();
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      if (live) {
+        const unsigned o = (unsigned)((j * P_ + t.ic) * (int)sizeof(T));
+        if (!nsf) {
+          bstore(wc.dax, b.dout, o, PAX * b.ps);
+          bstore(wc.day, b.dout, o, PAY * b.ps);
+        }
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+          bstore(wc.du[q], b.dout, o, (PU1 + 2 * q) * b.ps);
+          bstore(wc.dv[q], b.dout, o, (PV1 + 2 * q) * b.ps);
+        }
+      }
+    };
+    const int groups = (t.j1 - 1 - t.j0 + 2) / 3;
+    int j = jb;
+#pragma clang loop unroll(disable)
+    for (int g = 0; g < groups; ++g, j += 3 * D) {
+      row(j, S0(), S1(), S2(), std::true_type());
+      row(j + D, S1(), S2(), S0(), std::true_type());
+      row(j + 2 * D, S2(), S0(), S1(), std::true_type());
+    }
+    row(j, S0(), S1(), S2(), std::false_type());
+  };
+  if (up)
+    march(std::true_type());
+  else
+    march(std::false_type());
+
+  publish_partial<5, COMP>(acc_s, acc_c, s.part_p, t.n, t.tile, s.tiles_x * s.tiles_y);
+}
+
+template <class T, int NC, bool NSF, bool PL>
+__global__ __launch_bounds__(kBlock) void k_poly(StepArgs a) {
+  const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
+  poly_body<T, NC, NSF, PL>(a, a.strings[it.string], it);
+}
+
+// ---------------------------------------------------------------------------
+// Pending update applied in place (updateFields, nlcg.cu:1372-1396), used
+// before anything other than the next gradient reads the fields.
+template <class T>
+__global__ __launch_bounds__(kBlock) void k_update(StepArgs a) {
+  const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
+  const DevString& s = a.strings[it.string];
+  const Strip t = strip_of(s, it);
+  if (t.i >= s.N) return;
+  const long long ib = (long long)t.n * s.image_stride;
+  T* X = (T*)s.x[a.xin] + ib;
+  const T* D = (const T*)s.d[a.din] + ib;
+  const T al = (T)a.sc[s.img0 + t.n].alpha;
+  const int p0 = s.nsf ? 2 : 0;
+  for (int j = t.j0; j < t.j1; ++j) {
+    const long long off = (long long)j * s.pitch + t.i;
+    for (int p = p0; p < s.nplanes; ++p) {
+      const long long o = p * s.plane_stride + off;
+      X[o] = fma(al, D[o], X[o]);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host launchers of the step kernels.
+template <class T, int NC, bool NSF, bool PL>
+static void launch_gp_pl(const StepArgs& a, int which, const dim3& g, const dim3& b,
+                         hipStream_t st) {
+  if (which == kKernelPoly)
+    hipLaunchKernelGGL((k_poly<T, NC, NSF, PL>), g, b, 0, st, a);
+  else if (a.maxwell && a.upd)
+    hipLaunchKernelGGL((k_grad<T, NC, true, NSF, PL, true>), g, b, 0, st, a);
+  else if (a.maxwell)
+    hipLaunchKernelGGL((k_grad<T, NC, false, NSF, PL, true>), g, b, 0, st, a);
+  else if (a.upd)
+    hipLaunchKernelGGL((k_grad<T, NC, true, NSF, PL, false>), g, b, 0, st, a);
+  else
+    hipLaunchKernelGGL((k_grad<T, NC, false, NSF, PL, false>), g, b, 0, st, a);
+}
+
+template <class T, int NC, bool NSF>
+static void launch_gp(const StepArgs& a, int which, const dim3& g, const dim3& b, hipStream_t st) {
+  if (a.pl)
+    launch_gp_pl<T, NC, NSF, true>(a, which, g, b, st);
+  else
+    launch_gp_pl<T, NC, NSF, false>(a, which, g, b, st);
+}
+
+// Launches the step kernel `which` for items of NC components (the in-place
+// update, which does not depend on NC, is launched by the NC = 1 unit).
+template <class T, int NC>
+hipError_t launch_step_nc(const StepArgs& a, int which, hipStream_t st) {
+  if (a.nitems <= 0) return hipSuccess;
+  const dim3 g((unsigned)a.nitems), b(kBlock);
+  switch (which) {
+    case kKernelGrad:
+    case kKernelPoly:
+      a.nsf ? launch_gp<T, NC, true>(a, which, g, b, st) : launch_gp<T, NC, false>(a, which, g, b, st);
+      break;
+    case kKernelUpdate:
+      if constexpr (NC != 1) return hipErrorInvalidValue;
+      hipLaunchKernelGGL(k_update<T>, g, b, 0, st, a);
+      break;
+    default:
+      return hipErrorInvalidValue;
+  }
+  return hipGetLastError();
+}
+
+}  // namespace mepgl
