@@ -3,7 +3,7 @@
 // launches of an NLCG step, the energy and JB passes, arclength and
 // reparametrisation, gauge alignment, scalar maintenance, and the host
 // launchers of all kernels.
-#include "kernels_common.hpp"
+#include "row_window.hpp"
 
 namespace mepgl {
 
@@ -228,40 +228,154 @@ __global__ __launch_bounds__(kBlock) void k_jb(AuxArgs a) {
 // Arclength partials (arcLengthParametrization + arcLengthParametrizationJB,
 // interpolation.cu:39-92): squared distance of image n to image n-1 in the
 // plain metric and in the gauge-invariant (b, j) metric, with b and j
-// recomputed on the fly for both images.  Image 0 contributes nothing.
-template <class T, int NC>
-__device__ void arclen_body(const AuxArgs& a, const DevString& s, const WorkItem& it) {
-  const Strip t = strip_of(s, it);
+// recomputed on the fly for both images.  Image 0 contributes nothing.  The
+// wavefront keeps a three-row window of both images, as the G kernel does:
+// buffer loads of row j + 2 issued while row j is computed, x-neighbours
+// through DPP, and neighbours outside the grid clamped to its edge, so each
+// site's values are those of the reference's clamped 5-point gather.
+// Buffers of images n - 1 and n of an arclength work item (image 0 uses
+// itself; it contributes nothing).  They are built by the kernel before it
+// branches on the component count, so that the buffer descriptors stay
+// wave-uniform scalars.
+struct ArcBufs {
+  GBufs b0, b1;
+};
+
+template <class T>
+__device__ __forceinline__ ArcBufs arc_bufs(const AuxArgs& a, const DevString& s, const Strip& t) {
+  const long long ibytes = s.image_stride * (long long)sizeof(T);
+  const T* X = (const T*)s.x[a.xin];
+  ArcBufs ab;
+  ab.b0.x = rsrc_of(X + (long long)(t.n > 0 ? t.n - 1 : 0) * s.image_stride, ibytes);
+  ab.b1.x = rsrc_of(X + (long long)t.n * s.image_stride, ibytes);
+  ab.b0.nn = ab.b1.nn = rsrc_of(s.nn, (long long)s.N * s.pitch * (long long)sizeof(uint16_t));
+  ab.b0.ps = ab.b1.ps = (unsigned)(s.plane_stride * (long long)sizeof(T));
+  return ab;
+}
+
+// PL: some parameter is a per-site plane (read per row); otherwise the
+// parameters are read once, so no load outside the row batches drains the
+// loads in flight.
+template <class T, int NC, bool PL>
+__device__ __forceinline__ void arclen_body(const AuxArgs& a, const DevString& s, const Strip& t,
+                                            const ArcBufs& ab) {
   double acc_s[2] = {0, 0}, acc_c[2] = {0, 0};
-  if (t.n > 0 && t.i < s.N) {
-    const T* X1 = (const T*)s.x[a.xin] + (long long)t.n * s.image_stride;
-    const T* X0 = X1 - s.image_stride;
-    for (int j = t.j0; j < t.j1; ++j) {
-      const long long off = (long long)j * s.pitch + t.i;
-      double d2 = 0;
-      for (int p = 0; p < 2 + 2 * NC; ++p) {
-        const double dv =
-            (double)X1[p * s.plane_stride + off] - (double)X0[p * s.plane_stride + off];
-        d2 += dv * dv;
-      }
-      CompConst<double> k[NC];
-      load_comp_const<double, NC>(s, off, k);
-      const uint16_t nnv = s.nn[off];
-      double b[2], jj[2][NC][2];
+  if (t.n > 0) {
+    const int N = s.N, P_ = s.pitch;
+    const GBufs& b0 = ab.b0;  // image n - 1
+    const GBufs& b1 = ab.b1;  // image n
+    const bool lane_in = t.i < N;
+    const double idx = s.inv_dx;
+    CompConst<double> kconst[NC];
+    if constexpr (!PL) load_comp_const<double, NC>(s, 0, kconst);
+    auto nnoff = [&](int row) { return (unsigned)((row * P_ + t.ic) * (int)sizeof(uint16_t)); };
+    GRaw<T, NC> r0, r1;
+    GRow<T, NC> w0[3], w1[3];
 #pragma unroll
-      for (int m = 0; m < 2; ++m) {
-        PolyNbhd<double, NC> nb;
-        double um[NC], vm[NC], uym[NC], vym[NC];
-        gather5<T, NC>(s, m ? X1 : X0, t.i, j, nb, um, vm, uym, vym);
-        site_jb<NC>(nb, um, vm, uym, vym, k, s.inv_dx, nnv & 0xff, nnv >> 8, b[m], jj[m]);
-      }
-      double djb = sq(b[1] - b[0]);
-#pragma unroll
-      for (int c = 0; c < NC; ++c)
-        djb += sq(jj[1][c][0] - jj[0][c][0]) + sq(jj[1][c][1] - jj[0][c][1]);
-      two_sum_acc(acc_s[0], acc_c[0], d2);
-      two_sum_acc(acc_s[1], acc_c[1], djb);
+    for (int k = 0; k < 3; ++k) {
+      const int row = clampr(t.j0 - 1 + k, N);
+      g_load<T, NC, false, false>(r0, b0, row, P_, t, nnoff(row));
+      g_load<T, NC, false, false>(r1, b1, row, P_, t, nnoff(row));
+      g_process<T, NC>(w0[k], r0, (T)0, false);
+      g_process<T, NC>(w1[k], r1, (T)0, false);
     }
+    // b and j of one image at row j from its window rows (previous,
+    // current, next).
+    auto jb = [&](const GRow<T, NC>& wp, const GRow<T, NC>& wc, const GRow<T, NC>& wn,
+                  const CompConst<double> (&k)[NC], double& b, double (&jj)[NC][2]) {
+      PolyNbhd<double, NC> nb;
+      double um[NC], vm[NC], uym[NC], vym[NC];
+      nb.ax[0] = (double)wc.ax;  nb.ay[0] = (double)wc.ay;
+      nb.ax[1] = (double)wc.axr; nb.ay[1] = (double)wc.ayr;  // +x
+      nb.ax[2] = (double)wc.axl; nb.ay[2] = (double)wc.ayl;  // -x
+      nb.ax[3] = (double)wn.ax;  nb.ay[3] = (double)wn.ay;   // +y
+      nb.ax[4] = (double)wp.ax;  nb.ay[4] = (double)wp.ay;   // -y
+#pragma unroll
+      for (int q = 0; q < 5; ++q) nb.dax[q] = nb.day[q] = 0;
+#pragma unroll
+      for (int c = 0; c < NC; ++c) {
+        nb.u[c][0] = (double)wc.u[c];
+        nb.v[c][0] = (double)wc.v[c];
+        nb.u[c][1] = (double)from_right(wc.u[c], wc.eu[c]);
+        nb.v[c][1] = (double)from_right(wc.v[c], wc.ev[c]);
+        nb.u[c][2] = (double)wn.u[c];
+        nb.v[c][2] = (double)wn.v[c];
+        um[c] = (double)from_left(wc.u[c], wc.eu[c]);
+        vm[c] = (double)from_left(wc.v[c], wc.ev[c]);
+        uym[c] = (double)wp.u[c];
+        vym[c] = (double)wp.v[c];
+#pragma unroll
+        for (int q = 0; q < 3; ++q) nb.du[c][q] = nb.dv[c][q] = 0;
+      }
+      nb.h0 = 0;
+      site_jb<NC>(nb, um, vm, uym, vym, k, idx, wc.nn & 0xff, wc.nn >> 8, b, jj);
+    };
+    // Row j: loads of row j + 2 for both images, the row's distances, the
+    // loads into the previous row's slots (see the G kernel's loop).
+    auto row = [&](int j, auto ip_tag, auto ic_tag, auto in_tag, auto batch_tag) {
+      constexpr int IP = decltype(ip_tag)::value, IC = decltype(ic_tag)::value;
+      constexpr int IN = decltype(in_tag)::value;
+      constexpr bool BATCH = decltype(batch_tag)::value;
+      if constexpr (BATCH) {
+        const int r2 = clampr(j + 2, N);
+        g_load<T, NC, false, false>(r0, b0, r2, P_, t, nnoff(r2));
+        g_load<T, NC, false, false>(r1, b1, r2, P_, t, nnoff(r2));
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      if (lane_in && j < t.j1) {
+        const GRow<T, NC>& c0 = w0[IC];
+        const GRow<T, NC>& c1 = w1[IC];
+        // Plain metric, planes in storage order.
+        T p0[2 + 2 * NC], p1[2 + 2 * NC];
+        p0[PAX] = c0.ax, p0[PAY] = c0.ay, p1[PAX] = c1.ax, p1[PAY] = c1.ay;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+          p0[PU1 + 2 * c] = c0.u[c], p0[PV1 + 2 * c] = c0.v[c];
+          p1[PU1 + 2 * c] = c1.u[c], p1[PV1 + 2 * c] = c1.v[c];
+        }
+        double d2 = 0;
+#pragma unroll
+        for (int p = 0; p < 2 + 2 * NC; ++p) {
+          const double dv = (double)p1[p] - (double)p0[p];
+          d2 += dv * dv;
+        }
+        CompConst<double> k[NC];
+        if constexpr (PL) {
+          load_comp_const<double, NC>(s, (long long)j * P_ + t.i, k);
+        } else {
+#pragma unroll
+          for (int c = 0; c < NC; ++c) k[c] = kconst[c];
+        }
+        double b[2], jj[2][NC][2];
+        jb(w0[IP], c0, w0[IN], k, b[0], jj[0]);
+        jb(w1[IP], c1, w1[IN], k, b[1], jj[1]);
+        double djb = sq(b[1] - b[0]);
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+          djb += sq(jj[1][c][0] - jj[0][c][0]) + sq(jj[1][c][1] - jj[0][c][1]);
+        two_sum_acc(acc_s[0], acc_c[0], d2);
+        two_sum_acc(acc_s[1], acc_c[1], djb);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      if constexpr (BATCH) {
+        g_process<T, NC>(w0[IP], r0, (T)0, false);
+        g_process<T, NC>(w1[IP], r1, (T)0, false);
+        pin_row<T, NC>(w0[IP]);
+        pin_row<T, NC>(w1[IP]);
+      }
+    };
+    using S0 = std::integral_constant<int, 0>;
+    using S1 = std::integral_constant<int, 1>;
+    using S2 = std::integral_constant<int, 2>;
+    const int groups = (t.j1 - 1 - t.j0 + 2) / 3;
+    int j = t.j0;
+#pragma clang loop unroll(disable)
+    for (int g = 0; g < groups; ++g, j += 3) {
+      row(j, S0(), S1(), S2(), std::true_type());
+      row(j + 1, S1(), S2(), S0(), std::true_type());
+      row(j + 2, S2(), S0(), S1(), std::true_type());
+    }
+    row(j, S0(), S1(), S2(), std::false_type());
   }
   publish_partial<2, true>(acc_s, acc_c, s.part_a, t.n, t.tile, s.tiles_x * s.tiles_y);
 }
@@ -270,10 +384,19 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_arclen(AuxArgs a) {
   const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
   const DevString& s = a.strings[it.string];
-  if (s.nc == 2)
-    arclen_body<T, 2>(a, s, it);
-  else
-    arclen_body<T, 1>(a, s, it);
+  const Strip t = strip_of(s, it);
+  const ArcBufs ab = arc_bufs<T>(a, s, t);
+  if (s.nc == 2) {
+    if (s.planes)
+      arclen_body<T, 2, true>(a, s, t, ab);
+    else
+      arclen_body<T, 2, false>(a, s, t, ab);
+  } else {
+    if (s.planes)
+      arclen_body<T, 1, true>(a, s, t, ab);
+    else
+      arclen_body<T, 1, false>(a, s, t, ab);
+  }
 }
 
 // computeArcLength (interpolation.cu:126-140) for both metrics, then the
