@@ -257,8 +257,7 @@ __device__ __forceinline__ ArcBufs arc_bufs(const AuxArgs& a, const DevString& s
 // parameters are read once, so no load outside the row batches drains the
 // loads in flight.
 template <class T, int NC, bool PL>
-__device__ __forceinline__ void arclen_body(const AuxArgs& a, const DevString& s, const Strip& t,
-                                            const ArcBufs& ab) {
+__device__ __forceinline__ void arclen_body(const DevString& s, const Strip& t, const ArcBufs& ab) {
   double acc_s[2] = {0, 0}, acc_c[2] = {0, 0};
   if (t.n > 0) {
     const int N = s.N, P_ = s.pitch;
@@ -388,14 +387,14 @@ __global__ __launch_bounds__(kBlock) void k_arclen(AuxArgs a) {
   const ArcBufs ab = arc_bufs<T>(a, s, t);
   if (s.nc == 2) {
     if (s.planes)
-      arclen_body<T, 2, true>(a, s, t, ab);
+      arclen_body<T, 2, true>(s, t, ab);
     else
-      arclen_body<T, 2, false>(a, s, t, ab);
+      arclen_body<T, 2, false>(s, t, ab);
   } else {
     if (s.planes)
-      arclen_body<T, 1, true>(a, s, t, ab);
+      arclen_body<T, 1, true>(s, t, ab);
     else
-      arclen_body<T, 1, false>(a, s, t, ab);
+      arclen_body<T, 1, false>(s, t, ab);
   }
 }
 

@@ -155,13 +155,14 @@ __device__ __forceinline__ void bond_bwd(T u0, T v0, T A0, T um, T vm, T Am, T q
 }
 
 // Full gradient of one site (gradientPsi + gradientA, nlcg.cu:504-888),
-// already multiplied by dx^2.  do_psi = false skips the psi part (Maxwell
-// mode, nlcg.cu:1521); do_a = false skips the A part (NO_SELF_FIELD).
+// already multiplied by dx^2.  do_a = false skips the A part
+// (NO_SELF_FIELD).  The psi part is always evaluated: its kinetic bonds also
+// give the bond terms of the A gradient, and in Maxwell mode (nlcg.cu:1521)
+// the caller keeps the frozen psi gradient instead of gpsi.
 template <class T, int NC>
 __device__ __forceinline__ void site_gradient(const Nbhd<T, NC>& s, const CompConst<T> (&k)[NC],
                                               T eta, T gamma, T delta, T idx, T dx2, int sc,
-                                              int cm, bool do_psi, bool do_a, T (&gpsi)[NC][2],
-                                              T (&ga)[2]) {
+                                              int cm, bool do_a, T (&gpsi)[NC][2], T (&ga)[2]) {
   const T af = (T)quad_count(sc) * (T)0.25;
   const T wxf = (T)(bitof(sc, 1) * bitof(sc, 2) + bitof(sc, 6) * bitof(sc, 7)) * (T)0.5;
   const T wxb = (T)(bitof(sc, 2) * bitof(sc, 3) + bitof(sc, 5) * bitof(sc, 6)) * (T)0.5;

@@ -198,7 +198,7 @@ __device__ __forceinline__ void grad_body(const StepArgs& a, const DevString& s,
       }
 
       T gp[NC][2], ga[2] = {0, 0};
-      site_gradient<T, NC>(nb, k, eta, gam, del, idx, dx2, wc.nn & 0xff, wc.nn >> 8, !MW, do_a,
+      site_gradient<T, NC>(nb, k, eta, gam, del, idx, dx2, wc.nn & 0xff, wc.nn >> 8, do_a,
                            gp, ga);
       if constexpr (MW) {
 #pragma unroll
