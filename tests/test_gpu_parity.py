@@ -132,7 +132,8 @@ def test_maxwell_mode(oracle):
     assert normwise(p1, ref.psi_1) < 1e-11
 
 
-@pytest.mark.parametrize("case", ["one_comp", "two_comp", "two_comp_thin_film"])
+@pytest.mark.parametrize("case", ["one_comp", "one_comp_masked_planes", "two_comp",
+                                  "two_comp_masked", "two_comp_thin_film"])
 def test_arclength_and_linear_redistribution(case, oracle):
     s = small_string(N=20, F=6, seed=9, **CASES[case])
     say(f"[gpu] arclength + linear case={case}")
@@ -149,6 +150,21 @@ def test_arclength_and_linear_redistribution(case, oracle):
     assert normwise(p1, ref.psi_1) < 1e-12
     if s.multicomponent:
         assert normwise(p2, ref.psi_2) < 1e-12
+
+
+@pytest.mark.parametrize("rows", [4, 5])
+def test_arclength_multi_strip(rows, oracle):
+    """Two strips (the second 6 columns wide) and short segments: the halo
+    columns and rows of the arclength pass's row window at every seam."""
+    s = small_string(N=70, F=5, seed=49, multicomponent=True, masked=True)
+    say(f"[gpu] arclength N=70 rows={rows}")
+    ref = oracle.OracleString(s)
+    s_ref, sjb_ref = ref.arc_lengths()
+    with solver_for([s]) as sv:
+        sv.set_segment_rows(0, rows)
+        (s_gpu, sjb_gpu), = sv.arcLength()
+    assert rel_err(s_gpu, s_ref) < 1e-12
+    assert rel_err(sjb_gpu, sjb_ref) < 1e-12
 
 
 @pytest.mark.parametrize("textbook", [False, True])
