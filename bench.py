@@ -274,7 +274,8 @@ def main(argv=None):
                    "components": strings[0].ncomp, "mode": "L", "reparam_every": R,
                    "segment_rows": mixed["info"]["tile_rows"],
                    "segment_rows_fp64": fp64["info"]["tile_rows"]},
-        "fp64": {"value": value64, "ms_per_step": 1e3 * fp64["seconds"] / fp64["steps"],
+        "fp64": {"value": value64, "value_active_cells": value64 * active,
+                 "ms_per_step": 1e3 * fp64["seconds"] / fp64["steps"],
                  "roofline": {"kernel": kname64, "achieved": ach64, "peak": HBM_PEAK_GBS,
                               "unit": "GB/s", "frac": ach64 / HBM_PEAK_GBS,
                               "avg_ms": kms64}},
@@ -290,6 +291,9 @@ def main(argv=None):
                      "measured_ceilings_GBs": ceil,
                      "frac_of_triad_ceiling": ach / ceil["triad"] if ceil.get("triad") else None},
         "active_cell_fraction": active,
+        # For masked geometries the metric counts mask-active image*cells
+        # only; equal to value when the domain is the full box.
+        "value_active_cells": value * active,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(strings, args.cpu_seconds)
