@@ -186,7 +186,10 @@ int mepgl_download_state(mepgl_ctx *ctx, int string, int which, double *a, doubl
  * run from zero gradients and directions (mepgl.cu:124-140); a resume from
  * its reloaded output files is mepgl_upload + mepgl_reset_solver.  A call
  * sequence download, download_state x3, get_scalars x2 ... upload,
- * upload_state x3, set_scalars x2 continues bit for bit. */
+ * upload_state x3, set_scalars x2 continues bit for bit.  At sites whose
+ * neighbour map is empty in both domains the gradient and direction must be
+ * zero, as every state the loop produces has them: the step kernels skip
+ * such sites' stencils and keep those values. */
 int mepgl_upload_state(mepgl_ctx *ctx, int string, int which, const double *a,
                        const double *psi_1, const double *psi_2);
 int mepgl_set_scalars(mepgl_ctx *ctx, int string, int which, const double *values);

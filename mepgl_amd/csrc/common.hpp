@@ -87,8 +87,13 @@ struct WorkItem {
   int string;
   int image;
   int tile;   // strip + tiles_x * segment
-  int pad;
+  int flags;  // kItemNoBonds
 };
+
+// Every site of the item's tile has an empty neighbour map: it lies outside
+// both domains and has no bond, quadrant or condensation term, so its
+// gradient, direction and line-search terms are exact zeros.
+constexpr int kItemNoBonds = 1;
 
 constexpr int kPartG = 4;   // g_norm2, g_delta_g (+ compensation terms)
 constexpr int kPartP = 10;  // A..E (+ compensation terms)

@@ -32,6 +32,7 @@ struct StringHost {
   DevString dev;            // host copy of the device descriptor
   std::vector<Alloc> owned; // device allocations of this string
   int ntiles = 0;
+  std::vector<uint16_t> nn_host;  // neighbour map, [row][column] at the row pitch
   double* jb = nullptr;     // JB planes (lazy)
   size_t elem = 8;          // bytes per stored real
 };
@@ -340,6 +341,22 @@ int check_finite(mepgl_ctx* c) {
   return MEPGL_OK;
 }
 
+// Per-tile item flags of a string at its current segment height.
+std::vector<int> tile_flags(const StringHost& sh) {
+  const DevString& s = sh.dev;
+  std::vector<int> f((size_t)s.tiles_x * s.tiles_y, 0);
+  if (sh.nn_host.empty()) return f;
+  for (int t = 0; t < s.tiles_x * s.tiles_y; ++t) {
+    const int x0 = (t % s.tiles_x) * kTileX, x1 = std::min(x0 + kTileX, s.N);
+    const int j0 = (t / s.tiles_x) * s.seg_rows, j1 = std::min(j0 + s.seg_rows, s.N);
+    bool bonds = false;
+    for (int j = j0; j < j1 && !bonds; ++j)
+      for (int i = x0; i < x1 && !bonds; ++i) bonds = sh.nn_host[(size_t)j * s.pitch + i] != 0;
+    f[t] = bonds ? 0 : kItemNoBonds;
+  }
+  return f;
+}
+
 int build_items(mepgl_ctx* c) {
   std::vector<WorkItem> items;
   c->groups.clear();
@@ -351,8 +368,10 @@ int build_items(mepgl_ctx* c) {
           const DevString& s = c->strings[si].dev;
           const int spl = (s.planes || s.h.p) ? 1 : 0;
           if (s.nc != nc || s.nsf != nsf || spl != pl) continue;
+          const std::vector<int> flags = tile_flags(c->strings[si]);
           for (int n = 0; n < s.F; ++n)
-            for (int t = 0; t < s.tiles_x * s.tiles_y; ++t) items.push_back({(int)si, n, t, 0});
+            for (int t = 0; t < s.tiles_x * s.tiles_y; ++t)
+              items.push_back({(int)si, n, t, flags[t]});
         }
         const int count = (int)items.size() - begin;
         if (count > 0) c->groups.push_back({begin, count, nc, nsf, pl});
@@ -856,6 +875,11 @@ int mepgl_create(const mepgl_string_desc* strings, int n_strings, int precision,
     if (he == hipSuccess)
       he = launch_nn_map((const int*)dsc, (const int*)dcm, in.N, d.pitch, (uint16_t*)nn, c->stream);
     if (he == hipSuccess) he = hipStreamSynchronize(c->stream);
+    if (he == hipSuccess) {
+      s.nn_host.resize((size_t)in.N * d.pitch);
+      he = hipMemcpy(s.nn_host.data(), nn, s.nn_host.size() * sizeof(uint16_t),
+                     hipMemcpyDeviceToHost);
+    }
     (void)hipFree(dsc);
     (void)hipFree(dcm);
     c->bytes -= 2 * (long long)db;

@@ -32,7 +32,7 @@ __device__ __forceinline__ WorkItem load_item(const WorkItem* items, int k) {
   u.string = __builtin_amdgcn_readfirstlane(w.string);
   u.image = __builtin_amdgcn_readfirstlane(w.image);
   u.tile = __builtin_amdgcn_readfirstlane(w.tile);
-  u.pad = 0;
+  u.flags = __builtin_amdgcn_readfirstlane(w.flags);
   return u;
 }
 

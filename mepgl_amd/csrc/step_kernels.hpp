@@ -119,6 +119,22 @@ __device__ __forceinline__ void grad_body(const StepArgs& a, const DevString& s,
   using S0 = std::integral_constant<int, 0>;
   using S1 = std::integral_constant<int, 1>;
   using S2 = std::integral_constant<int, 2>;
+  // Tiles without bonds (kItemNoBonds): g, d and every partial stay exact
+  // zeros (the loop never makes them non-zero there), so only the pending
+  // update of the fields is carried over, in the expression the window uses.
+  auto carry_update = [&]() {
+    if constexpr (UPD) {
+      if (!store) return;
+      constexpr int P0 = NSF ? 2 : 0;  // fixed A is not stored
+      for (int j = t.j0; j < t.j1; ++j) {
+        const unsigned o = (unsigned)((j * P_ + t.ic) * (int)sizeof(T));
+#pragma unroll
+        for (int p = P0; p < GRaw<T, NC>::P; ++p)
+          bstore(fma(al, bload<T>(b.d, o, p * b.ps), bload<T>(b.x, o, p * b.ps)), b.xo, o,
+                 p * b.ps);
+      }
+    }
+  };
   auto march = [&](auto up_tag) {
     constexpr bool UP = decltype(up_tag)::value;
     constexpr int D = UP ? -1 : 1;  // march direction in rows
@@ -279,7 +295,9 @@ __device__ __forceinline__ void grad_body(const StepArgs& a, const DevString& s,
     }
     row(j, S0(), S1(), S2(), std::false_type());
   };
-  if (up)
+  if (it.flags & kItemNoBonds)
+    carry_update();
+  else if (up)
     march(std::true_type());
   else
     march(std::false_type());
@@ -548,10 +566,14 @@ __device__ __forceinline__ void poly_body(const StepArgs& a, const DevString& s,
     }
     row(j, S0(), S1(), S2(), std::false_type());
   };
-  if (up)
-    march(std::true_type());
-  else
-    march(std::false_type());
+  // Tiles without bonds: the new direction is -0 + beta 0 = 0, which the
+  // other direction buffer already holds, and every quartic term is zero.
+  if (!(it.flags & kItemNoBonds)) {
+    if (up)
+      march(std::true_type());
+    else
+      march(std::false_type());
+  }
 
   publish_partial<5, COMP>(acc_s, acc_c, s.part_p, t.n, t.tile, s.tiles_x * s.tiles_y);
 }

@@ -167,6 +167,40 @@ def test_arclength_multi_strip(rows, oracle):
     assert rel_err(sjb_gpu, sjb_ref) < 1e-12
 
 
+def test_tiles_without_bonds(oracle):
+    """A domain that leaves whole tiles without any bond (the 2-column third
+    strip, the top and bottom segments): the step kernels skip their stencils
+    and only carry the fields over, and the loop must still match the oracle,
+    arclength over every cell and reparametrisation included."""
+    from mepgl_amd import configs
+
+    N, rows = 130, 7
+    s = small_string(N=N, F=4, seed=50, multicomponent=True)
+    x, y = configs.grid(N, s.dx)
+    dom = (np.hypot(x, y) < 0.3 * N * s.dx).astype(np.int32)
+    s.comp_domain = dom
+    s.sc_domain = dom.copy()
+    s.validate()
+    # The geometry does leave tiles (64 columns x 7 rows) whose neighbour
+    # maps are empty in both domains.
+    nn = (oracle.build_nn_map(dom) != 0).T
+    empty = [not nn[j0:j0 + rows, i0:i0 + 64].any()
+             for j0 in range(0, N, rows) for i0 in range(0, N, 64)]
+    assert sum(empty) >= 6
+    say(f"[gpu] tiles without bonds N={N} rows={rows} empty={sum(empty)}")
+    ref = oracle.OracleString(s)
+    ref.loop(2, 3, "L")
+    with solver_for([s]) as sv:
+        sv.set_segment_rows(0, rows)
+        sv.loop(2, 3, "L")
+        a, p1, p2 = sv.download(0)
+        fen = sv.scalars(0, "fenergy")
+    assert normwise(a, ref.a) < 1e-10
+    assert normwise(p1, ref.psi_1) < 1e-10
+    assert normwise(p2, ref.psi_2) < 1e-10
+    assert rel_err(fen, ref.fenergy) < 1e-12
+
+
 @pytest.mark.parametrize("textbook", [False, True])
 def test_cubic_redistribution(textbook, oracle):
     s = small_string(N=20, F=7, seed=10, multicomponent=True)
