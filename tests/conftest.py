@@ -52,6 +52,20 @@ def small_string(N=24, F=5, multicomponent=False, masked=False, no_self_field=Fa
     return s.validate()
 
 
+def bondless_string(no_self_field=False):
+    """N = 130 with a centred disk domain: the 2-column third strip and the
+    top and bottom segments of 7-row tiles have no bond at all."""
+    from mepgl_amd import configs
+
+    N = 130
+    s = small_string(N=N, F=4, seed=50, multicomponent=True, no_self_field=no_self_field)
+    x, y = configs.grid(N, s.dx)
+    dom = (np.hypot(x, y) < 0.3 * N * s.dx).astype(np.int32)
+    s.comp_domain = dom
+    s.sc_domain = dom.copy()
+    return s.validate()
+
+
 def rel_err(a, b):
     a = np.asarray(a, dtype=np.float64)
     b = np.asarray(b, dtype=np.float64)

@@ -13,7 +13,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import normwise, rel_err, small_string
+from conftest import bondless_string, normwise, rel_err, small_string
 
 pytestmark = pytest.mark.gpu
 
@@ -79,3 +79,19 @@ def test_mixed_loop_energies_and_barrier(case, oracle):
     b, _ = observables.barriers(e)
     b_ref, _ = observables.barriers(e_ref)
     assert abs(b - b_ref) <= 1e-3 * max(abs(b_ref), 1e-12) + 1e-5 * np.max(np.abs(e_ref))
+
+
+def test_mixed_tiles_without_bonds(oracle):
+    """The mixed path with tiles whose stencils are skipped (see
+    test_gpu_parity.test_tiles_without_bonds) against the FP64 oracle."""
+    from mepgl_amd.solver import Solver
+
+    s = bondless_string()
+    say("[gpu] mixed tiles without bonds")
+    with Solver([s], precision="mixed") as sv:
+        sv.set_segment_rows(0, 7)
+        sv.loop(2, 3, "L")
+        e = sv.computeFreeEnergy()[0]
+    ref = oracle.OracleString(s)
+    ref.loop(2, 3, "L")
+    assert rel_err(e, ref.free_energy()) < 1e-5

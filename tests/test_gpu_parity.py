@@ -15,7 +15,7 @@ import sys
 import numpy as np
 import pytest
 
-from conftest import normwise, rel_err, small_string
+from conftest import bondless_string, normwise, rel_err, small_string
 
 pytestmark = pytest.mark.gpu
 
@@ -167,27 +167,20 @@ def test_arclength_multi_strip(rows, oracle):
     assert rel_err(sjb_gpu, sjb_ref) < 1e-12
 
 
-def test_tiles_without_bonds(oracle):
-    """A domain that leaves whole tiles without any bond (the 2-column third
-    strip, the top and bottom segments): the step kernels skip their stencils
-    and only carry the fields over, and the loop must still match the oracle,
-    arclength over every cell and reparametrisation included."""
-    from mepgl_amd import configs
-
-    N, rows = 130, 7
-    s = small_string(N=N, F=4, seed=50, multicomponent=True)
-    x, y = configs.grid(N, s.dx)
-    dom = (np.hypot(x, y) < 0.3 * N * s.dx).astype(np.int32)
-    s.comp_domain = dom
-    s.sc_domain = dom.copy()
-    s.validate()
+@pytest.mark.parametrize("nsf", [False, True])
+def test_tiles_without_bonds(nsf, oracle):
+    """The step kernels skip the stencils of tiles without bonds and only
+    carry the fields over; the loop must still match the oracle, arclength
+    over every cell and reparametrisation included."""
+    s = bondless_string(no_self_field=nsf)
+    N, rows = s.N, 7
     # The geometry does leave tiles (64 columns x 7 rows) whose neighbour
     # maps are empty in both domains.
-    nn = (oracle.build_nn_map(dom) != 0).T
+    nn = (oracle.build_nn_map(s.comp_domain) != 0).T
     empty = [not nn[j0:j0 + rows, i0:i0 + 64].any()
              for j0 in range(0, N, rows) for i0 in range(0, N, 64)]
     assert sum(empty) >= 6
-    say(f"[gpu] tiles without bonds N={N} rows={rows} empty={sum(empty)}")
+    say(f"[gpu] tiles without bonds N={N} rows={rows} empty={sum(empty)} nsf={nsf}")
     ref = oracle.OracleString(s)
     ref.loop(2, 3, "L")
     with solver_for([s]) as sv:
