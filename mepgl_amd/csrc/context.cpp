@@ -131,7 +131,7 @@ size_t max_tiles(const DevString& d) {
 // Segment heights the library picks from.  The G and P kernels march rows
 // in rotations of three after the first, so heights of 3k + 1 rows waste no
 // work (any other height pads its last rotation).
-constexpr int kRowChoices[] = {4, 7, 10, 16, 31, 64, 127};
+constexpr int kRowChoices[] = {4, 7, 10, 13, 16, 22, 31, 64, 127};
 
 // Rows marched by one wavefront.  Taller segments re-read fewer halo rows;
 // shorter ones give more wavefronts per launch.  Pick the tallest segment

@@ -37,29 +37,31 @@ __global__ __launch_bounds__(kBlock) void k_finish_grad(FinishArgs f) {
 }
 
 // After P: the quartic A..E and the line search (lineSearch).  One
-// 1024-thread workgroup per image: wavefront 0 reduces the partials while
-// every thread loads its scan candidates p and p + 1024, so the scan takes
-// at most two rounds; the first rise is found with an LDS minimum.
-constexpr int kFinishPolyBlock = 1024;
+// 256-thread workgroup per image.  Thread t holds the scan candidates
+// 8t .. 8t + 7 in registers (x and x^4 at p and p + 1) while wavefront 0
+// reduces the partials.  The first rise is the lowest p with
+// f(x_{p+1}) > f(x_p): each lane finds its own, a ballot gives each
+// wavefront's lowest, and thread 0 takes the minimum over the wavefronts.
+constexpr int kFinishPolyBlock = 256;
+constexpr int kLsPerLane = 8;
+static_assert(kFinishPolyBlock * kLsPerLane >= kLsSteps, "the scan must cover every candidate");
 
 template <bool COMP>
 __global__ __launch_bounds__(kFinishPolyBlock) void k_finish_poly(FinishArgs f) {
   __shared__ double s_q[5];
-  __shared__ int s_first;
+  __shared__ int s_first[kFinishPolyBlock / kWave];
   const int2 si = f.img[blockIdx.x];
   const DevString& s = f.strings[si.x];
   const LineSearchTable& t = *f.ls;
   const int tid = threadIdx.x;
-  double x[2][2], x4[2][2];
+  const int p0 = tid * kLsPerLane;
+  double x[kLsPerLane + 1], x4[kLsPerLane + 1];
 #pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    const int p = min(tid + r * kFinishPolyBlock, kLsSteps - 1);
-    x[r][0] = t.x[p];
-    x[r][1] = t.x[p + 1];
-    x4[r][0] = t.x4[p];
-    x4[r][1] = t.x4[p + 1];
+  for (int k = 0; k <= kLsPerLane; ++k) {
+    const int p = min(p0 + k, kLsSteps);
+    x[k] = t.x[p];
+    x4[k] = t.x4[p];
   }
-  if (tid == 0) s_first = kLsSteps;
   if (tid < kWave) {
     double fin[5];
     reduce_partials<5, COMP>(s.part_p, si.y, s.tiles_x * s.tiles_y, fin);
@@ -77,22 +79,32 @@ __global__ __launch_bounds__(kFinishPolyBlock) void k_finish_poly(FinishArgs f) 
     v = __dadd_rn(v, __dmul_rn(B, xv));
     return __dadd_rn(v, A);
   };
+  double fv[kLsPerLane + 1];
 #pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    const int p = tid + r * kFinishPolyBlock;
-    if (p < kLsSteps && fq(x[r][1], x4[r][1]) > fq(x[r][0], x4[r][0])) atomicMin(&s_first, p);
-  }
+  for (int k = 0; k <= kLsPerLane; ++k) fv[k] = fq(x[k], x4[k]);
+  int first = kLsSteps;
+#pragma unroll
+  for (int k = kLsPerLane - 1; k >= 0; --k)
+    if (p0 + k < kLsSteps && fv[k + 1] > fv[k]) first = p0 + k;
+  // Lane ranges increase with the lane, so the lowest lane with a rise
+  // holds the wavefront's first rise.
+  const unsigned long long rise = __ballot(first < kLsSteps);
+  const int wfirst = __shfl(first, rise ? __builtin_ctzll(rise) : 0);
+  if (tid % kWave == 0) s_first[tid / kWave] = rise ? wfirst : kLsSteps;
   __syncthreads();
   if (tid == 0) {
-    const int first = s_first < kLsSteps ? s_first : kLsSteps - 1;
+    int fr = kLsSteps;
+#pragma unroll
+    for (int w = 0; w < kFinishPolyBlock / kWave; ++w) fr = min(fr, s_first[w]);
+    const int first_rise = fr < kLsSteps ? fr : kLsSteps - 1;
     ImageScalars& sc = f.sc[blockIdx.x];
     sc.fenergy = A;
     sc.poly[0] = B;
     sc.poly[1] = C;
     sc.poly[2] = D;
     sc.poly[3] = E;
-    sc.alpha = t.alpha[first];
-    sc.ls_index = first;
+    sc.alpha = t.alpha[first_rise];
+    sc.ls_index = first_rise;
   }
 }
 
