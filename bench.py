@@ -35,8 +35,7 @@ def barrier(e):
     return float(np.max(e - e[0])), float(np.max(e - e[-1]))
 
 
-KERNEL_NAMES = ((0, "grad_update"), (1, "poly"), (2, "grad_first"), (3, "finish_grad"),
-                (4, "finish_poly"))
+KERNEL_NAMES = ((0, "grad_update"), (1, "poly"), (2, "grad_first"), (4, "finish_poly"))
 
 
 def run_path(strings, precision, steps, warmup, profile_iters=3, segment_rows=0, tune=True):
@@ -49,7 +48,7 @@ def run_path(strings, precision, steps, warmup, profile_iters=3, segment_rows=0,
     sv.synchronize()
     # The library's autotuner picks each string class's segment height on
     # the live problem (the state is restored afterwards).
-    rows = sv.autotune(steps=4) if (tune and not segment_rows) else None
+    rows = sv.autotune(steps=10) if (tune and not segment_rows) else None
     # Re-start from the initial guess so that both precisions time the same
     # trajectory; warm-up only compiled graphs and touched memory.
     for i, s in enumerate(strings):

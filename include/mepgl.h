@@ -205,7 +205,8 @@ int mepgl_get_control(mepgl_ctx *ctx, int key, int *value);
  * brackets every launch with events and disables graph replay; read returns
  * the average duration in ms and the launch count of kernel `kernel`:
  * 0 = G with the pending update, 1 = P (direction + quartic),
- * 2 = G without an update, 3 = finishing launch after G (beta),
+ * 2 = G without an update, 3 = reserved (P reduces beta itself, so there
+ * is no finishing launch after G; reads a count of 0),
  * 4 = finishing launch after P (quartic totals + line search). */
 int mepgl_profile(mepgl_ctx *ctx, int enable);
 int mepgl_profile_read(mepgl_ctx *ctx, int kernel, double *avg_ms, int *count);
@@ -217,9 +218,10 @@ int mepgl_synchronize(mepgl_ctx *ctx);
  * size (config.cuh:25-31, blockSizeX); here one wavefront marches a
  * 64-column strip down `rows` rows (heights of 3k + 1 rows waste no work).
  * mepgl_set_segment_rows sets one string's height (clamped to
- * [min(N, 4), N]).  mepgl_autotune times `steps` NLCG steps per candidate
- * height (4 .. 127) for each class of strings with equal N, components and
- * fixed-A setting, keeps the fastest, and restores the solver state, so
+ * [min(N, 4), N]).  mepgl_autotune times three rounds of `steps` NLCG
+ * steps per candidate height (4 .. 127), the fastest round counting, for
+ * each class of strings with equal N, components and fixed-A setting, keeps
+ * the fastest height, and restores the solver state, so
  * tuning leaves fields and scalars as it found them; it needs device memory
  * for one copy of the fields, gradients and directions.  rows_out
  * (n_strings entries) may be NULL.  Sums over cells are taken per strip, so

@@ -252,24 +252,24 @@ FinishArgs finish_args(mepgl_ctx* c) {
   f.sc = c->d_sc;
   f.ls = c->d_ls;
   f.out = c->d_out;
-  f.nlcg = 1;
   f.xin = c->xp;
   return f;
 }
 
-// Timing ids of the finishing launches.
-constexpr int kTimeFinishGrad = 3, kTimeFinishPoly = 4;
+// Timing id of the finishing launch after P (id 3, a finishing launch after
+// G, is no longer made: P reduces beta itself).
+constexpr int kTimeFinishPoly = 4;
 
 int run_finish(mepgl_ctx* c, const FinishArgs& f, int which) {
   if (c->dry) return MEPGL_OK;
-  const bool timed = c->timing.on && (which == kKernelGrad || which == kKernelPoly);
+  const bool timed = c->timing.on && which == kKernelPoly;
   const size_t s0 = c->timing.used;
   hipError_t e = timed ? record_timing_event(c) : hipSuccess;
   if (e == hipSuccess) e = launch_finish(f, which, c->fp32(), c->stream);
   if (timed && e == hipSuccess) e = record_timing_event(c);
   if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, std::string("launch: ") + hipGetErrorString(e));
   if (timed)
-    c->timing.marks.push_back({which == kKernelGrad ? kTimeFinishGrad : kTimeFinishPoly, {s0, s0 + 1}});
+    c->timing.marks.push_back({kTimeFinishPoly, {s0, s0 + 1}});
   return MEPGL_OK;
 }
 
@@ -303,11 +303,6 @@ int nlcg_step(mepgl_ctx* c, int k, int nlcg, int maxwell) {
   a.xout = 1 - c->xp;
   int rc = run_step(c, a, kKernelGrad);
   if (rc) return rc;
-  FinishArgs f = finish_args(c);
-  f.scur = a.scur;
-  f.sold = a.sold;
-  f.nlcg = nlcg;
-  if ((rc = run_finish(c, f, kKernelGrad))) return rc;
   if (c->pending) {
     c->xp = 1 - c->xp;
     c->pending = false;
@@ -316,9 +311,11 @@ int nlcg_step(mepgl_ctx* c, int k, int nlcg, int maxwell) {
   a.xin = c->xp;
   a.din = c->dp;
   a.dout = 1 - c->dp;
+  // P totals the gradient partials into beta itself (there is no finishing
+  // launch after G); the quartic totals and the line search follow P.
   rc = run_step(c, a, kKernelPoly);
   if (rc) return rc;
-  if ((rc = run_finish(c, f, kKernelPoly))) return rc;
+  if ((rc = run_finish(c, finish_args(c), kKernelPoly))) return rc;
   c->dp = 1 - c->dp;
   c->pending = true;
   return MEPGL_OK;
@@ -742,29 +739,38 @@ int restore_snapshot(mepgl_ctx* c, const Snapshot& sn) {
   return MEPGL_OK;
 }
 
-// Milliseconds per NLCG step of the current schedule, from `steps` steps
-// after one untimed step, starting from the snapshot.
+// Milliseconds per NLCG step of the current schedule, starting from the
+// snapshot: one untimed step, then kTuneRounds rounds of `steps` steps, of
+// which the fastest counts (candidate heights often differ by a few percent,
+// about as much as one round's timing varies).
+constexpr int kTuneRounds = 3;
+
 int time_schedule(mepgl_ctx* c, const Snapshot& sn, int steps, double* ms) {
   int rc;
   if ((rc = apply_schedule(c)) || (rc = restore_snapshot(c, sn))) return rc;
   if ((rc = nlcg_step(c, 0, 1, 0))) return rc;
-  hipEvent_t e0, e1;
-  HIP_TRY(c, hipEventCreate(&e0));
-  if (hipEventCreate(&e1) != hipSuccess) {
-    (void)hipEventDestroy(e0);
-    return fail(c, MEPGL_ERR_HIP, "autotune timing: event creation failed");
+  hipEvent_t ev[kTuneRounds + 1];
+  int made = 0;
+  hipError_t e = hipSuccess;
+  for (; made <= kTuneRounds && e == hipSuccess; ++made) e = hipEventCreate(&ev[made]);
+  if (e != hipSuccess) --made;  // the failed creation made no event
+  if (e == hipSuccess) e = hipEventRecord(ev[0], c->stream);
+  int k = 1;
+  for (int r = 1; r <= kTuneRounds && !rc && e == hipSuccess; ++r) {
+    for (int i = 0; i < steps && !rc; ++i, ++k) rc = nlcg_step(c, k, 1, 0);
+    if (!rc) e = hipEventRecord(ev[r], c->stream);
   }
-  hipError_t e = hipEventRecord(e0, c->stream);
-  for (int k = 0; k < steps && !rc && e == hipSuccess; ++k) rc = nlcg_step(c, k + 1, 1, 0);
-  if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
-  if (e == hipSuccess) e = hipEventSynchronize(e1);
-  float t = 0;
-  if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (!rc && e == hipSuccess) e = hipEventSynchronize(ev[kTuneRounds]);
+  float best = 0;
+  for (int r = 1; r <= kTuneRounds && !rc && e == hipSuccess; ++r) {
+    float t = 0;
+    e = hipEventElapsedTime(&t, ev[r - 1], ev[r]);
+    if (r == 1 || t < best) best = t;
+  }
+  for (int i = 0; i < made; ++i) (void)hipEventDestroy(ev[i]);
   if (rc) return rc;
   if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, std::string("autotune timing: ") + hipGetErrorString(e));
-  *ms = (double)t / std::max(steps, 1);
+  *ms = (double)best / std::max(steps, 1);
   return MEPGL_OK;
 }
 

@@ -15,27 +15,6 @@ __device__ __forceinline__ const DevString& string_of_image(const FinishArgs& f,
   return f.strings[si.x];
 }
 
-// After G: |g|^2, g.(g - g_old) and the Polak-Ribiere beta
-// (computeStepDirection, nlcg.cu:972-982).
-template <bool COMP>
-__global__ __launch_bounds__(kBlock) void k_finish_grad(FinishArgs f) {
-  int n;
-  const DevString& s = string_of_image(f, n);
-  double fin[2];
-  reduce_partials<2, COMP>(s.part_g, n, s.tiles_x * s.tiles_y, fin);
-  if (threadIdx.x == 0) {
-    ImageScalars& sc = f.sc[blockIdx.x];
-    sc.g_norm2[f.scur] = fin[0];
-    sc.g_delta_g = fin[1];
-    double beta = 0;
-    if (f.nlcg) {
-      beta = fin[1] / sc.g_norm2[f.sold];
-      beta = fmax(beta, 0.0);
-    }
-    sc.beta = beta;
-  }
-}
-
 // After P: the quartic A..E and the line search (lineSearch).  One
 // 256-thread workgroup per image.  Thread t holds the scan candidates
 // 8t .. 8t + 7 in registers (x and x^4 at p and p + 1) while wavefront 0
@@ -833,12 +812,6 @@ hipError_t launch_finish(const FinishArgs& f, int which, bool fp32, hipStream_t 
   if (f.nimages <= 0) return hipSuccess;
   const dim3 g((unsigned)f.nimages), b(kBlock);
   switch (which) {
-    case kKernelGrad:
-      if (fp32)
-        hipLaunchKernelGGL(k_finish_grad<true>, g, b, 0, st, f);
-      else
-        hipLaunchKernelGGL(k_finish_grad<false>, g, b, 0, st, f);
-      break;
     case kKernelPoly:
       if (fp32)
         hipLaunchKernelGGL(k_finish_poly<true>, g, dim3(kFinishPolyBlock), 0, st, f);

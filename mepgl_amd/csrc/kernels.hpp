@@ -68,7 +68,6 @@ struct FinishArgs {
   ImageScalars* sc;
   const LineSearchTable* ls;
   double* out;
-  int scur, sold, nlcg;
   int xin;           // current field buffer (winding)
 };
 

@@ -36,6 +36,17 @@ __device__ __forceinline__ WorkItem load_item(const WorkItem* items, int k) {
   return u;
 }
 
+// A value that every lane of the wavefront holds, moved to scalar registers.
+__device__ __forceinline__ double uniform_value(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ float uniform_value(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+
 // Pointers read from descriptors are generic; the hot kernels cast them to
 // the global address space so that loads and stores compile to global_*
 // instructions instead of flat_*.

@@ -20,13 +20,14 @@
 //   G  fused pending update x += alpha d (updateFields), gradient
 //      (gradientPsi + gradientA), |g|^2 and g.(g - g_old) partials
 //      (gradientDensity).
-//   P  direction d = -g + beta d (computeStepDirection), evaluated on the
-//      stencil, line-search quartic partials (polynomialExpansion).
-// Each is followed by a finishing launch with one workgroup per image that
-// reduces the partials in a fixed order: beta after G; A..E and the line
-// search (lineSearch, a parallel scan over a host-built table) after P.
-// Reductions are deterministic (fixed partial order, fixed trees), with
-// two-sum compensation in the mixed-precision path.
+//   P  beta from G's partials (every wavefront totals its image's partials
+//      itself), direction d = -g + beta d (computeStepDirection), evaluated
+//      on the stencil, line-search quartic partials (polynomialExpansion).
+// P is followed by a finishing launch with one workgroup per image that
+// totals the quartic partials and runs the line search (lineSearch, a
+// parallel scan over a host-built table).  Reductions are deterministic
+// (fixed partial order, fixed trees), with two-sum compensation in the
+// mixed-precision path.
 #pragma once
 
 #include "row_window.hpp"
@@ -416,8 +417,22 @@ __device__ __forceinline__ void poly_body(const StepArgs& a, const DevString& s,
   b.dout = rsrc_of((const T*)s.d[a.dout] + ib, ibytes);
   b.nn = rsrc_of(s.nn, (long long)s.N * s.pitch * (long long)sizeof(uint16_t));
   b.ps = (unsigned)(s.plane_stride * (long long)sizeof(T));
-  const ImageScalars& sc = a.sc[s.img0 + t.n];
-  const T beta = (T)sc.beta;
+  // The Polak-Ribiere beta (computeStepDirection, nlcg.cu:972-982): every
+  // wavefront totals the gradient kernel's partials of its image in the same
+  // fixed order, so all of them hold the same beta.  The old norm read here
+  // sits in the other slot than the one the image's first tile stores at the
+  // end of the body.
+  ImageScalars* const scp = a.sc + s.img0 + t.n;
+  auto totals = [&](double& gnorm2, double& gdelta, double& beta_d) {
+    double fin[2];
+    reduce_partials<2, COMP>(s.part_g, t.n, s.tiles_x * s.tiles_y, fin);
+    gnorm2 = fin[0];
+    gdelta = fin[1];
+    beta_d = a.nlcg ? fmax(gdelta / scp->g_norm2[a.sold], 0.0) : 0.0;
+  };
+  double gnorm2, gdelta, beta_d;
+  totals(gnorm2, gdelta, beta_d);
+  const T beta = uniform_value((T)beta_d);
   constexpr bool nsf = NSF;
   const bool exact = s.poly_exact;
   const bool store = t.i < N;
@@ -576,6 +591,18 @@ __device__ __forceinline__ void poly_body(const StepArgs& a, const DevString& s,
   }
 
   publish_partial<5, COMP>(acc_s, acc_c, s.part_p, t.n, t.tile, s.tiles_x * s.tiles_y);
+  // The first tile's wavefront recomputes the totals here (same order, same
+  // values) rather than keeping them in registers across the march, and
+  // stores them only now: a store before the march would make the compiler
+  // read the descriptor with vector loads.
+  if (it.tile == 0) {
+    totals(gnorm2, gdelta, beta_d);
+    if (threadIdx.x == 0) {
+      scp->g_norm2[a.scur] = gnorm2;
+      scp->g_delta_g = gdelta;
+      scp->beta = beta_d;
+    }
+  }
 }
 
 template <class T, int NC, bool NSF, bool PL>

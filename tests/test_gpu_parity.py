@@ -110,12 +110,39 @@ def test_nlcg_steps_call_boundaries(case, steps, oracle):
         a, p1, p2 = sv.download(0)
         gn0 = sv.scalars(0, "g_norm2")
         gn1 = sv.scalars(0, "g_norm2_old")
+        gdg = sv.scalars(0, "g_delta_g")
+        beta = sv.scalars(0, "beta")
         g = sv.download_state(0, "g")
     assert normwise(p1, ref.psi_1) < 1e-11
     assert normwise(a, ref.a) < 1e-11
     assert rel_err(gn0, ref.g_norm2[0]) < 1e-10
     assert rel_err(gn1, ref.g_norm2[1]) < 1e-10
     assert normwise(g[1], ref.g_p1[0]) < 1e-10
+    # The last step's g.(g - g_old) and Polak-Ribiere beta: step k of a call
+    # stores |g|^2 in slot 1 when k is even, and beta divides by the other.
+    assert rel_err(gdg, ref.g_delta_g) < 1e-9
+    cur = 1 if (steps - 1) % 2 == 0 else 0
+    want = np.maximum(ref.g_delta_g / ref.g_norm2[1 - cur], 0.0)
+    assert np.any(want > 0)
+    assert rel_err(beta, want) < 1e-9
+
+
+def test_steepest_descent_steps(oracle):
+    """nlcg = 0 (the stdin `C` toggle): beta is 0 on every step, so each
+    direction is -g."""
+    s = small_string(N=20, F=4, seed=9, multicomponent=False)
+    say("[gpu] steepest descent steps")
+    ref = oracle.OracleString(s)
+    ref.nlcg_steps(3, nlcg=False)
+    with solver_for([s]) as sv:
+        sv.nlcgSteps(3, nlcg=False)
+        a, p1, _ = sv.download(0)
+        beta = sv.scalars(0, "beta")
+        gdg = sv.scalars(0, "g_delta_g")
+    np.testing.assert_array_equal(beta, np.zeros(s.F))
+    assert rel_err(gdg, ref.g_delta_g) < 1e-9
+    assert normwise(p1, ref.psi_1) < 1e-11
+    assert normwise(a, ref.a) < 1e-11
 
 
 def test_maxwell_mode(oracle):
