@@ -592,10 +592,22 @@ int arc_length_all(mepgl_ctx* c) {
     if (s.dev.F < 2)
       return fail(c, MEPGL_ERR_ARG, "arclength and reparametrisation need at least 2 images");
   // k_arc_finalize reads the arclength partial totals of every image.
-  int rc = flush(c);
-  if (rc) return rc;
+  int rc;
   AuxArgs a = aux_args(c);
-  if ((rc = run_aux(c, a, kKernelArclen, 0))) return rc;
+  if (c->pending && c->fp32()) {
+    // Mixed precision: the arclength pass applies the pending update itself
+    // and writes the new fields to the other buffer.
+    a.dcur = c->dp;
+    a.upd = 1;
+    if ((rc = run_aux(c, a, kKernelArclen, 0))) return rc;
+    c->xp = 1 - c->xp;
+    c->pending = false;
+    a = aux_args(c);
+  } else {
+    if ((rc = flush(c))) return rc;
+    a = aux_args(c);
+    if ((rc = run_aux(c, a, kKernelArclen, 0))) return rc;
+  }
   if ((rc = run_finish(c, finish_args(c), kKernelArclen))) return rc;
   return run_aux(c, a, kKernelArcFinalize, (int)c->strings.size());
 }
@@ -630,7 +642,9 @@ int redistribute_all(mepgl_ctx* c, int cubic) {
   }
   // NO_SELF_FIELD strings keep the same fixed A in both field buffers: copy
   // the new A (buffer xout) over the old one (buffer xin).
-  if ((rc = run_aux(c, a, kKernelCopyA, 0))) return rc;
+  bool any_nsf = false;
+  for (auto& s : c->strings) any_nsf = any_nsf || s.dev.nsf;
+  if (any_nsf && (rc = run_aux(c, a, kKernelCopyA, 0))) return rc;
   c->xp = 1 - c->xp;
   return MEPGL_OK;
 }

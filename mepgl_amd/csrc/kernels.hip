@@ -225,49 +225,78 @@ __global__ __launch_bounds__(kBlock) void k_jb(AuxArgs a) {
 // through DPP, and neighbours outside the grid clamped to its edge, so each
 // site's values are those of the reference's clamped 5-point gather.
 // Buffers of images n - 1 and n of an arclength work item (image 0 uses
-// itself; it contributes nothing).  They are built by the kernel before it
-// branches on the component count, so that the buffer descriptors stay
-// wave-uniform scalars.
+// itself; it contributes nothing), with the directions and step lengths of
+// a pending update.  They are built by the kernel before it branches on the
+// component count, so that the buffer descriptors stay wave-uniform scalars.
 struct ArcBufs {
-  GBufs b0, b1;
+  GBufs b0, b1;       // b1.xo: the new fields of image n (pending update)
+  double al0, al1;    // alpha of images n - 1 and n
 };
 
 template <class T>
 __device__ __forceinline__ ArcBufs arc_bufs(const AuxArgs& a, const DevString& s, const Strip& t) {
   const long long ibytes = s.image_stride * (long long)sizeof(T);
   const T* X = (const T*)s.x[a.xin];
+  const T* D = (const T*)s.d[a.dcur];
+  const int n0 = t.n > 0 ? t.n - 1 : 0;
   ArcBufs ab;
-  ab.b0.x = rsrc_of(X + (long long)(t.n > 0 ? t.n - 1 : 0) * s.image_stride, ibytes);
+  ab.b0.x = rsrc_of(X + (long long)n0 * s.image_stride, ibytes);
   ab.b1.x = rsrc_of(X + (long long)t.n * s.image_stride, ibytes);
+  ab.b0.d = rsrc_of(D + (long long)n0 * s.image_stride, ibytes);
+  ab.b1.d = rsrc_of(D + (long long)t.n * s.image_stride, ibytes);
+  ab.b1.xo = rsrc_of((const T*)s.x[a.xout] + (long long)t.n * s.image_stride, ibytes);
   ab.b0.nn = ab.b1.nn = rsrc_of(s.nn, (long long)s.N * s.pitch * (long long)sizeof(uint16_t));
   ab.b0.ps = ab.b1.ps = (unsigned)(s.plane_stride * (long long)sizeof(T));
+  ab.al0 = a.sc[s.img0 + n0].alpha;
+  ab.al1 = a.sc[s.img0 + t.n].alpha;
   return ab;
 }
 
 // PL: some parameter is a per-site plane (read per row); otherwise the
 // parameters are read once, so no load outside the row batches drains the
 // loads in flight.
-template <class T, int NC, bool PL>
+//
+// UPD: the fields carry a pending update (updateFields, nlcg.cu:1372-1396).
+// Both images are read with their directions and updated in the window, in
+// the expression of the G kernel and of k_update, the distances are taken
+// between the updated images, and the wavefront stores its rows of the
+// updated image n to the other field buffer: the update and the arclength
+// pass share one sweep over the fields.  As in the G kernel, the fixed A of
+// a NO_SELF_FIELD string takes no update and is not stored.
+template <class T, int NC, bool PL, bool UPD>
 __device__ __forceinline__ void arclen_body(const DevString& s, const Strip& t, const ArcBufs& ab) {
   double acc_s[2] = {0, 0}, acc_c[2] = {0, 0};
+  const int N = s.N, P_ = s.pitch;
+  const bool nsf = s.nsf;
+  const T al0 = UPD ? (T)ab.al0 : (T)0, al1 = UPD ? (T)ab.al1 : (T)0;
   if (t.n > 0) {
-    const int N = s.N, P_ = s.pitch;
     const GBufs& b0 = ab.b0;  // image n - 1
     const GBufs& b1 = ab.b1;  // image n
     const bool lane_in = t.i < N;
     const double idx = s.inv_dx;
     CompConst<double> kconst[NC];
     if constexpr (!PL) load_comp_const<double, NC>(s, 0, kconst);
+    const ParamView<NC> pv = param_view<NC>(s);
     auto nnoff = [&](int row) { return (unsigned)((row * P_ + t.ic) * (int)sizeof(uint16_t)); };
+    // A row's raw loads into window values; the fixed A takes no update.
+    auto process = [&](GRow<T, NC>& w, GRaw<T, NC>& r, T al) {
+      if constexpr (UPD) {
+        r.d[PAX] = nsf ? (T)0 : r.d[PAX];
+        r.d[PAY] = nsf ? (T)0 : r.d[PAY];
+        r.hd[PAX] = nsf ? (T)0 : r.hd[PAX];
+        r.hd[PAY] = nsf ? (T)0 : r.hd[PAY];
+      }
+      g_process<T, NC>(w, r, al, UPD);
+    };
     GRaw<T, NC> r0, r1;
     GRow<T, NC> w0[3], w1[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
       const int row = clampr(t.j0 - 1 + k, N);
-      g_load<T, NC, false, false>(r0, b0, row, P_, t, nnoff(row));
-      g_load<T, NC, false, false>(r1, b1, row, P_, t, nnoff(row));
-      g_process<T, NC>(w0[k], r0, (T)0, false);
-      g_process<T, NC>(w1[k], r1, (T)0, false);
+      g_load<T, NC, UPD, false>(r0, b0, row, P_, t, nnoff(row));
+      g_load<T, NC, UPD, false>(r1, b1, row, P_, t, nnoff(row));
+      process(w0[k], r0, al0);
+      process(w1[k], r1, al1);
     }
     // b and j of one image at row j from its window rows (previous,
     // current, next).
@@ -308,8 +337,8 @@ __device__ __forceinline__ void arclen_body(const DevString& s, const Strip& t, 
       constexpr bool BATCH = decltype(batch_tag)::value;
       if constexpr (BATCH) {
         const int r2 = clampr(j + 2, N);
-        g_load<T, NC, false, false>(r0, b0, r2, P_, t, nnoff(r2));
-        g_load<T, NC, false, false>(r1, b1, r2, P_, t, nnoff(r2));
+        g_load<T, NC, UPD, false>(r0, b0, r2, P_, t, nnoff(r2));
+        g_load<T, NC, UPD, false>(r1, b1, r2, P_, t, nnoff(r2));
       }
       __builtin_amdgcn_sched_barrier(0);
       if (lane_in && j < t.j1) {
@@ -331,7 +360,7 @@ __device__ __forceinline__ void arclen_body(const DevString& s, const Strip& t, 
         }
         CompConst<double> k[NC];
         if constexpr (PL) {
-          load_comp_const<double, NC>(s, (long long)j * P_ + t.i, k);
+          load_comp_planes<double, NC>(pv, (long long)j * P_ + t.i, k);
         } else {
 #pragma unroll
           for (int c = 0; c < NC; ++c) k[c] = kconst[c];
@@ -345,11 +374,23 @@ __device__ __forceinline__ void arclen_body(const DevString& s, const Strip& t, 
           djb += sq(jj[1][c][0] - jj[0][c][0]) + sq(jj[1][c][1] - jj[0][c][1]);
         two_sum_acc(acc_s[0], acc_c[0], d2);
         two_sum_acc(acc_s[1], acc_c[1], djb);
+        if constexpr (UPD) {
+          const unsigned o = (unsigned)((j * P_ + t.ic) * (int)sizeof(T));
+          if (!nsf) {
+            bstore_keep(c1.ax, b1.xo, o, PAX * b1.ps);
+            bstore_keep(c1.ay, b1.xo, o, PAY * b1.ps);
+          }
+#pragma unroll
+          for (int c = 0; c < NC; ++c) {
+            bstore_keep(c1.u[c], b1.xo, o, (PU1 + 2 * c) * b1.ps);
+            bstore_keep(c1.v[c], b1.xo, o, (PV1 + 2 * c) * b1.ps);
+          }
+        }
       }
       __builtin_amdgcn_sched_barrier(0);
       if constexpr (BATCH) {
-        g_process<T, NC>(w0[IP], r0, (T)0, false);
-        g_process<T, NC>(w1[IP], r1, (T)0, false);
+        process(w0[IP], r0, al0);
+        process(w1[IP], r1, al1);
         pin_row<T, NC>(w0[IP]);
         pin_row<T, NC>(w1[IP]);
       }
@@ -366,27 +407,55 @@ __device__ __forceinline__ void arclen_body(const DevString& s, const Strip& t, 
       row(j + 2, S2(), S0(), S1(), std::true_type());
     }
     row(j, S0(), S1(), S2(), std::false_type());
+  } else if constexpr (UPD) {
+    // Image 0 has no distance; its rows only take the update.
+    const GBufs& b1 = ab.b1;
+    if (t.i < N) {
+      const int p0 = nsf ? 2 : 0;
+      for (int j = t.j0; j < t.j1; ++j) {
+        const unsigned o = (unsigned)((j * P_ + t.ic) * (int)sizeof(T));
+#pragma unroll
+        for (int p = 0; p < 2 + 2 * NC; ++p)
+          if (p >= p0)
+            bstore_keep(fma(al1, bload<T>(b1.d, o, p * b1.ps), bload<T>(b1.x, o, p * b1.ps)),
+                        b1.xo, o, p * b1.ps);
+      }
+    }
   }
   publish_partial<2, true>(acc_s, acc_c, s.part_a, t.n, t.tile, s.tiles_x * s.tiles_y);
 }
 
+template <class T, bool UPD>
+__device__ __forceinline__ void arclen_dispatch(const DevString& s, const Strip& t, const ArcBufs& ab) {
+  if (s.nc == 2) {
+    if (s.planes)
+      arclen_body<T, 2, true, UPD>(s, t, ab);
+    else
+      arclen_body<T, 2, false, UPD>(s, t, ab);
+  } else {
+    if (s.planes)
+      arclen_body<T, 1, true, UPD>(s, t, ab);
+    else
+      arclen_body<T, 1, false, UPD>(s, t, ab);
+  }
+}
+
+// The fused update is built for the mixed-precision fields only: in FP64
+// the update's direction loads would push the kernel past 256 VGPRs (one
+// wavefront per SIMD), so FP64 applies the update with k_update first.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_arclen(AuxArgs a) {
   const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
   const DevString& s = a.strings[it.string];
   const Strip t = strip_of(s, it);
   const ArcBufs ab = arc_bufs<T>(a, s, t);
-  if (s.nc == 2) {
-    if (s.planes)
-      arclen_body<T, 2, true>(s, t, ab);
-    else
-      arclen_body<T, 2, false>(s, t, ab);
-  } else {
-    if (s.planes)
-      arclen_body<T, 1, true>(s, t, ab);
-    else
-      arclen_body<T, 1, false>(s, t, ab);
+  if constexpr (sizeof(T) == 4) {
+    if (a.upd) {
+      arclen_dispatch<T, true>(s, t, ab);
+      return;
+    }
   }
+  arclen_dispatch<T, false>(s, t, ab);
 }
 
 // computeArcLength (interpolation.cu:126-140) for both metrics, then the
@@ -804,6 +873,8 @@ static hipError_t launch_aux_impl(const AuxArgs& a, int which, int arg, hipStrea
 }
 
 hipError_t launch_aux(const AuxArgs& a, int which, int arg, bool fp32, hipStream_t st) {
+  // The arclength pass applies a pending update only for mixed-precision fields.
+  if (a.upd && (which != kKernelArclen || !fp32)) return hipErrorInvalidValue;
   return fp32 ? launch_aux_impl<float>(a, which, arg, st)
               : launch_aux_impl<double>(a, which, arg, st);
 }

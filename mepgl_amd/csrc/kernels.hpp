@@ -56,6 +56,8 @@ struct AuxArgs {
   ImageScalars* sc;
   int xin, xout;
   int dcur;             // current direction buffer
+  int upd;              // arclength applies the pending update x += alpha d
+                        // and writes the new fields to buffer xout
   double* out;          // per global image (energy)
   double* const* jb;    // per string JB planes (device array of pointers)
 };

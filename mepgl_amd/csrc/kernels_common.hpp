@@ -260,6 +260,16 @@ __device__ __forceinline__ void bstore(double v, Rsrc r, unsigned vo, unsigned s
 __device__ __forceinline__ void bstore(float v, Rsrc r, unsigned vo, unsigned so) {
   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vo, so, kStoreNT);
 }
+// Stores whose reader comes next and reads little else (the new fields of the
+// arclength pass, read by the redistribution): ordinary stores, so the data
+// stay in the Infinity Cache for that reader.
+__device__ __forceinline__ void bstore_keep(double v, Rsrc r, unsigned vo, unsigned so) {
+  typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, vo, so, 0);
+}
+__device__ __forceinline__ void bstore_keep(float v, Rsrc r, unsigned vo, unsigned so) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), r, vo, so, 0);
+}
 __device__ __forceinline__ uint16_t bload_u16(Rsrc r, unsigned vo) {
   return __builtin_amdgcn_raw_buffer_load_b16(r, vo, 0, 0);
 }

@@ -95,3 +95,36 @@ def test_mixed_tiles_without_bonds(oracle):
     ref = oracle.OracleString(s)
     ref.loop(2, 3, "L")
     assert rel_err(e, ref.free_energy()) < 1e-5
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_mixed_arclength_applies_pending_update(case):
+    """In mixed precision the arclength pass applies a pending update itself
+    and writes the new fields to the other buffer.  The fields, s and s_jb
+    are bit-identical to those of the update applied first (by the download,
+    which flushes it) and the arclength taken after.  N = 70 gives two
+    strips and, at 10 rows, segments of both march directions."""
+    from mepgl_amd.solver import Solver
+
+    s = small_string(N=70, F=5, seed=37, **CASES[case])
+    say(f"[gpu] mixed arclength with a pending update case={case}")
+    with Solver([s], precision="mixed", segment_rows=10) as fused, \
+            Solver([s], precision="mixed", segment_rows=10) as first:
+        fused.nlcgSteps(3)
+        first.nlcgSteps(3)
+        want_fields = first.download(0)
+        (s_f, sjb_f), = fused.arcLength()
+        (s_w, sjb_w), = first.arcLength()
+        got_fields = fused.download(0)
+        # The next steps continue from the same state.
+        fused.nlcgSteps(2)
+        first.nlcgSteps(2)
+        got_after = fused.download(0)
+        want_after = first.download(0)
+    nf = 3 if s.multicomponent else 2  # (a, psi_1[, psi_2])
+    for k in range(nf):
+        np.testing.assert_array_equal(got_fields[k], want_fields[k])
+        np.testing.assert_array_equal(got_after[k], want_after[k])
+    np.testing.assert_array_equal(s_f, s_w)
+    np.testing.assert_array_equal(sjb_f, sjb_w)
+    assert np.all(np.diff(s_f) > 0)
