@@ -128,3 +128,27 @@ def test_mixed_arclength_applies_pending_update(case):
     np.testing.assert_array_equal(s_f, s_w)
     np.testing.assert_array_equal(sjb_f, sjb_w)
     assert np.all(np.diff(s_f) > 0)
+
+
+@pytest.mark.parametrize("mode", ["L", "C"])
+def test_mixed_loop_graph_and_eager(mode, oracle):
+    """Mixed-precision outer iterations (the arclength pass applies the
+    pending update) replayed from a captured graph equal eager launches bit
+    for bit, and agree with the FP64 oracle within the mixed tolerance."""
+    from mepgl_amd.solver import Solver
+
+    s = small_string(N=20, F=6, seed=11, multicomponent=False)
+    say(f"[gpu] mixed loop mode={mode} graph and eager")
+    out, fen = [], []
+    for use_graph in (True, False):
+        with Solver([s], precision="mixed", use_graph=use_graph) as sv:
+            sv.loop(3, 4, mode)
+            out.append(sv.download(0))
+            fen.append(sv.scalars(0, "fenergy"))
+    for k in range(2):
+        np.testing.assert_array_equal(out[0][k], out[1][k])
+    np.testing.assert_array_equal(fen[0], fen[1])
+    ref = oracle.OracleString(s)
+    ref.loop(3, 4, mode)
+    assert rel_err(fen[0], ref.fenergy) < 1e-5
+    assert normwise(out[0][1], ref.psi_1) < 1e-4
