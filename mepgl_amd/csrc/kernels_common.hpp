@@ -100,9 +100,10 @@ __device__ __forceinline__ void wave_reduce(double (&s)[K], double (&c)[K]) {
 }
 
 // Publishes this wavefront's K (sum, compensation) pairs as partial `tile`
-// of image n.  Partials are combined by a separate finishing launch: a
-// cross-wavefront handshake inside the kernel would need an agent-scope
-// release per wavefront, which on a multi-XCD device writes back the L2.
+// of image n.  Partials are combined by a later launch (a finishing launch,
+// or the wavefronts of the next kernel): a cross-wavefront handshake inside
+// the kernel would need an agent-scope release per wavefront, which on a
+// multi-XCD device writes back the L2.
 template <int K, bool COMP>
 __device__ __forceinline__ void publish_partial(double (&s)[K], double (&c)[K], double* part,
                                                 int n, int tile, int ntiles) {
