@@ -19,19 +19,26 @@
  *                          uniform field, constant-phase kinetic term),
  *                          per-term toggles, hand-computed 2x2 grids
  *   or_gradient            central finite differences of or_free_energy at
- *                          interior sites; boundary sites: parity unpinned
- *                          beyond this restatement (reference weights, Q4)
+ *                          every site, boundaries included, with uniform
+ *                          masses (bond weights and curl gates are symmetric);
+ *                          the own-site mass of backward bonds (Q4) by its
+ *                          closed-form difference, in the interior and at a
+ *                          half-weight boundary bond
  *   or_compute_jb          closed forms (B of a linear gauge, j of a uniform
  *                          real order parameter, phase invariance) and the
  *                          identity j = -dF_kin/dA at interior sites
  *   or_poly_expansion      exact quartic (Vandermonde against the energy) in
  *                          corrected mode, and in quirk mode with d_A = 0;
- *                          the Q1/Q2 terms with d_A != 0: parity unpinned
+ *                          Q1 by equality with the exact quartic at charge
+ *                          -1, Q2 by the halved y-bond cubic term (the
+ *                          difference of two Vandermonde fits)
  *   or_line_search         the literal loop against an independent scan,
  *                          closed-form special cases, invariants
  *   or_arc_length(_jb)     affine strings give s_n = n/(F-1)
  *   or_redistribute_*      exactness on linear / cubic data, node
- *                          interpolation, banded solve of the spline system
+ *                          interpolation, banded solve of the spline system;
+ *                          the reference's sigma_{k-1} sign (Q7) by its
+ *                          closed-form offset on quadratic data
  *   or_gauge_align         removes a known constant phase between images;
  *                          energy, s_jb unchanged; no neighbour distance grows
  */

@@ -202,6 +202,37 @@ def test_jb_closed_forms(oracle):
     assert np.max(np.abs(jt)) == 0.0
 
 
+def test_jb_is_the_mean_over_valid_quadrants_on_a_mask(oracle):
+    """computeJBKernel (nlcg.cu:262-280): B is the mean of curl A over the
+    site's quadrants whose four corners lie in the domain, and 0 where none
+    does.  A linear gauge has curl H on every plaquette, so B = H exactly at
+    every site with a valid quadrant, edge sites with one to three of them
+    included; a sum or a fixed divisor of 4 would differ there."""
+    from mepgl_amd import configs
+
+    N, dx, H = 12, 0.25, 0.9
+    dom = np.ones((N, N), np.int32)
+    dom[0:3, 0:4] = 0
+    dom[5, 3:9] = 0
+    dom[8:, 8:] = 0
+    dom[2, 9] = 0
+    s = _uniform(N, 1, u=0.8, h=H, dx=dx, domain=dom)
+    s.a = configs.uniform_gauge(N, dx, 1, H)
+    b, _, _ = oracle.compute_jb(s)
+    d = np.pad(dom, 1)  # outside the grid counts as outside the domain
+    counts = np.zeros((N, N), int)
+    for j in range(N):
+        for i in range(N):
+            if not dom[j, i]:
+                continue
+            for dj, di in ((1, 1), (1, -1), (-1, -1), (-1, 1)):
+                counts[j, i] += int(d[1 + j + dj, 1 + i] and d[1 + j, 1 + i + di] and
+                                    d[1 + j + dj, 1 + i + di])
+    assert np.any((counts > 0) & (counts < 4))
+    np.testing.assert_allclose(b[0][counts > 0], H, rtol=1e-12)
+    assert np.all(b[0][counts == 0] == 0.0)
+
+
 def test_current_is_minus_kinetic_derivative(oracle):
     """j_x(i) = -(1/dx^2) dF_kin/dA_x(i) at interior sites."""
     s = small_string(N=14, F=1, seed=8, multicomponent=True)
@@ -252,3 +283,83 @@ def test_gradient_uses_own_site_mass_on_backward_bonds(oracle):
     du = u[i] - u[i - 1]
     np.testing.assert_allclose(g1[0, j, i, c] - fd, du / m[i] - du / m[i - 1], rtol=1e-5,
                                atol=1e-9)
+
+
+def _boundary_geometries():
+    """(label, string) with grid edges, a disk, concave corners, a one-cell
+    hole and neck, comp != sc, two components and fixed A."""
+    N = 12
+    out = [("box", small_string(N=N, F=1, seed=42)),
+           ("box_2c", small_string(N=N, F=1, seed=43, multicomponent=True)),
+           ("disk_2c_fixed_a", small_string(N=N, F=1, seed=46, multicomponent=True,
+                                            masked=True, no_self_field=True))]
+    s = small_string(N=N, F=1, seed=44)
+    dom = np.ones((N, N), np.int32)
+    dom[0:3, 0:4] = 0
+    dom[5, 3:9] = 0
+    dom[5, 6] = 1
+    dom[8:, 8:] = 0
+    dom[2, 9] = 0
+    s.comp_domain, s.sc_domain = dom, dom.copy()
+    out.append(("irregular", s.validate()))
+    s = small_string(N=N, F=1, seed=45)
+    sc = np.ones((N, N), np.int32)
+    sc[:, :2] = 0
+    comp = np.ones((N, N), np.int32)
+    comp[:3, :] = 0
+    s.sc_domain, s.comp_domain = sc, comp
+    out.append(("comp_ne_sc", s.validate()))
+    return out
+
+
+@pytest.mark.parametrize("label,s", _boundary_geometries(), ids=lambda v: v if isinstance(v, str) else "")
+def test_gradient_is_energy_derivative_at_every_site_with_uniform_masses(oracle, label, s):
+    """Q4 (nlcg.cu:552-615, 853-883).  Bond weights and curl-term gates are
+    read from the site's own neighbour map.  A bond or plaquette has the
+    same presence seen from any of its sites, since sites outside the
+    domain have an empty map.  So the only departure from the exact
+    derivative is the own site's mass on backward bonds.  With uniform
+    masses the gradient is the central difference of the energy at every
+    site, the boundary included."""
+    grads = dict(zip(("a", "psi_1", "psi_2"), oracle.gradient(s)))
+    fields = ["psi_1"] + (["psi_2"] if s.multicomponent else []) + \
+             ([] if s.no_self_field else ["a"])
+    eps = 1e-5
+    for f in fields:
+        g = grads[f]
+        for j in range(s.N):
+            for i in range(s.N):
+                for c in (0, 1):
+                    up, dn = s.copy(), s.copy()
+                    getattr(up, f)[0, j, i, c] += eps
+                    getattr(dn, f)[0, j, i, c] -= eps
+                    fd = (oracle.free_energy(up)[0] - oracle.free_energy(dn)[0]) / (2 * eps)
+                    assert abs(fd - g[0, j, i, c]) <= 1e-8 * max(1.0, abs(g[0, j, i, c])), \
+                        (label, f, j, i, c, fd, g[0, j, i, c])
+
+
+def test_gradient_own_site_mass_at_a_boundary_site(oracle):
+    """At a boundary site whose backward x-bond has weight 1/2 (one of its two
+    quadrant pairs is outside the domain), the analytic gradient differs from
+    the energy derivative by half the mass substitution of that bond."""
+    N = 12
+    s = small_string(N=N, F=1, seed=23, planes=True)
+    s.a[:] = 0.0
+    s.h = 0.0
+    dom = np.ones((N, N), np.int32)
+    dom[:5, :] = 0           # rows 0..4 outside: row 5 is the lower edge
+    s.comp_domain, s.sc_domain = dom, dom.copy()
+    s.validate()
+    _, g1, _ = oracle.gradient(s)
+    j, i, c = 5, 6, 0
+    eps = 1e-6
+    up, dn = s.copy(), s.copy()
+    up.psi_1[0, j, i, c] += eps
+    dn.psi_1[0, j, i, c] -= eps
+    fd = (oracle.free_energy(up)[0] - oracle.free_energy(dn)[0]) / (2 * eps)
+    u = s.psi_1[0, j, :, 0]
+    m = s.m_xx_1[j]
+    du = u[i] - u[i - 1]
+    w = 0.5  # the upper quadrant pair is present, the lower one is not
+    np.testing.assert_allclose(g1[0, j, i, c] - fd, w * (du / m[i] - du / m[i - 1]),
+                               rtol=1e-5, atol=1e-9)

@@ -70,6 +70,68 @@ def test_reference_quartic_differs_when_a_moves(oracle):
     assert np.all(np.abs(q[:, 3] - e[:, 3]) > 1e-12 * np.abs(e[:, 3]))
 
 
+def with_charges(s, q):
+    t = s.copy()
+    t.q_1 = q
+    if t.multicomponent:
+        t.q_2 = q
+    return t.validate()
+
+
+@pytest.mark.parametrize("mc", [False, True])
+def test_reference_x_bond_term_at_charge_minus_one(oracle, mc):
+    """Q1 (nlcg.cu:1162-1165, 1269-1272): the reference's x-bond first-order
+    term is +a_d,x psi where the exact expansion has -q a_d,x psi.  At
+    q = -1 the two coincide, so with the y-bond cubic factor out of play
+    (a_d,y = 0) the reference quartic is the energy polynomial.  The term
+    with the charge kept, or with the exact sign, would differ here."""
+    s = with_charges(small_string(N=12, F=2, multicomponent=mc, seed=35), -1.0)
+    da, d1, d2 = random_direction(s, 5)
+    da[..., 1] = 0.0
+    d = (da, d1, d2)
+    q = oracle.poly_expansion(s, *d, poly_quirks=True)
+    e = oracle.poly_expansion(s, *d, poly_quirks=False)
+    np.testing.assert_allclose(q, e, rtol=1e-12, atol=1e-14 * np.max(np.abs(e)))
+    fit = vandermonde_fit(oracle, s, d)
+    scale = np.max(np.abs(fit), axis=1, keepdims=True)
+    assert np.max(np.abs(q - fit) / scale) < 1e-9
+    # The same direction at the charge +1 does move B (the test has teeth).
+    s1 = with_charges(s, 1.0)
+    q1 = oracle.poly_expansion(s1, *d, poly_quirks=True)
+    e1 = oracle.poly_expansion(s1, *d, poly_quirks=False)
+    assert np.all(np.abs(q1[:, 1] - e1[:, 1]) > 1e-9 * np.abs(e1[:, 1]))
+
+
+@pytest.mark.parametrize("mc", [False, True])
+def test_reference_y_bond_cubic_is_halved(oracle, mc):
+    """Q2 (nlcg.cu:1251, 1358): the reference's y-bond cubic term carries an
+    extra factor 0.5.  With a_d,x = 0 the x-bond terms are exact.  The only
+    part of the exact D that needs a_d,y is the y-bond cubic term, so that
+    term is D(d) - D(d with a_d,y = 0), both from Vandermonde fits of the
+    energy.  The reference's D is then D(d) minus half of it, and A, B, C
+    and E are exact."""
+    s = small_string(N=12, F=2, multicomponent=mc, seed=36)
+    da, d1, d2 = random_direction(s, 6)
+    da[..., 0] = 0.0
+    da[..., 1] *= 10.0  # a y-bond cubic term well above the fits' rounding
+    d = (da, d1, d2)
+    d0 = (np.zeros_like(da), d1, d2)
+    q = oracle.poly_expansion(s, *d, poly_quirks=True)
+    e = oracle.poly_expansion(s, *d, poly_quirks=False)
+    fit = vandermonde_fit(oracle, s, d)
+    fit0 = vandermonde_fit(oracle, s, d0)
+    ybond = fit[:, 3] - fit0[:, 3]
+    scale = np.max(np.abs(fit), axis=1)
+    # The term stands a thousand times above the fits' tolerance, so a wrong
+    # factor on it cannot pass the check below.
+    assert np.all(np.abs(ybond) > 1e-6 * scale)
+    want = fit[:, 3] - 0.5 * ybond
+    assert np.max(np.abs(q[:, 3] - want) / scale) < 1e-9
+    keep = [0, 1, 2, 4]
+    np.testing.assert_allclose(q[:, keep], e[:, keep], rtol=1e-12,
+                               atol=1e-14 * np.max(np.abs(e)))
+
+
 def test_quartic_constant_term_is_the_energy(oracle):
     s = small_string(N=12, F=3, multicomponent=True, masked=True, seed=34)
     d = random_direction(s, 4)
@@ -260,6 +322,31 @@ def test_cubic_redistribution_interpolates_nodes(oracle, textbook):
     f = rng.standard_normal((F, N, N, 2))
     out = oracle.redistribute_cubic(N, F, sn, f, F, textbook=textbook)
     np.testing.assert_allclose(out, f, rtol=1e-12, atol=1e-12)
+
+
+def test_reference_cubic_sigma_sign_on_quadratics(oracle):
+    """Q7 (interpolation.cu:394-395, 399-400): the reference weighs the
+    sigma_{k-1} term by (w-1)^3 - (w-1) where the Forsythe-Malcolm-Moler
+    spline has (1-w)^3 - (1-w).  On data c0 + c2 s^2 the spline's second
+    derivative is 2 c2 at every node (the end conditions reproduce cubics),
+    so the flipped term moves the value at w by exactly
+    -2 (h^2 / 6) (2 c2) ((1-w)^3 - (1-w)), with h the segment length and w
+    the fraction of the segment from node k-1."""
+    N, F, F_new = 3, 9, 14
+    rng = np.random.default_rng(17)
+    sn = np.sort(rng.uniform(0, 1, F))
+    sn[0], sn[-1] = 0.0, 1.0
+    c0 = rng.standard_normal((N, N, 2))
+    c2 = rng.standard_normal((N, N, 2))
+    f = np.stack([c0 + c2 * x ** 2 for x in sn])
+    out = oracle.redistribute_cubic(N, F, sn, f, F_new, textbook=False)
+    for n in range(F_new):
+        x = n / (F_new - 1)
+        k = min(max(int(np.searchsorted(sn, x, side="right")), 1), F - 1)
+        h = sn[k] - sn[k - 1]
+        w = (x - sn[k - 1]) / h
+        want = c0 + c2 * x ** 2 - 2.0 * (h * h / 6.0) * (2.0 * c2) * ((1 - w) ** 3 - (1 - w))
+        np.testing.assert_allclose(out[n], want, rtol=1e-10, atol=1e-10)
 
 
 def test_textbook_cubic_reproduces_cubic_polynomials(oracle):
