@@ -587,7 +587,9 @@ int copy_in_fields(mepgl_ctx* c, int si, void* dev, const double* a, const doubl
   return MEPGL_OK;
 }
 
-int arc_length_all(mepgl_ctx* c) {
+// reset_interior: the finalize launch also resets the interior images'
+// gradient norms, as an outer iteration does after its reparametrisation.
+int arc_length_all(mepgl_ctx* c, int reset_interior = 0) {
   for (auto& s : c->strings)
     if (s.dev.F < 2)
       return fail(c, MEPGL_ERR_ARG, "arclength and reparametrisation need at least 2 images");
@@ -609,6 +611,7 @@ int arc_length_all(mepgl_ctx* c) {
     if ((rc = run_aux(c, a, kKernelArclen, 0))) return rc;
   }
   if ((rc = run_finish(c, finish_args(c), kKernelArclen))) return rc;
+  a.reset_interior = reset_interior;
   return run_aux(c, a, kKernelArcFinalize, (int)c->strings.size());
 }
 
@@ -668,10 +671,10 @@ int outer_iteration(mepgl_ctx* c, int relax, char mode, int nlcg) {
   int rc;
   if (mode == 'L' || mode == 'C') {
     if ((c->flags & MEPGL_GAUGE_ALIGN) && (rc = gauge_align_all(c))) return rc;
-    if ((rc = arc_length_all(c))) return rc;
+    // The interior norms are reset (beta = 0 on the next step) by the
+    // arclength finalize; the redistribution does not read them.
+    if ((rc = arc_length_all(c, 1))) return rc;
     if ((rc = redistribute_all(c, mode == 'C'))) return rc;
-    AuxArgs a = aux_args(c);
-    if ((rc = run_aux(c, a, kKernelResetNorms, 1))) return rc;
   }
   const bool maxwell = mode == 'M';
   for (int k = 0; k < relax; ++k)

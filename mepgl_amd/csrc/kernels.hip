@@ -508,6 +508,13 @@ __global__ __launch_bounds__(kArcBlock) void k_arc_finalize(AuxArgs a, int nstri
     const double seg_start = k == 1 ? 0.0 : lsj[k - 1];
     s.plan_k[n] = k;
     s.plan_w[n] = (s_new - seg_start) / (lsj[k] - lsj[k - 1]);
+    // The reset of beta after a reparametrisation (mepgl.cu:231-234), done
+    // here rather than by a launch of its own: nothing reads the norms
+    // before the next NLCG step.
+    if (a.reset_interior && n != 0 && n != F - 1) {
+      sc[n].g_norm2[0] = INFINITY;
+      sc[n].g_norm2[1] = INFINITY;
+    }
   }
 }
 

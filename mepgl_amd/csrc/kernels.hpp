@@ -58,6 +58,8 @@ struct AuxArgs {
   int dcur;             // current direction buffer
   int upd;              // arclength applies the pending update x += alpha d
                         // and writes the new fields to buffer xout
+  int reset_interior;   // arclength finalize also resets the interior images'
+                        // gradient norms (beta = 0 on the next step)
   double* out;          // per global image (energy)
   double* const* jb;    // per string JB planes (device array of pointers)
 };
