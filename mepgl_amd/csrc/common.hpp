@@ -1,11 +1,17 @@
 // Shared host/device definitions of the MI355X gauged-string-method library.
 //
 // Data layout in HBM (one allocation set per string):
-//   field sets x[2], d[2], g[2]: [image n][plane p][row j][col i], row pitch
-//   `pitch` elements (a multiple of 64), N + kPlanePadRows rows per plane
-//   (the padding row breaks power-of-two strides), planes p = 0 A_x, 1 A_y, 2 u_1,
-//   3 v_1, 4 u_2, 5 v_2 (2 + 2*ncomp planes).  Column i runs along x, so a
-//   64-lane wavefront reads one contiguous row segment.
+//   field sets x[2], d[2], g[2]: [image n][row j][plane p][col i].  A plane
+//   row is N rounded up to a multiple of 64 elements (plane_stride); a grid
+//   row holds the rows of every plane one after another (pitch = nplanes *
+//   plane_stride); an image is N + kPlanePadRows grid rows (the padding row
+//   breaks power-of-two image strides).  Element (n, p, j, i) sits at
+//   n * image_stride + p * plane_stride + j * pitch + i.  Planes p = 0 A_x,
+//   1 A_y, 2 u_1, 3 v_1, 4 u_2, 5 v_2 (2 + 2*ncomp planes).  Column i runs
+//   along x, so a 64-lane wavefront reads one contiguous row segment, and the
+//   planes it reads at one row lie in one contiguous span.
+//   JB output (b, j_1, j_2 in double) and the cubic right-hand sides use the
+//   same layout.
 //   nn: uint16 per site, low byte = superconducting-domain neighbour bits,
 //   high byte = computational-domain neighbour bits (bit k: neighbour k,
 //   3 2 1 / 4 X 0 / 5 6 7).
@@ -24,7 +30,7 @@ constexpr int kLsSteps = 2000;     // line-search scan length (nlcg.cu:1421)
 constexpr int kNumXcd = 8;         // MI355X accelerator complex dies
 constexpr int kMaxImages = 2048;   // images per string (reparametrisation in LDS)
 constexpr int kArcBlock = 256;     // threads of the per-string arclength pass
-constexpr int kPlanePadRows = 1;   // unused rows after each field plane
+constexpr int kPlanePadRows = 1;   // unused grid rows after each image
 
 enum Plane { PAX = 0, PAY = 1, PU1 = 2, PV1 = 3, PU2 = 4, PV2 = 5 };
 
@@ -58,8 +64,8 @@ struct DevString {
   int seg_rows;             // rows per segment (one wavefront marches them)
   int poly_exact;           // 0: reference line-search quartic (Q1, Q2)
   int planes;               // some of a, b, m_xx, m_yy is a per-site plane
-  long long plane_stride;   // elements per plane (rows * pitch)
-  long long image_stride;   // elements per image (nplanes * plane_stride)
+  long long plane_stride;   // elements between planes (one plane row)
+  long long image_stride;   // elements per image ((N + kPlanePadRows) * pitch)
   double dx, inv_dx, dx2;
   double q[2], eta, gamma, delta;
   // Per-component constants of uniform parameters: q, 2a, 2b, b, a/b,
@@ -75,7 +81,7 @@ struct DevString {
   double* part_p;           // [F][ntiles][kPartP]
   double* part_a;           // [F][ntiles][kPartA] arclength partials
   double* cubic;            // [F][4] Thomas factors c', m, sub-diagonal, h
-  void* cubic_rhs;          // cubic right-hand sides [F][plane][rows][pitch]
+  void* cubic_rhs;          // cubic right-hand sides, the fields' layout in double
   int* plan_k;              // [F] linear redistribution: segment end node
   double* plan_w;           // [F] linear redistribution: weight of node k
   int* cplan_k;             // [F] cubic redistribution: segment end node

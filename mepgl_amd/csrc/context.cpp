@@ -843,12 +843,15 @@ int mepgl_create(const mepgl_string_desc* strings, int n_strings, int precision,
     d.tiles_x = (in.N + kTileX - 1) / kTileX;
     d.tiles_y = (in.N + d.seg_rows - 1) / d.seg_rows;
     s.ntiles = d.tiles_x * d.tiles_y;
-    // One padding row per plane: with N a power of two the plane and image
-    // strides would be powers of two, and the dozen streams one wavefront
-    // reads at the same (row, column) of different planes would fall on the
-    // same HBM channels.
-    d.plane_stride = (long long)(in.N + kPlanePadRows) * d.pitch;
-    d.image_stride = (long long)d.nplanes * d.plane_stride;
+    // Planes interleaved by row: row j of an image holds row j of every
+    // plane, one after another (plane stride = one plane row), so the planes
+    // a wavefront reads at one row lie in one contiguous span.  One padding
+    // row per image keeps the image stride off powers of two, so the streams
+    // of neighbouring images do not fall on the same HBM channels.
+    const int plane_row = d.pitch;
+    d.pitch = d.nplanes * plane_row;
+    d.plane_stride = plane_row;
+    d.image_stride = (long long)(in.N + kPlanePadRows) * d.pitch;
     d.dx = in.dx;
     d.inv_dx = 1.0 / in.dx;
     d.dx2 = in.dx * in.dx;
@@ -1072,8 +1075,8 @@ int mepgl_compute_jb(mepgl_ctx* c, int si, double* b, double* j1, double* j2) {
   for (auto& s : c->strings) {
     if (s.jb) continue;
     void* p;
-    if ((rc = dev_alloc(c, &s, &p, (size_t)s.dev.F * (1 + 2 * s.dev.nc) * s.dev.plane_stride *
-                                       sizeof(double))))
+    // b and j in the fields' layout (planes 0, 1 + 2c, 2 + 2c), in double.
+    if ((rc = dev_alloc(c, &s, &p, (size_t)s.dev.F * s.dev.image_stride * sizeof(double))))
       return rc;
     s.jb = (double*)p;
     changed = true;
@@ -1083,14 +1086,13 @@ int mepgl_compute_jb(mepgl_ctx* c, int si, double* b, double* j1, double* j2) {
   if ((rc = run_aux(c, a, kKernelJB, 0))) return rc;
   StringHost& s = c->strings[si];
   const DevString& d = s.dev;
-  const int np = 1 + 2 * d.nc;
-  std::vector<double> h((size_t)d.F * np * d.plane_stride);
+  std::vector<double> h((size_t)d.F * d.image_stride);
   HIP_TRY(c, hipMemcpyAsync(h.data(), s.jb, h.size() * sizeof(double), hipMemcpyDeviceToHost,
                             c->stream));
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   const int N = d.N, P = d.pitch;
   for (int n = 0; n < d.F; ++n) {
-    const double* base = h.data() + (size_t)n * np * d.plane_stride;
+    const double* base = h.data() + (size_t)n * d.image_stride;
     for (int j = 0; j < N; ++j)
       for (int i = 0; i < N; ++i) {
         const size_t o = (size_t)j * P + i, q = (size_t)n * N * N + (size_t)j * N + i;

@@ -186,7 +186,7 @@ __device__ void jb_body(const AuxArgs& a, const DevString& s, const WorkItem& it
   const Strip t = strip_of(s, it);
   if (t.i >= s.N) return;
   const T* X = (const T*)s.x[a.xin] + (long long)t.n * s.image_stride;
-  double* O = a.jb[it.string] + (long long)t.n * (1 + 2 * NC) * s.plane_stride;
+  double* O = a.jb[it.string] + (long long)t.n * s.image_stride;  // the fields' layout
   for (int j = t.j0; j < t.j1; ++j) {
     const long long off = (long long)j * s.pitch + t.i;
     PolyNbhd<double, NC> nb;
@@ -624,14 +624,15 @@ __global__ __launch_bounds__(kBlock) void k_redistribute_cubic(AuxArgs a, int te
   if (t.n != 0 || t.i >= s.N) return;
   const int F = s.F;
   const double* cb = s.cubic;
-  const long long cells = s.plane_stride;
-  double* Dl = (double*)s.cubic_rhs;  // [F][nplanes][cells]
+  double* Dl = (double*)s.cubic_rhs;  // in the fields' layout, in double
   for (int j = t.j0; j < t.j1; ++j) {
     const long long off = (long long)j * s.pitch + t.i;
     for (int p = 0; p < s.nplanes; ++p) {
       const T* Xp = (const T*)s.x[a.xin] + p * s.plane_stride + off;
       auto f = [&](int n) { return (double)Xp[(long long)n * s.image_stride]; };
-      auto D = [&](int n) -> double& { return Dl[((long long)n * s.nplanes + p) * cells + off]; };
+      auto D = [&](int n) -> double& {
+        return Dl[(long long)n * s.image_stride + p * s.plane_stride + off];
+      };
       auto h = [&](int n) { return cb[4 * n + 3]; };
       for (int n = 1; n < F; ++n) D(n) = (f(n) - f(n - 1)) / h(n);
       const double d3s =
