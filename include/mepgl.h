@@ -218,12 +218,15 @@ int mepgl_synchronize(mepgl_ctx *ctx);
  * size (config.cuh:25-31, blockSizeX); here one wavefront marches a
  * 64-column strip down `rows` rows (heights of 3k + 1 rows waste no work).
  * mepgl_set_segment_rows sets one string's height (clamped to
- * [min(N, 4), N]).  mepgl_autotune times three rounds of `steps` NLCG
- * steps per candidate height (4 .. 127), the fastest round counting, for
- * each class of strings with equal N, components and fixed-A setting, keeps
- * the fastest height, and restores the solver state, so
- * tuning leaves fields and scalars as it found them; it needs device memory
- * for one copy of the fields, gradients and directions.  rows_out
+ * [min(N, 4), N]).  mepgl_autotune times three rounds per candidate height
+ * (4 .. 127) of an outer iteration of mode L with `steps` NLCG steps
+ * (arclength, linear redistribution, the steps; only the steps when a string
+ * has one image), each round visiting every candidate in turn, for each
+ * class of strings with equal N, components and fixed-A setting.  It keeps
+ * the height with the fastest round and restores the solver state, so
+ * tuning leaves fields, scalars and reparametrisation plans as it found
+ * them; it needs device memory for one copy of the fields, gradients and
+ * directions.  rows_out
  * (n_strings entries) may be NULL.  Sums over cells are taken per strip, so
  * a height change can move results in the last bits (well inside 1e-12
  * relative). */

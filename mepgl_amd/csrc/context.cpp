@@ -131,7 +131,7 @@ size_t max_tiles(const DevString& d) {
 // Segment heights the library picks from.  The G and P kernels march rows
 // in rotations of three after the first, so heights of 3k + 1 rows waste no
 // work (any other height pads its last rotation).
-constexpr int kRowChoices[] = {4, 7, 10, 13, 16, 22, 31, 64, 127};
+constexpr int kRowChoices[] = {4, 7, 10, 13, 16, 19, 22, 25, 31, 64, 127};
 
 // Rows marched by one wavefront.  Taller segments re-read fewer halo rows;
 // shorter ones give more wavefronts per launch.  Pick the tallest segment
@@ -734,6 +734,12 @@ int take_snapshot(mepgl_ctx* c, Snapshot& sn) {
     for (int b = 0; b < 2; ++b)
       if ((rc = save(s.dev.x[b], fs)) || (rc = save(s.dev.d[b], fs)) || (rc = save(s.dev.g[b], fs)))
         return rc;
+    // The reparametrisation plans and spline factors: timed outer iterations
+    // recompute them, and a later redistribution must use the caller's.
+    const size_t F = (size_t)s.dev.F;
+    if ((rc = save(s.dev.plan_w, F * (2 * sizeof(int) + 2 * sizeof(double)))) ||
+        (rc = save(s.dev.cubic, F * 4 * sizeof(double))))
+      return rc;
   }
   const size_t scb = sizeof(ImageScalars) * c->total_images;
   if (hipMalloc(&sn.sc, scb) != hipSuccess) return fail(c, MEPGL_ERR_NOMEM, "autotune scalars");
@@ -756,38 +762,44 @@ int restore_snapshot(mepgl_ctx* c, const Snapshot& sn) {
   return MEPGL_OK;
 }
 
-// Milliseconds per NLCG step of the current schedule, starting from the
-// snapshot: one untimed step, then kTuneRounds rounds of `steps` steps, of
-// which the fastest counts (candidate heights often differ by a few percent,
-// about as much as one round's timing varies).
+// Timing rounds per candidate height.
 constexpr int kTuneRounds = 3;
 
+// Milliseconds per NLCG step of the current schedule, starting from the
+// snapshot: one untimed step, then an outer iteration of mode L with `steps`
+// steps (arclength, linear redistribution, the steps) when every string has
+// two images or more, else `steps` steps.  The reparametrisation kernels run
+// on the same work items as the steps, so a height is timed on what the
+// relaxation loop does: with very tall segments the steps alone can time
+// well while the arclength and redistribution launches have too few work
+// items.
 int time_schedule(mepgl_ctx* c, const Snapshot& sn, int steps, double* ms) {
   int rc;
   if ((rc = apply_schedule(c)) || (rc = restore_snapshot(c, sn))) return rc;
   if ((rc = nlcg_step(c, 0, 1, 0))) return rc;
-  hipEvent_t ev[kTuneRounds + 1];
-  int made = 0;
-  hipError_t e = hipSuccess;
-  for (; made <= kTuneRounds && e == hipSuccess; ++made) e = hipEventCreate(&ev[made]);
-  if (e != hipSuccess) --made;  // the failed creation made no event
-  if (e == hipSuccess) e = hipEventRecord(ev[0], c->stream);
-  int k = 1;
-  for (int r = 1; r <= kTuneRounds && !rc && e == hipSuccess; ++r) {
-    for (int i = 0; i < steps && !rc; ++i, ++k) rc = nlcg_step(c, k, 1, 0);
-    if (!rc) e = hipEventRecord(ev[r], c->stream);
+  bool reparam = true;
+  for (auto& s : c->strings) reparam = reparam && s.dev.F >= 2;
+  hipEvent_t e0, e1;
+  HIP_TRY(c, hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(c, MEPGL_ERR_HIP, "autotune timing: event creation failed");
   }
-  if (!rc && e == hipSuccess) e = hipEventSynchronize(ev[kTuneRounds]);
-  float best = 0;
-  for (int r = 1; r <= kTuneRounds && !rc && e == hipSuccess; ++r) {
-    float t = 0;
-    e = hipEventElapsedTime(&t, ev[r - 1], ev[r]);
-    if (r == 1 || t < best) best = t;
+  hipError_t e = hipEventRecord(e0, c->stream);
+  if (e == hipSuccess && reparam) {
+    rc = outer_iteration(c, steps, 'L', 1);
+  } else {
+    for (int k = 0; k < steps && !rc && e == hipSuccess; ++k) rc = nlcg_step(c, k + 1, 1, 0);
   }
-  for (int i = 0; i < made; ++i) (void)hipEventDestroy(ev[i]);
+  if (!rc && e == hipSuccess) e = hipEventRecord(e1, c->stream);
+  if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
+  float t = 0;
+  if (!rc && e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
   if (rc) return rc;
   if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, std::string("autotune timing: ") + hipGetErrorString(e));
-  *ms = (double)best / std::max(steps, 1);
+  *ms = (double)t / std::max(steps, 1);
   return MEPGL_OK;
 }
 
@@ -1445,26 +1457,29 @@ int mepgl_autotune(mepgl_ctx* c, int steps, int* rows_out) {
   }
   const bool timing = c->timing.on;
   c->timing.on = false;
-  double best_ms = 0;
-  if ((rc = time_schedule(c, sn, steps, &best_ms)) == MEPGL_OK) {
-    for (auto& cl : order) {
-      const int N = c->strings[cl.second[0]].dev.N;
-      const int start = c->strings[cl.second[0]].dev.seg_rows;
-      int best = start;
-      for (int h : kRowChoices) {
-        const int r = clamp_rows(h, N);
-        if (r == start || (h > N && h != kRowChoices[0])) continue;
-        for (int si : cl.second) c->strings[si].dev.seg_rows = r;
+  // Candidate heights are timed in rounds that visit every candidate once,
+  // and each keeps its fastest round: neighbouring heights often differ by a
+  // few percent, about as much as one timing varies, and a drift of the
+  // clock over the tuning run then favours no candidate.
+  for (auto& cl : order) {
+    const int N = c->strings[cl.second[0]].dev.N;
+    std::vector<int> cand{c->strings[cl.second[0]].dev.seg_rows};
+    for (int h : kRowChoices) {
+      const int r = clamp_rows(h, N);
+      if (h > N && h != kRowChoices[0]) continue;
+      if (std::find(cand.begin(), cand.end(), r) == cand.end()) cand.push_back(r);
+    }
+    std::vector<double> best(cand.size(), INFINITY);
+    for (int round = 0; round < kTuneRounds && !rc; ++round)
+      for (size_t k = 0; k < cand.size() && !rc; ++k) {
+        for (int si : cl.second) c->strings[si].dev.seg_rows = cand[k];
         double ms = 0;
         if ((rc = time_schedule(c, sn, steps, &ms))) break;
-        if (ms < best_ms) {
-          best_ms = ms;
-          best = r;
-        }
+        best[k] = std::min(best[k], ms);
       }
-      for (int si : cl.second) c->strings[si].dev.seg_rows = best;
-      if (rc) break;
-    }
+    if (rc) break;
+    const size_t pick = (size_t)(std::min_element(best.begin(), best.end()) - best.begin());
+    for (int si : cl.second) c->strings[si].dev.seg_rows = cand[pick];
   }
   c->timing.on = timing;
   int rc2 = apply_schedule(c);

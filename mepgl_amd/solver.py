@@ -287,9 +287,10 @@ class Solver:
         self._check(self._L.mepgl_set_segment_rows(self._ctx, i, int(rows)))
 
     def autotune(self, steps: int = 5):
-        """Pick each string class's segment height by timing three rounds of
-        `steps` NLCG steps per candidate (the fastest round counts); the
-        solver state is left unchanged."""
+        """Pick each string class's segment height by timing three rounds per
+        candidate of an outer iteration of mode L with `steps` NLCG steps
+        (each round visits every candidate in turn; the fastest round counts);
+        the solver state is left unchanged."""
         rows = (C.c_int * len(self.strings))()
         self._check(self._L.mepgl_autotune(self._ctx, int(steps), rows))
         return list(rows)

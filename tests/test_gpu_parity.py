@@ -340,6 +340,24 @@ def test_segment_rows_do_not_change_results(rows, oracle):
     assert rel_err(fen, ref.fenergy) < 1e-12
 
 
+def test_autotune_keeps_the_reparametrisation_plan():
+    """The autotuner times outer iterations, which recompute the arclength
+    plan; it restores the caller's plan, so a redistribution requested after
+    tuning equals one without tuning (redistribution is per element)."""
+    s = small_string(N=20, F=5, seed=49, multicomponent=False)
+    say("[gpu] autotune keeps the reparametrisation plan")
+    with solver_for([s]) as tuned, solver_for([s]) as plain:
+        for sv in (tuned, plain):
+            sv.nlcgSteps(3)
+            sv.arcLength()
+        tuned.autotune(steps=2)
+        for sv in (tuned, plain):
+            sv.redistributeLinear()
+        a, b = tuned.download(0), plain.download(0)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+
+
 def test_autotune_leaves_state_unchanged(oracle):
     strings = [small_string(N=20, F=4, seed=47, multicomponent=False),
                small_string(N=36, F=5, seed=48, multicomponent=True)]
