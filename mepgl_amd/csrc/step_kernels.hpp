@@ -613,7 +613,27 @@ __global__ __launch_bounds__(kBlock) void k_poly(StepArgs a) {
 
 // ---------------------------------------------------------------------------
 // Pending update applied in place (updateFields, nlcg.cu:1372-1396), used
-// before anything other than the next gradient reads the fields.
+// before anything other than the next gradient reads the fields.  The planes
+// of a row are loaded together (NP known at compile time), and the strides
+// are read before the first store, so they stay in scalar registers.
+template <class T, int NP>
+__device__ __forceinline__ void update_rows(T* X, const T* D, T al, bool nsf, long long pitch,
+                                            long long ps, const Strip& t) {
+  for (int j = t.j0; j < t.j1; ++j) {
+    const long long off = (long long)j * pitch + t.i;
+    T xv[NP], dv[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const bool own = !(nsf && p < 2);  // the fixed A takes no update
+      xv[p] = own ? X[p * ps + off] : (T)0;
+      dv[p] = own ? D[p * ps + off] : (T)0;
+    }
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+      if (!(nsf && p < 2)) X[p * ps + off] = fma(al, dv[p], xv[p]);
+  }
+}
+
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_update(StepArgs a) {
   const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
@@ -624,14 +644,12 @@ __global__ __launch_bounds__(kBlock) void k_update(StepArgs a) {
   T* X = (T*)s.x[a.xin] + ib;
   const T* D = (const T*)s.d[a.din] + ib;
   const T al = (T)a.sc[s.img0 + t.n].alpha;
-  const int p0 = s.nsf ? 2 : 0;
-  for (int j = t.j0; j < t.j1; ++j) {
-    const long long off = (long long)j * s.pitch + t.i;
-    for (int p = p0; p < s.nplanes; ++p) {
-      const long long o = p * s.plane_stride + off;
-      X[o] = fma(al, D[o], X[o]);
-    }
-  }
+  const bool nsf = s.nsf;
+  const long long pitch = s.pitch, ps = s.plane_stride;
+  if (s.nplanes == 6)
+    update_rows<T, 6>(X, D, al, nsf, pitch, ps, t);
+  else
+    update_rows<T, 4>(X, D, al, nsf, pitch, ps, t);
 }
 
 // ---------------------------------------------------------------------------
