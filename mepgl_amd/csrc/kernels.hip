@@ -608,55 +608,62 @@ __global__ __launch_bounds__(kBlock) void k_copy_a(AuxArgs a) {
   }
 }
 
-// redistributeCubic (interpolation.cu:292-481) in double, one thread per
-// site: for each plane, divided differences and end third differences
-// (computeDeltaX/Y), the Thomas solve of the spline system along the image
-// axis (in place of cusparseSgtsv2), and the spline evaluation
+// redistributeCubic (interpolation.cu:292-481) in double, one spline system
+// per (site, plane): divided differences and end third differences
+// (computeDeltaX/Y), the Thomas solve of the system along the image axis (in
+// place of cusparseSgtsv2), and the spline evaluation
 // (generateNewFramesCubic).  textbook = 0 keeps the reference's
 // (w-1)^3-(w-1) weight on sigma_{k-1}; 1 uses (1-w)^3-(1-w).
+//
+// The systems run along the image axis, so the F work items of a tile share
+// its (row, plane) pairs: item n takes pairs n, n + F, ..., and a thread
+// solves about rows * planes / F systems.  Every system's arithmetic and its
+// order are those of a sequential solve.
 template <class T>
 __global__ __launch_bounds__(kBlock) void k_redistribute_cubic(AuxArgs a, int textbook) {
   const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
   const DevString& s = a.strings[it.string];
   const Strip t = strip_of(s, it);
-  // The solve runs along the image axis: only the image-0 items of the
-  // shared work list take part, one thread per site.
-  if (t.n != 0 || t.i >= s.N) return;
-  const int F = s.F;
+  if (t.i >= s.N) return;
+  // Everything read from the descriptor is read before the first store.
+  const int F = s.F, NP = s.nplanes;
+  const long long IS = s.image_stride, PS = s.plane_stride, PT = s.pitch;
   const double* cb = s.cubic;
+  const int* ck = s.cplan_k;
+  const double* cw = s.cplan_w;
+  const T* X = (const T*)s.x[a.xin];
+  T* Y = (T*)s.x[a.xout];
   double* Dl = (double*)s.cubic_rhs;  // in the fields' layout, in double
-  for (int j = t.j0; j < t.j1; ++j) {
-    const long long off = (long long)j * s.pitch + t.i;
-    for (int p = 0; p < s.nplanes; ++p) {
-      const T* Xp = (const T*)s.x[a.xin] + p * s.plane_stride + off;
-      auto f = [&](int n) { return (double)Xp[(long long)n * s.image_stride]; };
-      auto D = [&](int n) -> double& {
-        return Dl[(long long)n * s.image_stride + p * s.plane_stride + off];
-      };
-      auto h = [&](int n) { return cb[4 * n + 3]; };
-      for (int n = 1; n < F; ++n) D(n) = (f(n) - f(n - 1)) / h(n);
-      const double d3s =
-          ((D(3) - D(2)) / (h(3) + h(2)) - (D(2) - D(1)) / (h(2) + h(1))) / (h(3) + h(2) + h(1));
-      const double d3e = ((D(F - 1) - D(F - 2)) / (h(F - 1) + h(F - 2)) -
-                          (D(F - 2) - D(F - 3)) / (h(F - 2) + h(F - 3))) /
-                         (h(F - 1) + h(F - 2) + h(F - 3));
-      for (int n = 1; n < F - 1; ++n) D(n) = D(n + 1) - D(n);
-      D(0) = h(1) * h(1) * d3s;
-      D(F - 1) = -(h(F - 1) * h(F - 1)) * d3e;
-      // Thomas forward sweep and back substitution.
-      D(0) = D(0) / cb[1];
-      for (int n = 1; n < F; ++n) D(n) = (D(n) - cb[4 * n + 2] * D(n - 1)) / cb[4 * n + 1];
-      for (int n = F - 2; n >= 0; --n) D(n) = D(n) - cb[4 * n + 0] * D(n + 1);
-      T* O = (T*)s.x[a.xout] + p * s.plane_stride + off;
-      for (int n = 0; n < F; ++n) {
-        const int k = s.cplan_k[n];
-        const double w = s.cplan_w[n];
-        const double wm = textbook ? ((1 - w) * (1 - w) * (1 - w) - (1 - w))
-                                   : ((w - 1) * (w - 1) * (w - 1) - (w - 1));
-        const double v = w * f(k) + (1 - w) * f(k - 1) +
-                         h(k) * h(k) * ((w * w * w - w) * D(k) + wm * D(k - 1));
-        O[(long long)n * s.image_stride] = (T)v;
-      }
+  const int pairs = (t.j1 - t.j0) * NP;
+  for (int q = t.n; q < pairs; q += F) {
+    const int j = t.j0 + q / NP, p = q % NP;
+    const long long off = p * PS + (long long)j * PT + t.i;
+    const T* Xp = X + off;
+    auto f = [&](int n) { return (double)Xp[(long long)n * IS]; };
+    auto D = [&](int n) -> double& { return Dl[(long long)n * IS + off]; };
+    auto h = [&](int n) { return cb[4 * n + 3]; };
+    for (int n = 1; n < F; ++n) D(n) = (f(n) - f(n - 1)) / h(n);
+    const double d3s =
+        ((D(3) - D(2)) / (h(3) + h(2)) - (D(2) - D(1)) / (h(2) + h(1))) / (h(3) + h(2) + h(1));
+    const double d3e = ((D(F - 1) - D(F - 2)) / (h(F - 1) + h(F - 2)) -
+                        (D(F - 2) - D(F - 3)) / (h(F - 2) + h(F - 3))) /
+                       (h(F - 1) + h(F - 2) + h(F - 3));
+    for (int n = 1; n < F - 1; ++n) D(n) = D(n + 1) - D(n);
+    D(0) = h(1) * h(1) * d3s;
+    D(F - 1) = -(h(F - 1) * h(F - 1)) * d3e;
+    // Thomas forward sweep and back substitution.
+    D(0) = D(0) / cb[1];
+    for (int n = 1; n < F; ++n) D(n) = (D(n) - cb[4 * n + 2] * D(n - 1)) / cb[4 * n + 1];
+    for (int n = F - 2; n >= 0; --n) D(n) = D(n) - cb[4 * n + 0] * D(n + 1);
+    T* O = Y + off;
+    for (int n = 0; n < F; ++n) {
+      const int k = ck[n];
+      const double w = cw[n];
+      const double wm = textbook ? ((1 - w) * (1 - w) * (1 - w) - (1 - w))
+                                 : ((w - 1) * (w - 1) * (w - 1) - (w - 1));
+      const double v = w * f(k) + (1 - w) * f(k - 1) +
+                       h(k) * h(k) * ((w * w * w - w) * D(k) + wm * D(k - 1));
+      O[(long long)n * IS] = (T)v;
     }
   }
 }
