@@ -642,19 +642,38 @@ __global__ __launch_bounds__(kBlock) void k_redistribute_cubic(AuxArgs a, int te
     auto f = [&](int n) { return (double)Xp[(long long)n * IS]; };
     auto D = [&](int n) -> double& { return Dl[(long long)n * IS + off]; };
     auto h = [&](int n) { return cb[4 * n + 3]; };
-    for (int n = 1; n < F; ++n) D(n) = (f(n) - f(n - 1)) / h(n);
-    const double d3s =
-        ((D(3) - D(2)) / (h(3) + h(2)) - (D(2) - D(1)) / (h(2) + h(1))) / (h(3) + h(2) + h(1));
-    const double d3e = ((D(F - 1) - D(F - 2)) / (h(F - 1) + h(F - 2)) -
-                        (D(F - 2) - D(F - 3)) / (h(F - 2) + h(F - 3))) /
-                       (h(F - 1) + h(F - 2) + h(F - 3));
-    for (int n = 1; n < F - 1; ++n) D(n) = D(n + 1) - D(n);
-    D(0) = h(1) * h(1) * d3s;
-    D(F - 1) = -(h(F - 1) * h(F - 1)) * d3e;
-    // Thomas forward sweep and back substitution.
-    D(0) = D(0) / cb[1];
-    for (int n = 1; n < F; ++n) D(n) = (D(n) - cb[4 * n + 2] * D(n - 1)) / cb[4 * n + 1];
-    for (int n = F - 2; n >= 0; --n) D(n) = D(n) - cb[4 * n + 0] * D(n + 1);
+    // First differences dl(n) = (f(n) - f(n-1)) / h(n); the end conditions
+    // need only the first and last three.
+    auto dl = [&](int n) { return (f(n) - f(n - 1)) / h(n); };
+    const double d1 = dl(1), d2 = dl(2), d3 = dl(3);
+    const double e1 = dl(F - 1), e2 = dl(F - 2), e3 = dl(F - 3);
+    const double d3s = ((d3 - d2) / (h(3) + h(2)) - (d2 - d1) / (h(2) + h(1))) / (h(3) + h(2) + h(1));
+    const double d3e =
+        ((e1 - e2) / (h(F - 1) + h(F - 2)) - (e2 - e3) / (h(F - 2) + h(F - 3))) /
+        (h(F - 1) + h(F - 2) + h(F - 3));
+    // One pass: the right-hand side dl(n+1) - dl(n) formed on the fly, and
+    // the Thomas forward sweep; then the back substitution.
+    double prev = h(1) * h(1) * d3s;
+    prev = prev / cb[1];
+    D(0) = prev;
+    double dprev = d1;
+    for (int n = 1; n < F - 1; ++n) {
+      const double dnext = dl(n + 1);
+      const double rhs = dnext - dprev;
+      prev = (rhs - cb[4 * n + 2] * prev) / cb[4 * n + 1];
+      D(n) = prev;
+      dprev = dnext;
+    }
+    {
+      const double rhs = -(h(F - 1) * h(F - 1)) * d3e;
+      prev = (rhs - cb[4 * (F - 1) + 2] * prev) / cb[4 * (F - 1) + 1];
+      D(F - 1) = prev;
+    }
+    double next = prev;
+    for (int n = F - 2; n >= 0; --n) {
+      next = D(n) - cb[4 * n + 0] * next;
+      D(n) = next;
+    }
     T* O = Y + off;
     for (int n = 0; n < F; ++n) {
       const int k = ck[n];
