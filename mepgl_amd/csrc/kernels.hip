@@ -726,14 +726,28 @@ __global__ __launch_bounds__(kBlock) void k_align(AuxArgs a) {
     align_body<T, 1>(a, s, it);
 }
 
-// Theta_n = sum_{m <= n} arg(z_m), one thread per string.
-__global__ void k_align_finalize(AuxArgs a, int nstrings) {
-  const int si = blockIdx.x * blockDim.x + threadIdx.x;
+// Theta_n = sum_{m <= n} arg(z_m), one workgroup per string: the angles in
+// parallel, then the running sum in image order by one thread, from LDS.
+__global__ __launch_bounds__(kArcBlock) void k_align_finalize(AuxArgs a, int nstrings) {
+  __shared__ double th[kMaxImages];
+  const int si = blockIdx.x;
   if (si >= nstrings) return;
   const DevString& s = a.strings[si];
   ImageScalars* sc = a.sc + s.img0;
-  sc[0].theta = 0;
-  for (int n = 1; n < s.F; ++n) sc[n].theta = sc[n - 1].theta + atan2(sc[n].align_im, sc[n].align_re);
+  const int F = s.F;
+  if (F > kMaxImages) return;
+  for (int n = threadIdx.x; n < F; n += kArcBlock)
+    th[n] = n == 0 ? 0.0 : atan2(sc[n].align_im, sc[n].align_re);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double run = 0.0;
+    for (int n = 1; n < F; ++n) {
+      run = run + th[n];
+      th[n] = run;
+    }
+  }
+  __syncthreads();
+  for (int n = threadIdx.x; n < F; n += kArcBlock) sc[n].theta = th[n];
 }
 
 // Rotates psi of image n by theta_n in the fields, the direction and both
@@ -895,7 +909,7 @@ static hipError_t launch_aux_impl(const AuxArgs& a, int which, int arg, hipStrea
       if (a.nitems > 0) hipLaunchKernelGGL(k_align<T>, g, b, 0, st, a);
       break;
     case kKernelAlignFinalize:
-      hipLaunchKernelGGL(k_align_finalize, dim3((arg + 63) / 64), dim3(64), 0, st, a, arg);
+      hipLaunchKernelGGL(k_align_finalize, dim3(arg), dim3(kArcBlock), 0, st, a, arg);
       break;
     case kKernelAlignApply:
       if (a.nitems > 0) hipLaunchKernelGGL(k_align_apply<T>, g, b, 0, st, a);
