@@ -8,7 +8,14 @@ A "step" is one NLCG iteration over all images; the timed region runs
 resident in HBM.  `value` is the mixed-precision path's throughput; the FP64
 path is timed on identical inputs and reported beside it.
 
+Further workloads (--legs, default two_band_512, sweep_48 and
+thin_film_disk_1024) run after it for min(K, 50) steps each and are
+reported under "legs": both precisions, the dominant kernel's roofline
+(mask-active cells only for the thin film), and for the 48-string sweep the
+batched context against one context per string in both precisions.
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+                    [--legs a,b,c] [--leg-steps K2]
 """
 from __future__ import annotations
 
@@ -38,7 +45,8 @@ def barrier(e):
 KERNEL_NAMES = ((0, "grad_update"), (1, "poly"), (2, "grad_first"), (4, "finish_poly"))
 
 
-def run_path(strings, precision, steps, warmup, profile_iters=3, segment_rows=0, tune=True):
+def run_path(strings, precision, steps, warmup, profile_iters=3, segment_rows=0, tune=True,
+             tune_steps=10):
     from mepgl_amd.solver import Solver
 
     sv = Solver(strings, precision=precision, segment_rows=segment_rows)
@@ -48,7 +56,7 @@ def run_path(strings, precision, steps, warmup, profile_iters=3, segment_rows=0,
     sv.synchronize()
     # The library's autotuner picks each string class's segment height on
     # the live problem (the state is restored afterwards).
-    rows = sv.autotune(steps=10) if (tune and not segment_rows) else None
+    rows = sv.autotune(steps=tune_steps) if (tune and not segment_rows) else None
     # Re-start from the initial guess so that both precisions time the same
     # trajectory; warm-up only compiled graphs and touched memory.
     for i, s in enumerate(strings):
@@ -72,12 +80,14 @@ def run_path(strings, precision, steps, warmup, profile_iters=3, segment_rows=0,
         print(f"# {precision} {name}: {1e3 * ms:.2f} us x {cnt}", flush=True)
     sv.profile(False)
     info = sv.info(0)
-    info["rows"] = rows
+    info["rows"] = rows if rows is not None else [sv.info(i)["tile_rows"]
+                                                  for i in range(len(strings))]
     sv.close()
-    return dict(seconds=dt, steps=iters * R, energies=energies, times=times, info=info)
+    return dict(seconds=dt, steps=iters * R, energies=energies, times=times, info=info,
+                profile_steps=profile_iters * R)
 
 
-def run_separately(strings, precision, steps, warmup):
+def run_separately(strings, precision, steps, warmup, tune_steps=2):
     """The same strings, one context (and so one launch sequence) each, run
     one after another: the unbatched reference point for a sweep."""
     from mepgl_amd.solver import Solver
@@ -87,7 +97,7 @@ def run_separately(strings, precision, steps, warmup):
     for s in strings:
         sv = Solver([s], precision=precision)
         sv.loop(max(warmup // R, 1), R, "L")
-        sv.autotune(steps=2)
+        sv.autotune(steps=tune_steps)
         sv.upload(0, s)
         sv.reset_solver()
         sv.synchronize()
@@ -97,6 +107,108 @@ def run_separately(strings, precision, steps, warmup):
         total += time.perf_counter() - t0
         sv.close()
     return dict(seconds=total, steps=iters * R)
+
+
+def c_f(s):
+    """Streamed reals per image*cell: A (unless fixed) and psi."""
+    return (0 if s.no_self_field else 2) + 2 * s.ncomp
+
+
+def active_cells(s):
+    """Image*cells inside the domains (every cell for the full box)."""
+    dom = (np.asarray(s.sc_domain) != 0) | (np.asarray(s.comp_domain) != 0)
+    return s.F * int(np.count_nonzero(dom))
+
+
+def kernel_classes(strings):
+    """The G and P kernels launch once per class of strings (components,
+    fixed A, parameter planes)."""
+    from mepgl_amd.problem import PARAM_NAMES
+
+    return len({(s.ncomp, bool(s.no_self_field),
+                 any(not np.isscalar(getattr(s, n)) for n in PARAM_NAMES)) for s in strings})
+
+
+def roofline(strings, r, sbytes):
+    """Achieved algorithmic GB/s of the dominant step kernel.
+
+    Algorithmic bytes per mask-active image*cell: G with the pending update
+    reads x, d, g_old and writes x, g (5 C_f s); P reads x, g, d and writes
+    d (4 C_f s).  The kernel time per step is the average launch time times
+    the number of item classes (each class is one launch per step)."""
+    ncls = kernel_classes(strings)
+    act = [active_cells(s) for s in strings]
+    out = None
+    for name, per_cf in (("grad_update", 5), ("poly", 4)):
+        ms, n = r["times"][name]
+        if not n:
+            continue
+        step_ms = ms * ncls
+        nbytes = sum(a * per_cf * c_f(s) * sbytes for a, s in zip(act, strings))
+        cand = {"kernel": name, "achieved": nbytes / (step_ms * 1e-3) / 1e9,
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "avg_ms": ms, "launches_per_step": ncls,
+                "algorithmic_bytes_per_step": nbytes,
+                "algorithmic_bytes_per_image_cell": per_cf * c_f(strings[0]) * sbytes
+                if len({c_f(s) for s in strings}) == 1 else None}
+        cand["frac"] = cand["achieved"] / HBM_PEAK_GBS
+        # The dominant kernel: the larger share of the step's time.
+        share = ms * n
+        if out is None or share > out[0]:
+            out = (share, cand)
+    return out[1] if out else None
+
+
+def barrier_errors(e_mixed, e_fp64):
+    """Mixed vs FP64 barriers and energies of every string (post.py:188-189)."""
+    rel_b, e_rel = [], 0.0
+    for m, d in zip(e_mixed, e_fp64):
+        bm, bd = barrier(m), barrier(d)
+        rel_b.append([abs(x - y) / abs(y) if y else None for x, y in zip(bm, bd)])
+        e_rel = max(e_rel, float(np.max(np.abs(m - d) / np.abs(d))))
+    vals = [v for b in rel_b for v in b if v is not None]
+    return (max(vals) if vals else None), e_rel
+
+
+def leg(name, steps, warmup, tune_steps=4, separate=False):
+    """One bounded run of a further workload: mixed and FP64 paths on the
+    same seeded strings (and, for a sweep, one context per string)."""
+    from mepgl_amd import configs
+
+    t0 = time.perf_counter()
+    strings = configs.WORKLOADS[name]()
+    gen_s = time.perf_counter() - t0
+    cells = sum(s.image_cells for s in strings)
+    act = sum(active_cells(s) for s in strings)
+    out = {"workload": name, "strings": len(strings), "image_cells": cells,
+           "active_cell_fraction": act / cells}
+    res = {}
+    for prec in ("mixed", "fp64"):
+        r = run_path(strings, prec, steps, warmup, tune_steps=tune_steps)
+        res[prec] = r
+        v = cells * r["steps"] / r["seconds"]
+        out[prec] = {"value": v, "value_active_cells": v * act / cells,
+                     "ms_per_step": 1e3 * r["seconds"] / r["steps"], "steps": r["steps"],
+                     "segment_rows": sorted(set(r["info"]["rows"])),
+                     "roofline": roofline(strings, r, 4 if prec == "mixed" else 8)}
+        print(f"# {name} {prec}: {r['steps']} steps in {r['seconds']:.4f} s", flush=True)
+    out["mixed_over_fp64"] = out["mixed"]["value"] / out["fp64"]["value"]
+    b_err, e_err = barrier_errors(res["mixed"]["energies"], res["fp64"]["energies"])
+    out["after_steps"] = res["mixed"]["steps"]
+    out["barrier_rel_err_mixed_vs_fp64_unconverged"] = b_err
+    out["energy_rel_err_mixed_vs_fp64"] = e_err
+    if separate:
+        for prec in ("mixed", "fp64"):
+            r = run_separately(strings, prec, steps, warmup)
+            out[f"separate_{prec}"] = cells * r["steps"] / r["seconds"]
+            print(f"# {name} separate {prec}: {r['steps']} steps in {r['seconds']:.4f} s",
+                  flush=True)
+        out["batched_over_separate_mixed"] = out["mixed"]["value"] / out["separate_mixed"]
+        # The target ratio: the mixed batched solver over the plain FP64 path
+        # (one string per context).
+        out["batched_mixed_over_plain_fp64"] = out["mixed"]["value"] / out["separate_fp64"]
+    out["wall_s"] = time.perf_counter() - t0
+    out["generate_s"] = gen_s
+    return out
 
 
 def cpu_model():
@@ -137,40 +249,48 @@ def cpu_baseline(strings, seconds):
         "cores": threads,
         "kind": "port",
         "sample": f"{done} NLCG steps ({dt:.1f} s, median of {len(rates)} chunks of 5) of "
-                  f"string 0 ({s.F} images x {s.N}^2), oracle/mepgl_oracle.c FP64 -O3 "
-                  f"-fopenmp, {threads} threads, {cpu_model()}",
+                  f"string 0 ({s.F} images x {s.N}^2), oracle/mepgl_oracle.c FP64 built "
+                  f"-O3 -march=x86-64-v3 -ffp-contract=off -fopenmp (oracle/Makefile), "
+                  f"{threads} threads, {cpu_model()}",
     }
 
 
 def streaming_ceilings():
-    """Plain copy and 3-read/1-write kernels of 1 GiB on this GPU
-    (mepgl_amd/copy_calib), for the roofline's measured ceilings."""
+    """Plain copy and 3-read/1-write kernels of 1 GiB per array on this GPU
+    (mepgl_amd/copy_calib), for the roofline's measured ceilings: copy and
+    triad with 16-byte accesses per lane, and triad with the 4- and 8-byte
+    accesses of the mixed and FP64 step kernels."""
     import re
     import subprocess
 
     exe = os.path.join(ROOT, "mepgl_amd", "copy_calib")
     out = {}
-    for mode in ("copy", "triad"):
+    for key, mode, width in (("copy", "copy", 16), ("triad", "triad", 16),
+                             ("triad_4B", "triad", 4), ("triad_8B", "triad", 8)):
         try:
-            r = subprocess.run([exe, "1024", "10", mode], capture_output=True, text=True,
-                               timeout=60)
+            r = subprocess.run([exe, "1024", "10", mode, str(width)], capture_output=True,
+                               text=True, timeout=60)
             m = re.search(r"([0-9.]+) GB/s", r.stdout)
-            out[mode] = float(m.group(1)) if (r.returncode == 0 and m) else None
+            out[key] = float(m.group(1)) if (r.returncode == 0 and m) else None
         except (OSError, subprocess.SubprocessError):
-            out[mode] = None
+            out[key] = None
     return out
 
 
-def measured_traffic(workload, precision, kernel):
-    """HBM bytes per launch of `kernel` from the calibrated rocprofv3 counter
-    runs recorded in profiles/traffic.json (null when not recorded)."""
+def measured_traffic(workload, precision, kernel, rows):
+    """HBM bytes per launch of `kernel` at segment height `rows` from the
+    calibrated rocprofv3 counter runs recorded in profiles/traffic.json
+    (null when that height was not recorded)."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             table = json.load(f)
-        return table[workload][precision][kernel]
-    except (OSError, KeyError, ValueError):
+        return table[workload][precision][str(rows)][kernel]["bytes"]
+    except (OSError, KeyError, ValueError, TypeError):
         return None
+
+
+LEGS = ("two_band_512", "sweep_48", "thin_film_disk_1024")
 
 
 def main(argv=None):
@@ -185,6 +305,10 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="both", choices=("both", "fp64", "mixed"))
     ap.add_argument("--segment-rows", type=int, default=0)
+    ap.add_argument("--legs", default=",".join(LEGS),
+                    help="further workloads run after the headline one ('' for none)")
+    ap.add_argument("--leg-steps", type=int, default=0,
+                    help="NLCG steps of each further workload (default: min(steps, 50))")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU: multi-GPU runs are not part of "
@@ -195,23 +319,12 @@ def main(argv=None):
 
     strings = configs.WORKLOADS[args.workload]()
     cells = sum(s.image_cells for s in strings)
-    C_f = [4 if s.no_self_field else 2 + 2 * s.ncomp for s in strings]
-    cf = C_f[0]
+    act_cells = sum(active_cells(s) for s in strings)
+    active = act_cells / cells
 
     if args.compare_batching:
-        steps = args.steps
-        bat = run_path(strings, "mixed", steps, args.warmup)
-        bat64 = run_path(strings, "fp64", steps, args.warmup)
-        sep = run_separately(strings, "mixed", steps, args.warmup)
-        v_b = cells * bat["steps"] / bat["seconds"]
-        print(json.dumps({
-            "workload": args.workload, "strings": len(strings), "image_cells": cells,
-            "batched_mixed": v_b,
-            "batched_fp64": cells * bat64["steps"] / bat64["seconds"],
-            "separate_mixed": cells * sep["steps"] / sep["seconds"],
-            "batched_over_separate": v_b / (cells * sep["steps"] / sep["seconds"]),
-            "mixed_over_fp64": v_b / (cells * bat64["steps"] / bat64["seconds"]),
-            "rows_mixed": bat["info"]["rows"], "unit": "image*cells/s"}))
+        print(json.dumps(leg(args.workload, args.steps, args.warmup, tune_steps=10,
+                             separate=True)))
         return 0
 
     res = {}
@@ -224,36 +337,31 @@ def main(argv=None):
         r = res[args.precision]
         print(json.dumps({"precision": args.precision, "ms_per_step": 1e3 * r["seconds"] / r["steps"],
                           "value": cells * r["steps"] / r["seconds"],
-                          "segment_rows": r["info"]["tile_rows"]}))
+                          "segment_rows": r["info"]["tile_rows"],
+                          "roofline": roofline(strings, r, 4 if args.precision == "mixed" else 8)}))
         return 0
 
     mixed, fp64 = res["mixed"], res["fp64"]
     value = cells * mixed["steps"] / mixed["seconds"]
     value64 = cells * fp64["steps"] / fp64["seconds"]
-
-    # Roofline of the dominant kernel (mixed path), algorithmic bytes:
-    # G with update reads x, d, g_old and writes x, g: 5 C_f s per image*cell;
-    # P reads x, g, d and writes d: 4 C_f s.
-    def roof(r, sbytes):
-        g_ms, g_n = r["times"]["grad_update"]
-        p_ms, p_n = r["times"]["poly"]
-        cand = [("grad_update", g_ms, g_n, 5 * cf * sbytes), ("poly", p_ms, p_n, 4 * cf * sbytes)]
-        name, ms, n, per = max(cand, key=lambda t: t[1] * t[2])
-        ach = cells * per / (ms * 1e-3) / 1e9
-        return name, ms, ach, per
-
-    kname, kms, ach, per = roof(mixed, 4)
-    kname64, kms64, ach64, per64 = roof(fp64, 8)
+    roof = roofline(strings, mixed, 4)
+    roof64 = roofline(strings, fp64, 8)
     b_m = barrier(mixed["energies"][0])
     b_d = barrier(fp64["energies"][0])
     rel_b = [abs(m - d) / abs(d) if d else None for m, d in zip(b_m, b_d)]
     e_rel = float(np.max(np.abs(mixed["energies"][0] - fp64["energies"][0]) /
                          np.abs(fp64["energies"][0])))
-    traffic = measured_traffic(args.workload, "mixed", kname)
+    rows_m = mixed["info"]["tile_rows"]
     ceil = streaming_ceilings()
-    # Cells inside the superconducting domain (the active fraction of masked
-    # geometries; the kernels stream every cell).
-    active = sum(s.F * float(np.count_nonzero(s.sc_domain)) for s in strings) / cells
+    roof.update({
+        "bound": "hbm",
+        "traffic": measured_traffic(args.workload, "mixed", roof["kernel"], rows_m),
+        "traffic_segment_rows": rows_m,
+        "algorithmic_bytes_per_launch": roof["algorithmic_bytes_per_step"],
+        "measured_ceilings_GBs": ceil,
+        "frac_of_triad_ceiling": roof["achieved"] / ceil["triad"] if ceil.get("triad") else None})
+    roof64["traffic"] = measured_traffic(args.workload, "fp64", roof64["kernel"],
+                                         fp64["info"]["tile_rows"])
 
     out = {
         "metric": "image*cells/s per NLCG step (mixed precision)",
@@ -271,24 +379,18 @@ def main(argv=None):
         "config": {"workload": args.workload, "strings": len(strings),
                    "images": sum(s.F for s in strings), "N": strings[0].N,
                    "components": strings[0].ncomp, "mode": "L", "reparam_every": R,
-                   "segment_rows": mixed["info"]["tile_rows"],
+                   "segment_rows": rows_m,
                    "segment_rows_fp64": fp64["info"]["tile_rows"]},
         "fp64": {"value": value64, "value_active_cells": value64 * active,
-                 "ms_per_step": 1e3 * fp64["seconds"] / fp64["steps"],
-                 "roofline": {"kernel": kname64, "achieved": ach64, "peak": HBM_PEAK_GBS,
-                              "unit": "GB/s", "frac": ach64 / HBM_PEAK_GBS,
-                              "avg_ms": kms64}},
+                 "ms_per_step": 1e3 * fp64["seconds"] / fp64["steps"], "roofline": roof64},
         "mixed_over_fp64": value / value64,
         "energy_rel_err_mixed_vs_fp64": e_rel,
         "barrier_fp64": {"entry": b_d[0], "exit": b_d[1]},
         "barrier_mixed": {"entry": b_m[0], "exit": b_m[1]},
+        # After `steps` steps from the initial guess, not at convergence
+        # (the converged comparison is a GPU test).
         "barrier_rel_err_mixed_vs_fp64": {"entry": rel_b[0], "exit": rel_b[1]},
-        "roofline": {"bound": "hbm", "kernel": kname, "achieved": ach, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": ach / HBM_PEAK_GBS, "traffic": traffic,
-                     "algorithmic_bytes_per_image_cell": per,
-                     "algorithmic_bytes_per_launch": per * cells, "avg_ms": kms,
-                     "measured_ceilings_GBs": ceil,
-                     "frac_of_triad_ceiling": ach / ceil["triad"] if ceil.get("triad") else None},
+        "roofline": roof,
         "active_cell_fraction": active,
         # For masked geometries the metric counts mask-active image*cells
         # only; equal to value when the domain is the full box.
@@ -296,6 +398,13 @@ def main(argv=None):
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(strings, args.cpu_seconds)
+    legs = [w for w in args.legs.split(",") if w and w != args.workload]
+    if legs:
+        steps = args.leg_steps or min(args.steps, 50)
+        out["legs"] = {}
+        for w in legs:
+            print(f"# leg {w}: {steps} steps", flush=True)
+            out["legs"][w] = leg(w, steps, min(args.warmup, 10), separate=(w == "sweep_48"))
     print(json.dumps(out))
     return 0
 

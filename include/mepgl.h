@@ -53,6 +53,15 @@ extern "C" {
 #define MEPGL_GAUGE_ALIGN 8u    /* modes L and C align the constant phase of
                                    neighbouring images before the arclength
                                    (mepgl_gauge_align); off = reference    */
+#define MEPGL_GAUGE_ALIGN_LOCAL 32u /* modes L and C align neighbouring images
+                                   by a local gauge transformation before the
+                                   arclength (mepgl_gauge_align_local), and
+                                   before the constant-phase alignment when
+                                   both are set; off = reference           */
+#define MEPGL_ROLLBACK 16u      /* mepgl_loop keeps a device copy of the
+                                   solver state from the start of each outer
+                                   iteration and restores it when that
+                                   iteration produces a non-finite value   */
 
 /* A per-site parameter: a uniform value, or an N*N plane when plane != NULL. */
 typedef struct {
@@ -83,7 +92,11 @@ typedef struct mepgl_ctx mepgl_ctx;
 /* Creates a context for n_strings strings on HIP device `device`, builds the
  * neighbour maps (buildNearestNeighbourMap, nlcg.cu:48-103, launched from
  * mepgl.cu:598-599) and puts the solver in the state loop() starts from
- * (mepgl.cu:124-140).  Fields start at zero until mepgl_upload. */
+ * (mepgl.cu:124-140).  Fields start at zero until mepgl_upload.
+ * Returns MEPGL_ERR_ARG, before touching the device and with *out NULL,
+ * when one image's field set of a string would exceed 2^31 - 1 bytes
+ * ((N + 1) * (2 + 2 ncomp) * roundup(N, 64) stored reals: N above about
+ * 8,190 for one component in FP64, 6,680 for two). */
 int mepgl_create(const mepgl_string_desc *strings, int n_strings, int precision,
                  unsigned flags, int device, mepgl_ctx **out);
 void mepgl_destroy(mepgl_ctx *ctx);
@@ -134,7 +147,10 @@ int mepgl_arc_length(mepgl_ctx *ctx);
 int mepgl_redistribute(mepgl_ctx *ctx, int cubic);
 
 /* The end-of-run resampling of mepgl.cu:699-766: arclength, then linear
- * redistribution of one string to F_new images (reallocates its buffers). */
+ * redistribution of one string to F_new images (reallocates its buffers).
+ * That string restarts from loop()'s initial solver state (zero gradients
+ * and direction, infinite norms); the other strings of the context keep
+ * their gradients, directions and per-image scalars. */
 int mepgl_resample(mepgl_ctx *ctx, int string, int F_new);
 
 /* Aligns the constant phase of neighbouring images: image n is rotated by
@@ -146,6 +162,20 @@ int mepgl_resample(mepgl_ctx *ctx, int string, int F_new);
  * angles with MEPGL_SCALAR_THETA. */
 int mepgl_gauge_align(mepgl_ctx *ctx);
 
+/* Aligns neighbouring images by a local gauge transformation: for image
+ * n >= 1, xi_n minimises |A_n + D xi - A_{n-1}|^2 (D: forward differences,
+ * zero in the last column for x and the last row for y), solved exactly in
+ * the DCT-II eigenbasis of the grid's Neumann Laplacian; chi_n = chi_{n-1} +
+ * xi_n is applied as A_n += D chi_n and psi_a,n *= exp(i q_a chi_n) to the
+ * fields, and to the psi parts of the direction and both gradients.  No
+ * counterpart in the reference, which measures image distances with the
+ * gauge-invariant metric s_jb (mepgl.cu:31-32) and interpolates the raw
+ * fields (interpolation.cu:190-227).  The plaquette curls of every image
+ * are unchanged and no neighbour A distance grows; the energy changes at
+ * first order in dx (the discretisation is gauge covariant to that order).
+ * Strings with a fixed A (no self field) are left unchanged. */
+int mepgl_gauge_align_local(mepgl_ctx *ctx);
+
 /* Resets the Polak-Ribiere memory of the interior images
  * (mepgl.cu:231-234). */
 int mepgl_reset_interior_beta(mepgl_ctx *ctx);
@@ -155,7 +185,12 @@ int mepgl_reset_interior_beta(mepgl_ctx *ctx);
  * reparametrisation then relaxation) or 'C' (cubic).  Each outer iteration
  * is one hipGraph replay unless MEPGL_NO_GRAPH.  As in the reference
  * (mepgl.cu:357-359), the stop control is checked after each iteration, so
- * a call runs at least one; the control stays set until cleared. */
+ * a call runs at least one; the control stays set until cleared.
+ * Every outer iteration is checked for a NaN or +-Inf energy, line-search
+ * coefficient or gradient norm; the first such iteration ends the call with
+ * MEPGL_ERR_NONFINITE (mepgl_last_error names it and the first bad image).
+ * Without MEPGL_ROLLBACK at most one further iteration has run by then;
+ * with it, the state from the start of the bad iteration is restored. */
 int mepgl_loop(mepgl_ctx *ctx, int iterations, int relaxation_steps, char mode, int nlcg);
 
 /* Per-image scalars of one string (F values, 4F for MEPGL_SCALAR_POLY). */

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(HERE, "libmepgl.so")
 
 OK, ERR_ARG, ERR_HIP, ERR_NONFINITE, ERR_STATE, ERR_NOMEM = 0, -1, -2, -3, -4, -5
 FP64, MIXED = 0, 1
-POLY_EXACT, CUBIC_TEXTBOOK, NO_GRAPH, GAUGE_ALIGN = 1, 2, 4, 8
+POLY_EXACT, CUBIC_TEXTBOOK, NO_GRAPH, GAUGE_ALIGN, ROLLBACK, GAUGE_ALIGN_LOCAL = 1, 2, 4, 8, 16, 32
 
 SCALARS = {
     "fenergy": 0, "g_norm2": 1, "alpha": 2, "s": 3, "s_jb": 4, "beta": 5, "g_delta_g": 6,
@@ -27,7 +27,7 @@ EXPORTS = [
     "mepgl_create", "mepgl_destroy", "mepgl_last_error", "mepgl_upload", "mepgl_download",
     "mepgl_reset_solver", "mepgl_free_energy", "mepgl_gradient", "mepgl_nlcg_steps",
     "mepgl_compute_jb", "mepgl_arc_length", "mepgl_redistribute", "mepgl_resample",
-    "mepgl_reset_interior_beta", "mepgl_gauge_align", "mepgl_loop", "mepgl_get_scalars", "mepgl_download_state",
+    "mepgl_reset_interior_beta", "mepgl_gauge_align", "mepgl_gauge_align_local", "mepgl_loop", "mepgl_get_scalars", "mepgl_download_state",
     "mepgl_set_control", "mepgl_get_control", "mepgl_profile", "mepgl_profile_read",
     "mepgl_synchronize", "mepgl_info", "mepgl_set_segment_rows", "mepgl_autotune", "mepgl_upload_state", "mepgl_set_scalars", "mepgl_winding",
 ]
@@ -92,6 +92,7 @@ def load():
     L.mepgl_resample.argtypes = [vp, C.c_int, C.c_int]
     L.mepgl_reset_interior_beta.argtypes = [vp]
     L.mepgl_gauge_align.argtypes = [vp]
+    L.mepgl_gauge_align_local.argtypes = [vp]
     L.mepgl_loop.argtypes = [vp, C.c_int, C.c_int, C.c_char, C.c_int]
     L.mepgl_get_scalars.argtypes = [vp, C.c_int, C.c_int, _dp]
     L.mepgl_download_state.argtypes = [vp, C.c_int, C.c_int, _dp, _dp, _dp]

@@ -59,6 +59,9 @@ struct DevString {
   int N, pitch, F, nc;      // grid side, row pitch, images, components
   int nplanes;              // 2 + 2*nc
   int nsf;                  // NO_SELF_FIELD: A is fixed, no A gradient
+  int a_shared;             // NO_SELF_FIELD with the same A in every image:
+                            // the step kernels read image 0's A planes and
+                            // the reparametrisation leaves A untouched
   int img0;                 // global image id of image 0
   int tiles_x, tiles_y;     // 64-column strips x row segments per image
   int seg_rows;             // rows per segment (one wavefront marches them)

@@ -16,6 +16,7 @@
 
 #include "../../include/mepgl.h"
 #include "common.hpp"
+#include "gauge_local.hpp"
 #include "kernels.hpp"
 
 using namespace mepgl;
@@ -34,6 +35,7 @@ struct StringHost {
   int ntiles = 0;
   std::vector<uint16_t> nn_host;  // neighbour map, [row][column] at the row pitch
   double* jb = nullptr;     // JB planes (lazy)
+  double* dct = nullptr;    // DCT-II matrix and Laplacian eigenvalues (lazy)
   size_t elem = 8;          // bytes per stored real
 };
 
@@ -78,6 +80,12 @@ struct mepgl_ctx {
   bool dry = false;          // advance the parity state without launching
   std::map<std::tuple<int, int, int, int, int, char>, hipGraphExec_t> graphs;
   long long bytes = 0;
+  // Non-finite flag: set to 1 by the step kernels (a gradient total) or the
+  // finishing launch (a line-search coefficient) when a per-image value is
+  // NaN or +-Inf.  Pinned host memory, written by the device, read by the
+  // host after an event that follows the writing launch.
+  int* health = nullptr;
+  hipEvent_t iter_done[2] = {nullptr, nullptr};
 
   bool fp32() const { return precision == MEPGL_MIXED; }
 };
@@ -171,6 +179,7 @@ StepArgs step_args(mepgl_ctx* c) {
   a.sc = c->d_sc;
   a.ls = c->d_ls;
   a.nlcg = 1;
+  a.health = c->health;
   return a;
 }
 
@@ -186,6 +195,7 @@ AuxArgs aux_args(mepgl_ctx* c) {
   a.dcur = c->dp;
   a.out = c->d_out;
   a.jb = c->d_jb;
+  a.only_string = -1;
   return a;
 }
 
@@ -253,6 +263,7 @@ FinishArgs finish_args(mepgl_ctx* c) {
   f.ls = c->d_ls;
   f.out = c->d_out;
   f.xin = c->xp;
+  f.health = c->health;
   return f;
 }
 
@@ -321,19 +332,35 @@ int nlcg_step(mepgl_ctx* c, int k, int nlcg, int maxwell) {
   return MEPGL_OK;
 }
 
-int check_finite(mepgl_ctx* c) {
+// Names the first image whose energy, line-search coefficients or gradient
+// norms are not finite (a gradient norm may be +Inf only in the slot of the
+// previous step, where a solver reset puts it), after the stream is idle.
+std::string first_nonfinite_image(mepgl_ctx* c) {
   std::vector<ImageScalars> h(c->total_images);
-  HIP_TRY(c, hipMemcpyAsync(h.data(), c->d_sc, sizeof(ImageScalars) * h.size(),
-                            hipMemcpyDeviceToHost, c->stream));
-  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (hipMemcpy(h.data(), c->d_sc, sizeof(ImageScalars) * h.size(), hipMemcpyDeviceToHost) !=
+      hipSuccess)
+    return "";
   for (size_t si = 0; si < c->strings.size(); ++si) {
     const DevString& s = c->strings[si].dev;
     for (int n = 0; n < s.F; ++n) {
       const ImageScalars& x = h[s.img0 + n];
-      if (!std::isfinite(x.fenergy) || std::isnan(x.g_norm2[0]) || std::isnan(x.g_norm2[1]))
-        return fail(c, MEPGL_ERR_NONFINITE, "non-finite energy or gradient norm in string " +
-                                                std::to_string(si) + " image " + std::to_string(n));
+      bool bad = !std::isfinite(x.fenergy) || std::isnan(x.g_norm2[0]) ||
+                 std::isnan(x.g_norm2[1]) || !std::isfinite(x.g_delta_g);
+      for (double v : x.poly) bad = bad || !std::isfinite(v);
+      if (bad) return " (string " + std::to_string(si) + " image " + std::to_string(n) + ")";
     }
+  }
+  return "";
+}
+
+// Reads and clears the non-finite flag once the stream is idle.
+int check_finite(mepgl_ctx* c, const std::string& where = "") {
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  const int flag = *(volatile int*)c->health;
+  const std::string img = first_nonfinite_image(c);
+  if (flag || !img.empty()) {
+    *(volatile int*)c->health = 0;
+    return fail(c, MEPGL_ERR_NONFINITE, "non-finite energy or gradient norm" + where + img);
   }
   return MEPGL_OK;
 }
@@ -644,11 +671,75 @@ int redistribute_all(mepgl_ctx* c, int cubic) {
     if ((rc = run_aux(c, a, kKernelLinear, 0))) return rc;
   }
   // NO_SELF_FIELD strings keep the same fixed A in both field buffers: copy
-  // the new A (buffer xout) over the old one (buffer xin).
-  bool any_nsf = false;
-  for (auto& s : c->strings) any_nsf = any_nsf || s.dev.nsf;
-  if (any_nsf && (rc = run_aux(c, a, kKernelCopyA, 0))) return rc;
+  // the new A (buffer xout) over the old one (buffer xin).  A shared by every
+  // image is not interpolated, so both buffers still hold it.
+  bool copy_a = false;
+  for (auto& s : c->strings) copy_a = copy_a || (s.dev.nsf && !s.dev.a_shared);
+  if (copy_a && (rc = run_aux(c, a, kKernelCopyA, 0))) return rc;
   c->xp = 1 - c->xp;
+  return MEPGL_OK;
+}
+
+// The orthonormal DCT-II matrix C[i][k] = w_k cos(pi k (i + 1/2) / N) and the
+// eigenvalues lam_k = 2 - 2 cos(pi k / N) of the Neumann graph Laplacian of a
+// path of N sites, per string (N * N + N doubles on the device).
+int ensure_dct(mepgl_ctx* c) {
+  for (auto& s : c->strings) {
+    if (s.dct || s.dev.nsf) continue;
+    const int N = s.dev.N;
+    const double pi = 3.14159265358979323846;
+    std::vector<double> h((size_t)N * N + N);
+    for (int i = 0; i < N; ++i)
+      for (int k = 0; k < N; ++k) {
+        const double w = k == 0 ? std::sqrt(1.0 / N) : std::sqrt(2.0 / N);
+        h[(size_t)i * N + k] = w * std::cos(pi * k * (i + 0.5) / N);
+      }
+    for (int k = 0; k < N; ++k) h[(size_t)N * N + k] = 2.0 - 2.0 * std::cos(pi * k / N);
+    void* p;
+    int rc = dev_alloc(c, &s, &p, h.size() * sizeof(double));
+    if (rc) return rc;
+    HIP_TRY(c, hipMemcpyAsync(p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice,
+                              c->stream));
+    HIP_TRY(c, hipStreamSynchronize(c->stream));
+    s.dct = (double*)p;
+  }
+  return MEPGL_OK;
+}
+
+// Local gauge alignment of neighbouring images (optional; no reference
+// function): chi_n from the least-squares problem of every neighbouring pair
+// (rhs, four DCT products, prefix along the string), then applied to the
+// fields, the direction and both gradient buffers (gauge_local.hip).
+int gauge_align_local_all(mepgl_ctx* c) {
+  for (auto& s : c->strings)
+    if (s.dev.F < 2) return fail(c, MEPGL_ERR_ARG, "alignment needs at least 2 images");
+  int rc = flush(c);
+  if (rc) return rc;
+  if ((rc = ensure_cubic_rhs(c)) || (rc = ensure_dct(c))) return rc;
+  if (c->dry) return MEPGL_OK;
+  AuxArgs a = aux_args(c);
+  hipError_t e = launch_glocal(a, 0, c->fp32(), c->stream);
+  for (auto& s : c->strings) {
+    const DevString& d = s.dev;
+    if (d.nsf || e != hipSuccess) continue;
+    const int N = d.N, F = d.F;
+    const long long IS = d.image_stride, PS = d.plane_stride, P = d.pitch;
+    double* w0 = (double*)d.cubic_rhs;
+    double* w1 = w0 + PS;
+    const double* C = s.dct;
+    const double* lam = s.dct + (size_t)N * N;
+    // b C, C^T (b C) / (lam_l + lam_k), C (.), (.) C^T
+    const GemmArgs st[4] = {
+        {w0, P, 1, IS, C, N, 1, 0, w1, P, IS, N, nullptr},
+        {C, 1, N, 0, w1, P, 1, IS, w0, P, IS, N, lam},
+        {C, N, 1, 0, w0, P, 1, IS, w1, P, IS, N, nullptr},
+        {w1, P, 1, IS, C, 1, N, 0, w0, P, IS, N, nullptr}};
+    for (const auto& g : st)
+      if (e == hipSuccess) e = launch_dct_gemm(g, F, c->stream);
+    if (e == hipSuccess) e = launch_glocal_prefix(w0, N, F, P, IS, c->stream);
+  }
+  if (e == hipSuccess) e = launch_glocal(a, 1, c->fp32(), c->stream);
+  if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, std::string("launch: ") + hipGetErrorString(e));
   return MEPGL_OK;
 }
 
@@ -670,6 +761,7 @@ int gauge_align_all(mepgl_ctx* c) {
 int outer_iteration(mepgl_ctx* c, int relax, char mode, int nlcg) {
   int rc;
   if (mode == 'L' || mode == 'C') {
+    if ((c->flags & MEPGL_GAUGE_ALIGN_LOCAL) && (rc = gauge_align_local_all(c))) return rc;
     if ((c->flags & MEPGL_GAUGE_ALIGN) && (rc = gauge_align_all(c))) return rc;
     // The interior norms are reset (beta = 0 on the next step) by the
     // arclength finalize; the redistribution does not read them.
@@ -762,6 +854,19 @@ int restore_snapshot(mepgl_ctx* c, const Snapshot& sn) {
   return MEPGL_OK;
 }
 
+// Copies the live solver state into an existing snapshot (same buffers).
+int refresh_snapshot(mepgl_ctx* c, Snapshot& sn) {
+  for (size_t k = 0; k < sn.buf.size(); ++k)
+    HIP_TRY(c, hipMemcpyAsync(sn.buf[k].first, sn.buf[k].second, sn.bytes[k],
+                              hipMemcpyDeviceToDevice, c->stream));
+  HIP_TRY(c, hipMemcpyAsync(sn.sc, c->d_sc, sizeof(ImageScalars) * c->total_images,
+                            hipMemcpyDeviceToDevice, c->stream));
+  sn.xp = c->xp;
+  sn.dp = c->dp;
+  sn.pending = c->pending;
+  return MEPGL_OK;
+}
+
 // Timing rounds per candidate height.
 constexpr int kTuneRounds = 3;
 
@@ -815,6 +920,18 @@ int mepgl_create(const mepgl_string_desc* strings, int n_strings, int precision,
   *out = nullptr;
   if (!strings || n_strings <= 0) return MEPGL_ERR_ARG;
   if (precision != MEPGL_FP64 && precision != MEPGL_MIXED) return MEPGL_ERR_ARG;
+  // The step kernels address one image's fields through a buffer resource
+  // with 32-bit byte offsets: every image's field set must stay below 2 GiB.
+  // Checked before any HIP call.
+  for (int i = 0; i < n_strings; ++i) {
+    const long long N = strings[i].N;
+    if (N < 2 || N > (1 << 20) || (strings[i].ncomp != 1 && strings[i].ncomp != 2))
+      continue;  // rejected with a message below
+    const long long elem = precision == MEPGL_MIXED ? 4 : 8;
+    const long long image_bytes =
+        (N + kPlanePadRows) * (2 + 2 * strings[i].ncomp) * ((N + 63) / 64 * 64) * elem;
+    if (image_bytes > 0x7fffffffLL) return MEPGL_ERR_ARG;
+  }
   std::unique_ptr<mepgl_ctx> c(new mepgl_ctx());
   c->precision = precision;
   c->flags = flags;
@@ -823,6 +940,18 @@ int mepgl_create(const mepgl_string_desc* strings, int n_strings, int precision,
   if (e != hipSuccess) return MEPGL_ERR_HIP;
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) return MEPGL_ERR_HIP;
+  if (hipHostMalloc((void**)&c->health, 64, hipHostMallocDefault) != hipSuccess) {
+    c->health = nullptr;
+    mepgl_destroy(c.release());
+    return MEPGL_ERR_NOMEM;
+  }
+  *(volatile int*)c->health = 0;
+  for (auto& ev : c->iter_done)
+    if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+      ev = nullptr;
+      mepgl_destroy(c.release());
+      return MEPGL_ERR_HIP;
+    }
 
   long long total_img = 0;
   for (int i = 0; i < n_strings; ++i) total_img += std::max(strings[i].F, 0);
@@ -850,6 +979,7 @@ int mepgl_create(const mepgl_string_desc* strings, int n_strings, int precision,
     d.nc = in.ncomp;
     d.nplanes = 2 + 2 * in.ncomp;
     d.nsf = in.no_self_field ? 1 : 0;
+    d.a_shared = d.nsf;  // the fields start at zero in every image
     d.poly_exact = (flags & MEPGL_POLY_EXACT) ? 1 : 0;
     d.seg_rows = choose_rows(in, total_img);
     d.tiles_x = (in.N + kTileX - 1) / kTileX;
@@ -978,6 +1108,9 @@ void mepgl_destroy(mepgl_ctx* c) {
   (void)hipFree(c->d_jb);
   (void)hipFree(c->d_out);
   for (auto e : c->timing.pool) (void)hipEventDestroy(e);
+  for (auto e : c->iter_done)
+    if (e) (void)hipEventDestroy(e);
+  if (c->health) (void)hipHostFree(c->health);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -991,6 +1124,19 @@ int mepgl_upload(mepgl_ctx* c, int si, const double* a, const double* p1, const 
   HIP_TRY(c, hipSetDevice(c->device));
   int rc = flush(c);  // other strings keep their owed update
   if (rc) return rc;
+  // A NO_SELF_FIELD string whose A is the same in every image keeps one A
+  // for all of them (image 0's planes); any other keeps A per image.
+  if (s.dev.nsf) {
+    const size_t img = (size_t)s.dev.N * s.dev.N * 2;
+    bool same = true;
+    for (int k = 1; k < s.dev.F && same; ++k)
+      same = std::memcmp(a, a + (size_t)k * img, img * sizeof(double)) == 0;
+    if ((int)same != s.dev.a_shared) {
+      s.dev.a_shared = same ? 1 : 0;
+      destroy_graphs(c);
+      if ((rc = upload_descriptors(c))) return rc;
+    }
+  }
   const size_t n = (size_t)s.dev.F * s.dev.image_stride;
   for (int b = 0; b < 2; ++b) {
     if (c->fp32()) {
@@ -1075,7 +1221,7 @@ int mepgl_nlcg_steps(mepgl_ctx* c, int iter_n, int nlcg, int maxwell) {
     int rc = nlcg_step(c, k, nlcg ? 1 : 0, maxwell ? 1 : 0);
     if (rc) return rc;
   }
-  return check_finite(c);
+  return check_finite(c, " after nlcg_steps");
 }
 
 int mepgl_compute_jb(mepgl_ctx* c, int si, double* b, double* j1, double* j2) {
@@ -1138,6 +1284,12 @@ int mepgl_gauge_align(mepgl_ctx* c) {
   if (!c) return MEPGL_ERR_ARG;
   HIP_TRY(c, hipSetDevice(c->device));
   return gauge_align_all(c);
+}
+
+int mepgl_gauge_align_local(mepgl_ctx* c) {
+  if (!c) return MEPGL_ERR_ARG;
+  HIP_TRY(c, hipSetDevice(c->device));
+  return gauge_align_local_all(c);
 }
 
 int mepgl_reset_interior_beta(mepgl_ctx* c) {
@@ -1213,12 +1365,28 @@ int mepgl_resample(mepgl_ctx* c, int si, int F_new) {
   if ((rc = alloc_images(c, s, F_new))) return rc;
   s.desc.F = F_new;
   destroy_graphs(c);
-  if ((rc = alloc_scalars(c)) || (rc = upload_descriptors(c)) || (rc = build_items(c))) return rc;
-  const bool saved_pending = c->pending;
-  c->pending = false;
+  // The other strings keep their solver state: their per-image scalars move
+  // to their new global image ids, and their gradients and directions are
+  // untouched.  String si starts from loop()'s initial state (zero gradients
+  // and direction from alloc_images, infinite norms).
+  std::vector<int> old_img0;
+  for (auto& t : c->strings) old_img0.push_back(t.dev.img0);
+  if ((rc = alloc_scalars(c))) return rc;
+  std::vector<ImageScalars> hn(c->total_images);
+  for (size_t k = 0; k < c->strings.size(); ++k) {
+    if ((int)k == si) continue;
+    const DevString& d = c->strings[k].dev;
+    for (int n = 0; n < d.F; ++n) hn[d.img0 + n] = h[old_img0[k] + n];
+  }
+  HIP_TRY(c, hipMemcpyAsync(c->d_sc, hn.data(), sizeof(ImageScalars) * hn.size(),
+                            hipMemcpyHostToDevice, c->stream));
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if ((rc = upload_descriptors(c)) || (rc = build_items(c))) return rc;
+  // arc_length_all applied any owed update, so nothing is pending here.
   if ((rc = mepgl_upload(c, si, na.data(), n1.data(), n2.data()))) return rc;
-  c->pending = saved_pending;
-  if ((rc = reset_solver(c))) return rc;
+  AuxArgs ra = aux_args(c);
+  ra.only_string = si;
+  if ((rc = run_aux(c, ra, kKernelResetNorms, 0))) return rc;
   return arc_length_all(c);
 }
 
@@ -1228,47 +1396,116 @@ int mepgl_loop(mepgl_ctx* c, int iterations, int relax, char mode, int nlcg) {
     return fail(c, MEPGL_ERR_ARG, std::string("unknown mode ") + mode);
   HIP_TRY(c, hipSetDevice(c->device));
   int rc;
-  if (mode == 'C' && (rc = ensure_cubic_rhs(c))) return rc;
+  // Buffers an outer iteration needs are allocated here, outside any capture.
+  const bool local = (c->flags & MEPGL_GAUGE_ALIGN_LOCAL) && (mode == 'L' || mode == 'C');
+  if ((mode == 'C' || local) && (rc = ensure_cubic_rhs(c))) return rc;
+  if (local && (rc = ensure_dct(c))) return rc;
   const bool use_graph = !(c->flags & MEPGL_NO_GRAPH) && !c->timing.on;
-  for (int it = 0; it < iterations; ++it) {
-    if (it > 0 && c->stop.load()) break;
-    if (!use_graph) {
-      if ((rc = outer_iteration(c, relax, mode, nlcg ? 1 : 0))) return rc;
-      continue;
-    }
-    // Replay a captured outer iteration.  The capture depends on the parity
-    // state it starts from, which repeats after one or two iterations.
-    auto key = std::make_tuple(c->xp, c->dp, (int)c->pending, relax, nlcg ? 1 : 0, mode);
-    auto itg = c->graphs.find(key);
-    if (itg == c->graphs.end()) {
-      const int xp0 = c->xp, dp0 = c->dp;
-      const bool pend0 = c->pending;
-      HIP_TRY(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      rc = outer_iteration(c, relax, mode, nlcg ? 1 : 0);
-      hipGraph_t g = nullptr;
-      hipError_t e = hipStreamEndCapture(c->stream, &g);
-      c->xp = xp0;
-      c->dp = dp0;
-      c->pending = pend0;
-      if (rc) {
-        if (g) (void)hipGraphDestroy(g);
-        return rc;
-      }
-      if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, "graph capture failed");
-      hipGraphExec_t ex;
-      e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, "graph instantiate failed");
-      itg = c->graphs.emplace(key, ex).first;
-    }
-    HIP_TRY(c, hipGraphLaunch(itg->second, c->stream));
-    // Advance the parity state exactly as the captured sequence does.
-    c->dry = true;
-    rc = outer_iteration(c, relax, mode, nlcg ? 1 : 0);
-    c->dry = false;
-    if (rc) return rc;
+  const bool rollback = (c->flags & MEPGL_ROLLBACK) != 0;
+  // Non-finite guard, once per outer iteration.  Without rollback the check
+  // of iteration it runs while iteration it + 1 is already queued, so the
+  // device never waits for the host; with rollback the solver state is
+  // copied before every iteration and checked right after it.
+  // A flag left by calls that do not report (the autotuner's trial steps)
+  // is stale here.
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  *(volatile int*)c->health = 0;
+  Snapshot sn;
+  if (rollback && (rc = take_snapshot(c, sn))) {
+    drop_snapshot(c, sn);
+    return rc;
   }
-  return check_finite(c);
+  auto bad_iteration = [&](int it) {
+    std::string msg = " in outer iteration " + std::to_string(it);
+    if (rollback) {
+      if (restore_snapshot(c, sn) == MEPGL_OK && hipStreamSynchronize(c->stream) == hipSuccess)
+        msg += "; the solver state was rolled back to the start of that iteration";
+      drop_snapshot(c, sn);
+    }
+    *(volatile int*)c->health = 0;
+    return fail(c, MEPGL_ERR_NONFINITE,
+                "non-finite energy or gradient norm" + msg + first_nonfinite_image(c));
+  };
+  int it = 0;
+  for (; it < iterations; ++it) {
+    if (it > 0 && c->stop.load()) break;
+    if (rollback && it > 0 && (rc = refresh_snapshot(c, sn))) break;
+    if (!use_graph) {
+      if ((rc = outer_iteration(c, relax, mode, nlcg ? 1 : 0))) break;
+    } else {
+      // Replay a captured outer iteration.  The capture depends on the parity
+      // state it starts from, which repeats after one or two iterations.
+      auto key = std::make_tuple(c->xp, c->dp, (int)c->pending, relax, nlcg ? 1 : 0, mode);
+      auto itg = c->graphs.find(key);
+      if (itg == c->graphs.end()) {
+        const int xp0 = c->xp, dp0 = c->dp;
+        const bool pend0 = c->pending;
+        hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+        if (e != hipSuccess) {
+          rc = fail(c, MEPGL_ERR_HIP, "graph capture failed to start");
+          break;
+        }
+        rc = outer_iteration(c, relax, mode, nlcg ? 1 : 0);
+        hipGraph_t g = nullptr;
+        e = hipStreamEndCapture(c->stream, &g);
+        c->xp = xp0;
+        c->dp = dp0;
+        c->pending = pend0;
+        if (rc) {
+          if (g) (void)hipGraphDestroy(g);
+          break;
+        }
+        if (e != hipSuccess) {
+          rc = fail(c, MEPGL_ERR_HIP, "graph capture failed");
+          break;
+        }
+        hipGraphExec_t ex;
+        e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+          rc = fail(c, MEPGL_ERR_HIP, "graph instantiate failed");
+          break;
+        }
+        itg = c->graphs.emplace(key, ex).first;
+      }
+      hipError_t e = hipGraphLaunch(itg->second, c->stream);
+      if (e != hipSuccess) {
+        rc = fail(c, MEPGL_ERR_HIP, std::string("graph launch: ") + hipGetErrorString(e));
+        break;
+      }
+      // Advance the parity state exactly as the captured sequence does.
+      c->dry = true;
+      rc = outer_iteration(c, relax, mode, nlcg ? 1 : 0);
+      c->dry = false;
+      if (rc) break;
+    }
+    hipEvent_t done = c->iter_done[it & 1];
+    hipError_t e = hipEventRecord(done, c->stream);
+    if (e == hipSuccess && rollback) e = hipEventSynchronize(done);
+    if (e != hipSuccess) {
+      rc = fail(c, MEPGL_ERR_HIP, std::string("iteration event: ") + hipGetErrorString(e));
+      break;
+    }
+    if (rollback) {
+      if (*(volatile int*)c->health) return bad_iteration(it);
+    } else if (it > 0) {
+      // Iteration it - 1 is done once its event is; iteration it runs.
+      if ((e = hipEventSynchronize(c->iter_done[(it - 1) & 1])) != hipSuccess) {
+        rc = fail(c, MEPGL_ERR_HIP, std::string("iteration event: ") + hipGetErrorString(e));
+        break;
+      }
+      if (*(volatile int*)c->health) {
+        (void)hipStreamSynchronize(c->stream);
+        return bad_iteration(it - 1);
+      }
+    }
+  }
+  if (rollback) drop_snapshot(c, sn);
+  if (rc) return rc;
+  // The last iteration (checked right after it with rollback).
+  HIP_TRY(c, hipStreamSynchronize(c->stream));
+  if (!rollback && it > 0 && *(volatile int*)c->health) return bad_iteration(it - 1);
+  return check_finite(c, " after the loop");
 }
 
 int mepgl_get_scalars(mepgl_ctx* c, int si, int which, double* out) {

@@ -1,78 +1,109 @@
 // Plain streaming kernels of known size: they calibrate the HBM byte
 // counters (FETCH_SIZE, WRITE_SIZE) that rocprofv3 reports for the
-// relaxation kernels, and measure the bandwidth a simple access mix reaches.
-//   copy_calib [MiB] [repeats] [copy|read|write|triad]
+// relaxation kernels, and measure the bandwidth a simple access mix reaches
+// at a given access width per lane.
+//   copy_calib [MiB] [repeats] [copy|read|write|triad] [bytes per lane]
 // copy: 1 read + 1 write; read: reads only; write: writes only; triad:
 // 3 reads + 1 write per element (the read/write mix of the P kernel).
+// Bytes per lane: 4 (one dword: the mixed-precision stencil kernels' load
+// width with one column per lane), 8 (dwordx2: FP64 with one column per
+// lane, or mixed with two), 16 (dwordx4, the default).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 
-// One 16-byte element per thread over the whole array (the fastest plain
-// copy form measured on MI355X: about 6.3 TB/s read + write for 1 GiB).
-__global__ __launch_bounds__(256) void copy_kernel(const double2* __restrict__ src,
-                                                   double2* __restrict__ dst, long long n) {
+// One element of W bytes per thread over the whole array.
+template <int W>
+struct Elem;
+template <>
+struct Elem<4> {
+  using T = float;
+  __device__ static T val(float a) { return a; }
+  __device__ static T mix(T x, T y, T z) { return x + 0.5f * y - z; }
+  __device__ static bool hit(T v) { return v == 1234.5f; }
+};
+template <>
+struct Elem<8> {
+  using T = double;
+  __device__ static T val(float a) { return a; }
+  __device__ static T mix(T x, T y, T z) { return x + 0.5 * y - z; }
+  __device__ static bool hit(T v) { return v == 1234.5; }
+};
+template <>
+struct Elem<16> {
+  using T = double2;
+  __device__ static T val(float a) { return double2{a, 2.0 * a}; }
+  __device__ static T mix(T x, T y, T z) {
+    return double2{x.x + 0.5 * y.x - z.x, x.y + 0.5 * y.y - z.y};
+  }
+  __device__ static bool hit(T v) { return v.x == 1234.5 && v.y == -1234.5; }
+};
+
+template <int W>
+__global__ __launch_bounds__(256) void copy_kernel(const typename Elem<W>::T* __restrict__ src,
+                                                   typename Elem<W>::T* __restrict__ dst,
+                                                   long long n) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i < n) dst[i] = src[i];
 }
 
-__global__ __launch_bounds__(256) void read_kernel(const double2* __restrict__ src,
+template <int W>
+__global__ __launch_bounds__(256) void read_kernel(const typename Elem<W>::T* __restrict__ src,
                                                    double* __restrict__ sink, long long n) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (i < n) {
-    const double2 v = src[i];
-    if (v.x == 1234.5 && v.y == -1234.5) sink[0] = v.x;  // zero data: never taken
+    const typename Elem<W>::T v = src[i];
+    if (Elem<W>::hit(v)) sink[0] = 1.0;  // zero data: never taken
   }
 }
 
-__global__ __launch_bounds__(256) void write_kernel(double2* __restrict__ dst, long long n) {
+template <int W>
+__global__ __launch_bounds__(256) void write_kernel(typename Elem<W>::T* __restrict__ dst,
+                                                    long long n) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i < n) dst[i] = double2{1.0, 2.0};
+  if (i < n) dst[i] = Elem<W>::val(1.0f);
 }
 
-__global__ __launch_bounds__(256) void triad_kernel(const double2* __restrict__ a,
-                                                    const double2* __restrict__ b,
-                                                    const double2* __restrict__ c,
-                                                    double2* __restrict__ d, long long n) {
+template <int W>
+__global__ __launch_bounds__(256) void triad_kernel(const typename Elem<W>::T* __restrict__ a,
+                                                    const typename Elem<W>::T* __restrict__ b,
+                                                    const typename Elem<W>::T* __restrict__ c,
+                                                    typename Elem<W>::T* __restrict__ d,
+                                                    long long n) {
   const long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (i < n) {
-    const double2 x = a[i], y = b[i], z = c[i];
-    d[i] = double2{x.x + 0.5 * y.x - z.x, x.y + 0.5 * y.y - z.y};
-  }
+  if (i < n) d[i] = Elem<W>::mix(a[i], b[i], c[i]);
 }
 
-int main(int argc, char** argv) {
-  const long long mib = argc > 1 ? atoll(argv[1]) : 1024;
-  const int reps = argc > 2 ? atoi(argv[2]) : 20;
-  const char* mode = argc > 3 ? argv[3] : "copy";
+template <int W>
+static int run(long long mib, int reps, const char* mode) {
+  using T = typename Elem<W>::T;
   const long long bytes = mib << 20;
-  const long long n = bytes / (long long)sizeof(double2);
+  const long long n = bytes / (long long)sizeof(T);
   const bool triad = !strcmp(mode, "triad");
-  double2 *a = nullptr, *b = nullptr, *c = nullptr, *d = nullptr;
+  T *a = nullptr, *b = nullptr, *c = nullptr, *d = nullptr;
   if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&d, bytes) != hipSuccess) return 1;
   if (triad && (hipMalloc(&b, bytes) != hipSuccess || hipMalloc(&c, bytes) != hipSuccess)) return 1;
   if (hipMemset(a, 0, bytes) != hipSuccess || hipMemset(d, 0, bytes) != hipSuccess) return 1;
   if (triad && (hipMemset(b, 0, bytes) != hipSuccess || hipMemset(c, 0, bytes) != hipSuccess))
     return 1;
   hipEvent_t e0, e1;
-  hipEventCreate(&e0);
-  hipEventCreate(&e1);
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return 1;
   const unsigned blocks = (unsigned)((n + 255) / 256);
   double moved = 0;
   auto launch = [&]() {
     if (!strcmp(mode, "read")) {
-      hipLaunchKernelGGL(read_kernel, dim3(blocks), dim3(256), 0, 0, a, (double*)d, n);
+      hipLaunchKernelGGL(read_kernel<W>, dim3(blocks), dim3(256), 0, 0, a, (double*)d, n);
       moved = (double)bytes;
     } else if (!strcmp(mode, "write")) {
-      hipLaunchKernelGGL(write_kernel, dim3(blocks), dim3(256), 0, 0, d, n);
+      hipLaunchKernelGGL(write_kernel<W>, dim3(blocks), dim3(256), 0, 0, d, n);
       moved = (double)bytes;
     } else if (triad) {
-      hipLaunchKernelGGL(triad_kernel, dim3(blocks), dim3(256), 0, 0, a, b, c, d, n);
+      hipLaunchKernelGGL(triad_kernel<W>, dim3(blocks), dim3(256), 0, 0, a, b, c, d, n);
       moved = 4.0 * bytes;
     } else {
-      hipLaunchKernelGGL(copy_kernel, dim3(blocks), dim3(256), 0, 0, a, d, n);
+      hipLaunchKernelGGL(copy_kernel<W>, dim3(blocks), dim3(256), 0, 0, a, d, n);
       moved = 2.0 * bytes;
     }
   };
@@ -84,10 +115,30 @@ int main(int argc, char** argv) {
   float ms = 0;
   hipEventElapsedTime(&ms, e0, e1);
   const double per = ms / reps;
-  printf("%s %lld MiB: %.4f ms per pass, %.1f GB/s\n", mode, mib, per, moved / (per * 1e-3) / 1e9);
+  printf("%s %lld MiB, %d bytes per lane: %.4f ms per pass, %.1f GB/s\n", mode, mib, W, per,
+         moved / (per * 1e-3) / 1e9);
   hipFree(a);
   hipFree(d);
   if (b) hipFree(b);
   if (c) hipFree(c);
   return 0;
+}
+
+int main(int argc, char** argv) {
+  const long long mib = argc > 1 ? atoll(argv[1]) : 1024;
+  const int reps = argc > 2 ? atoi(argv[2]) : 20;
+  const char* mode = argc > 3 ? argv[3] : "copy";
+  const int width = argc > 4 ? atoi(argv[4]) : 16;
+  if (mib <= 0 || mib > 16384 || reps <= 0) {
+    fprintf(stderr, "copy_calib: MiB in 1..16384 and repeats > 0\n");
+    return 2;
+  }
+  switch (width) {
+    case 4: return run<4>(mib, reps, mode);
+    case 8: return run<8>(mib, reps, mode);
+    case 16: return run<16>(mib, reps, mode);
+    default:
+      fprintf(stderr, "copy_calib: bytes per lane must be 4, 8 or 16\n");
+      return 2;
+  }
 }

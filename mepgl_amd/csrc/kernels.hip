@@ -84,6 +84,9 @@ __global__ __launch_bounds__(kFinishPolyBlock) void k_finish_poly(FinishArgs f) 
     sc.poly[3] = E;
     sc.alpha = t.alpha[first_rise];
     sc.ls_index = first_rise;
+    if (!__builtin_isfinite(A) || !__builtin_isfinite(B) || !__builtin_isfinite(C) ||
+        !__builtin_isfinite(D) || !__builtin_isfinite(E))
+      *f.health = 1;
   }
 }
 
@@ -242,6 +245,8 @@ __device__ __forceinline__ ArcBufs arc_bufs(const AuxArgs& a, const DevString& s
   ArcBufs ab;
   ab.b0.x = rsrc_of(X + (long long)n0 * s.image_stride, ibytes);
   ab.b1.x = rsrc_of(X + (long long)t.n * s.image_stride, ibytes);
+  ab.b0.xa = s.a_shared ? rsrc_of(X, ibytes) : ab.b0.x;
+  ab.b1.xa = s.a_shared ? rsrc_of(X, ibytes) : ab.b1.x;
   ab.b0.d = rsrc_of(D + (long long)n0 * s.image_stride, ibytes);
   ab.b1.d = rsrc_of(D + (long long)t.n * s.image_stride, ibytes);
   ab.b1.xo = rsrc_of((const T*)s.x[a.xout] + (long long)t.n * s.image_stride, ibytes);
@@ -581,9 +586,11 @@ __global__ __launch_bounds__(kBlock) void k_redistribute_linear(AuxArgs a) {
   const T* Xk = (const T*)s.x[a.xin] + (long long)k * s.image_stride;
   const T* Xk1 = Xk - s.image_stride;
   T* O = (T*)s.x[a.xout] + (long long)t.n * s.image_stride;
+  // A fixed and shared by every image stays as it is, in both buffers.
+  const int p0 = s.a_shared ? 2 : 0;
   for (int j = t.j0; j < t.j1; ++j) {
     const long long off = (long long)j * s.pitch + t.i;
-    for (int p = 0; p < s.nplanes; ++p) {
+    for (int p = p0; p < s.nplanes; ++p) {
       const long long o = p * s.plane_stride + off;
       O[o] = (T)fma(w, (double)Xk[o], (1 - w) * (double)Xk1[o]);
     }
@@ -596,7 +603,7 @@ template <class T>
 __global__ __launch_bounds__(kBlock) void k_copy_a(AuxArgs a) {
   const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
   const DevString& s = a.strings[it.string];
-  if (!s.nsf) return;
+  if (!s.nsf || s.a_shared) return;
   const Strip t = strip_of(s, it);
   if (t.i >= s.N) return;
   const T* S = (const T*)s.x[a.xout] + (long long)t.n * s.image_stride;
@@ -626,7 +633,8 @@ __global__ __launch_bounds__(kBlock) void k_redistribute_cubic(AuxArgs a, int te
   const Strip t = strip_of(s, it);
   if (t.i >= s.N) return;
   // Everything read from the descriptor is read before the first store.
-  const int F = s.F, NP = s.nplanes;
+  // A fixed and shared by every image is not interpolated (planes from p0).
+  const int F = s.F, p0 = s.a_shared ? 2 : 0, NP = s.nplanes - p0;
   const long long IS = s.image_stride, PS = s.plane_stride, PT = s.pitch;
   const double* cb = s.cubic;
   const int* ck = s.cplan_k;
@@ -636,7 +644,7 @@ __global__ __launch_bounds__(kBlock) void k_redistribute_cubic(AuxArgs a, int te
   double* Dl = (double*)s.cubic_rhs;  // in the fields' layout, in double
   const int pairs = (t.j1 - t.j0) * NP;
   for (int q = t.n; q < pairs; q += F) {
-    const int j = t.j0 + q / NP, p = q % NP;
+    const int j = t.j0 + q / NP, p = p0 + q % NP;
     const long long off = p * PS + (long long)j * PT + t.i;
     const T* Xp = X + off;
     auto f = [&](int n) { return (double)Xp[(long long)n * IS]; };
@@ -851,12 +859,21 @@ __global__ void k_nn_map(const int* sc_dom, const int* comp_dom, int N, int pitc
 // Per-image scalar maintenance (graph-safe, no memset nodes).
 __global__ void k_reset_norms(AuxArgs a, int nstrings, int interior_only) {
   const int si = blockIdx.x;
-  if (si >= nstrings) return;
+  if (si >= nstrings || (a.only_string >= 0 && si != a.only_string)) return;
   const DevString& s = a.strings[si];
   for (int n = threadIdx.x; n < s.F; n += blockDim.x) {
     if (interior_only && (n == 0 || n == s.F - 1)) continue;
-    a.sc[s.img0 + n].g_norm2[0] = INFINITY;
-    a.sc[s.img0 + n].g_norm2[1] = INFINITY;
+    ImageScalars& x = a.sc[s.img0 + n];
+    x.g_norm2[0] = INFINITY;
+    x.g_norm2[1] = INFINITY;
+    if (!interior_only) {
+      // A full reset (loop()'s initial state) also clears the last step's
+      // totals and line-search coefficients.
+      x.g_delta_g = 0;
+      x.beta = 0;
+      x.fenergy = 0;
+      for (int k = 0; k < 4; ++k) x.poly[k] = 0;
+    }
   }
 }
 

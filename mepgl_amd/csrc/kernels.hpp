@@ -44,6 +44,7 @@ struct StepArgs {
   int scratch;      // G writes the diagnostic gradient buffer only
   int nc, nsf;      // class of every item in `items` (components, fixed A)
   int pl;           // ... and whether its strings have parameter planes
+  int* health;      // set to 1 when a step's gradient totals are not finite
 };
 
 // Arguments of the auxiliary launches (energy, JB, arclength,
@@ -62,6 +63,7 @@ struct AuxArgs {
                         // gradient norms (beta = 0 on the next step)
   double* out;          // per global image (energy)
   double* const* jb;    // per string JB planes (device array of pointers)
+  int only_string;      // scalar maintenance: string index, or -1 for all
 };
 
 // Arguments of the per-image finishing launches (one wavefront per image).
@@ -73,6 +75,7 @@ struct FinishArgs {
   const LineSearchTable* ls;
   double* out;
   int xin;           // current field buffer (winding)
+  int* health;       // set to 1 when a line-search quartic is not finite
 };
 
 hipError_t launch_step(const StepArgs& a, int which, bool fp32, hipStream_t st);

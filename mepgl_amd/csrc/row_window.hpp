@@ -25,9 +25,11 @@ struct GRow {
   uint16_t nn;
 };
 
-// Buffers of one work item of the G kernel or the arclength pass.
+// Buffers of one work item of the G kernel or the arclength pass.  xa is
+// where the A planes are read: the image's own field set, or image 0's for a
+// string whose fixed A is shared by every image.
 struct GBufs {
-  Rsrc x, xo, d, g, go, nn;
+  Rsrc x, xa, xo, d, g, go, nn;
   unsigned ps;  // plane stride in bytes
 };
 
@@ -37,7 +39,7 @@ __device__ __forceinline__ void g_load(GRaw<T, NC>& r, const GBufs& b, int row, 
   const RowOff o = row_off<T>(row, P_, t);
 #pragma unroll
   for (int p = 0; p < GRaw<T, NC>::P; ++p) {
-    r.x[p] = bload<T>(b.x, o.own, p * b.ps);
+    r.x[p] = bload<T>(p < 2 ? b.xa : b.x, o.own, p * b.ps);
     r.d[p] = (UPD && !(NSF && p < 2)) ? bload<T>(b.d, o.own, p * b.ps) : (T)0;
   }
   // Halo columns: only lanes 0 and 63 read the columns outside the strip
@@ -47,7 +49,7 @@ __device__ __forceinline__ void g_load(GRaw<T, NC>& r, const GBufs& b, int row, 
   if (t.edge) {
 #pragma unroll
     for (int p = 0; p < GRaw<T, NC>::P; ++p) {
-      r.hx[p] = bload<T>(b.x, o.halo, p * b.ps);
+      r.hx[p] = bload<T>(p < 2 ? b.xa : b.x, o.halo, p * b.ps);
       r.hd[p] = (UPD && !(NSF && p < 2)) ? bload<T>(b.d, o.halo, p * b.ps) : (T)0;
     }
   }

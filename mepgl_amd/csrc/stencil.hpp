@@ -16,11 +16,32 @@ __device__ __forceinline__ int bitof(int m, int k) { return (m >> k) & 1; }
 template <class T>
 __device__ __forceinline__ T sq(T v) { return v * v; }
 
-// Quadrant area weight (S0S1S2 + S2S3S4 + S4S5S6 + S6S7S0)/4, nlcg.cu:140-143.
-__device__ __forceinline__ int quad_count(int m) {
-  return bitof(m, 0) * bitof(m, 1) * bitof(m, 2) + bitof(m, 2) * bitof(m, 3) * bitof(m, 4) +
-         bitof(m, 4) * bitof(m, 5) * bitof(m, 6) + bitof(m, 6) * bitof(m, 7) * bitof(m, 0);
+// Neighbour-pair and quadrant presence of an 8-bit neighbour mask, as bit
+// masks: bit k of pair_bits is set when neighbours k and k + 1 (mod 8) are
+// both present; bit 2q of quad_bits when quadrant q (neighbours 2q, 2q + 1,
+// 2q + 2 mod 8) is complete.  Weights and flags below are popcounts and bits
+// of these, so they are the same small integers as the products of single
+// bits they replace.
+__device__ __forceinline__ unsigned pair_bits(int m) {
+  const unsigned w = (unsigned)m | ((unsigned)m << 8);
+  return w & (w >> 1);
 }
+__device__ __forceinline__ unsigned quad_bits(int m) {
+  const unsigned p = pair_bits(m);
+  return p & (p >> 1) & 0x55u;
+}
+__device__ __forceinline__ int popc(unsigned v) { return __builtin_popcount(v); }
+
+// Quadrant area weight (S0S1S2 + S2S3S4 + S4S5S6 + S6S7S0)/4, nlcg.cu:140-143.
+__device__ __forceinline__ int quad_count(int m) { return popc(quad_bits(m)); }
+
+// Bond weights of nlcg.cu:188-189 and 552-571 as pair counts:
+// (S1S2 + S6S7)/2 (forward x), (S2S3 + S5S6)/2 (backward x),
+// (S0S1 + S3S4)/2 (forward y), (S4S5 + S7S0)/2 (backward y).
+__device__ __forceinline__ int wx_fwd(unsigned p) { return popc(p & 0x42u); }
+__device__ __forceinline__ int wx_bwd(unsigned p) { return popc(p & 0x24u); }
+__device__ __forceinline__ int wy_fwd(unsigned p) { return popc(p & 0x09u); }
+__device__ __forceinline__ int wy_bwd(unsigned p) { return popc(p & 0x90u); }
 
 // Per-site constants of one component.
 template <class T>
@@ -163,11 +184,12 @@ template <class T, int NC>
 __device__ __forceinline__ void site_gradient(const Nbhd<T, NC>& s, const CompConst<T> (&k)[NC],
                                               T eta, T gamma, T delta, T idx, T dx2, int sc,
                                               int cm, bool do_a, T (&gpsi)[NC][2], T (&ga)[2]) {
+  const unsigned sp = pair_bits(sc);
   const T af = (T)quad_count(sc) * (T)0.25;
-  const T wxf = (T)(bitof(sc, 1) * bitof(sc, 2) + bitof(sc, 6) * bitof(sc, 7)) * (T)0.5;
-  const T wxb = (T)(bitof(sc, 2) * bitof(sc, 3) + bitof(sc, 5) * bitof(sc, 6)) * (T)0.5;
-  const T wyf = (T)(bitof(sc, 0) * bitof(sc, 1) + bitof(sc, 3) * bitof(sc, 4)) * (T)0.5;
-  const T wyb = (T)(bitof(sc, 4) * bitof(sc, 5) + bitof(sc, 7) * bitof(sc, 0)) * (T)0.5;
+  const T wxf = (T)wx_fwd(sp) * (T)0.5;
+  const T wxb = (T)wx_bwd(sp) * (T)0.5;
+  const T wyf = (T)wy_fwd(sp) * (T)0.5;
+  const T wyb = (T)wy_bwd(sp) * (T)0.5;
   const T AX0 = s.ax[1][1], AY0 = s.ay[1][1];
   T gax = 0, gay = 0;
 
@@ -236,10 +258,11 @@ __device__ __forceinline__ void site_gradient(const Nbhd<T, NC>& s, const CompCo
     const T dxb = (X[1][1] - X[0][1]) * idx;
     const T h0 = s.h[0];
     // Quadrant presence folded into the quadrant's weight, as for the bonds.
-    const T c0 = bitof(cm, 0) && bitof(cm, 1) && bitof(cm, 2) ? c4 : (T)0;
-    const T c1 = bitof(cm, 2) && bitof(cm, 3) && bitof(cm, 4) ? c4 : (T)0;
-    const T c2 = bitof(cm, 4) && bitof(cm, 5) && bitof(cm, 6) ? c4 : (T)0;
-    const T c3 = bitof(cm, 6) && bitof(cm, 7) && bitof(cm, 0) ? c4 : (T)0;
+    const unsigned cq = quad_bits(cm);
+    const T c0 = (cq & 0x01u) ? c4 : (T)0;
+    const T c1 = (cq & 0x04u) ? c4 : (T)0;
+    const T c2 = (cq & 0x10u) ? c4 : (T)0;
+    const T c3 = (cq & 0x40u) ? c4 : (T)0;
     {
       const T e = dyf - dxf - h0;
       const T n1 = dyf - (X[2][2] - X[1][2]) * idx - s.h[1];
@@ -387,10 +410,11 @@ __device__ __forceinline__ Quartic<T> site_poly(const PolyNbhd<T, NC>& s,
     const T dxf = (s.dax[3] - s.dax[0]) * idx, dxb = (s.dax[0] - s.dax[4]) * idx;
     // Quadrant presence folded into the quadrant's weight (absent: 0).
     T fq[4];
-    fq[0] = bitof(cm, 0) && bitof(cm, 1) && bitof(cm, 2) ? f : (T)0;
-    fq[1] = bitof(cm, 2) && bitof(cm, 3) && bitof(cm, 4) ? f : (T)0;
-    fq[2] = bitof(cm, 4) && bitof(cm, 5) && bitof(cm, 6) ? f : (T)0;
-    fq[3] = bitof(cm, 6) && bitof(cm, 7) && bitof(cm, 0) ? f : (T)0;
+    const unsigned cq = quad_bits(cm);
+    fq[0] = (cq & 0x01u) ? f : (T)0;
+    fq[1] = (cq & 0x04u) ? f : (T)0;
+    fq[2] = (cq & 0x10u) ? f : (T)0;
+    fq[3] = (cq & 0x40u) ? f : (T)0;
     T e0[4], e1[4];
     e0[0] = yf - xf - s.h0;  e1[0] = dyf - dxf;
     e0[1] = yb - xf - s.h0;  e1[1] = dyb - dxf;
@@ -404,8 +428,9 @@ __device__ __forceinline__ Quartic<T> site_poly(const PolyNbhd<T, NC>& s,
     }
   }
   // Kinetic bonds, nlcg.cu:1160-1366.
-  const T cxf = (T)(bitof(sc, 1) * bitof(sc, 2) + bitof(sc, 6) * bitof(sc, 7)) * (T)0.5;
-  const T cyf = (T)(bitof(sc, 0) * bitof(sc, 1) + bitof(sc, 3) * bitof(sc, 4)) * (T)0.5;
+  const unsigned sp = pair_bits(sc);
+  const T cxf = (T)wx_fwd(sp) * (T)0.5;
+  const T cyf = (T)wy_fwd(sp) * (T)0.5;
   const T c2y = exact ? (T)1 : (T)0.5;
   // Bond presence (bits 0 and 2) folded into the bond weights (absent: 0).
   const T bx = bitof(sc, 0) ? (T)1 : (T)0, by = bitof(sc, 2) ? (T)1 : (T)0;

@@ -91,6 +91,7 @@ __device__ __forceinline__ void grad_body(const StepArgs& a, const DevString& s,
   const long long ib = (long long)t.n * s.image_stride;
   GBufs b;
   b.x = rsrc_of((const T*)s.x[a.xin] + ib, ibytes);
+  b.xa = NSF && s.a_shared ? rsrc_of(s.x[a.xin], ibytes) : b.x;
   b.xo = rsrc_of((const T*)s.x[a.xout] + ib, ibytes);
   b.d = rsrc_of((const T*)s.d[a.din] + ib, ibytes);
   b.g = rsrc_of((const T*)(a.scratch ? s.gs : s.g[a.gcur]) + ib, ibytes);
@@ -332,7 +333,7 @@ struct PRow {
 };
 
 struct PBufs {
-  Rsrc x, g, d, dout, nn;
+  Rsrc x, xa, g, d, dout, nn;  // xa: where the A planes are read (GBufs)
   unsigned ps;
 };
 
@@ -343,7 +344,7 @@ __device__ __forceinline__ void p_load(PRaw<T, NC>& r, const PBufs& b, int row, 
 #pragma unroll
   for (int p = 0; p < PRaw<T, NC>::P; ++p) {
     const bool dir = !(NSF && p < 2);
-    r.x[p] = bload<T>(b.x, o.own, p * b.ps);
+    r.x[p] = bload<T>(p < 2 ? b.xa : b.x, o.own, p * b.ps);
     r.g[p] = dir ? bload<T>(b.g, o.own, p * b.ps) : (T)0;
     r.d[p] = dir ? bload<T>(b.d, o.own, p * b.ps) : (T)0;
   }
@@ -353,7 +354,7 @@ __device__ __forceinline__ void p_load(PRaw<T, NC>& r, const PBufs& b, int row, 
 #pragma unroll
     for (int p = 0; p < PRaw<T, NC>::P; ++p) {
       const bool dir = !(NSF && p < 2);
-      r.hx[p] = bload<T>(b.x, o.halo, p * b.ps);
+      r.hx[p] = bload<T>(p < 2 ? b.xa : b.x, o.halo, p * b.ps);
       r.hg[p] = dir ? bload<T>(b.g, o.halo, p * b.ps) : (T)0;
       r.hd[p] = dir ? bload<T>(b.d, o.halo, p * b.ps) : (T)0;
     }
@@ -412,6 +413,7 @@ __device__ __forceinline__ void poly_body(const StepArgs& a, const DevString& s,
   const long long ib = (long long)t.n * s.image_stride;
   PBufs b;
   b.x = rsrc_of((const T*)s.x[a.xin] + ib, ibytes);
+  b.xa = NSF && s.a_shared ? rsrc_of(s.x[a.xin], ibytes) : b.x;
   b.g = rsrc_of((const T*)s.g[a.gcur] + ib, ibytes);
   b.d = rsrc_of((const T*)s.d[a.din] + ib, ibytes);
   b.dout = rsrc_of((const T*)s.d[a.dout] + ib, ibytes);
@@ -430,6 +432,18 @@ __device__ __forceinline__ void poly_body(const StepArgs& a, const DevString& s,
     gdelta = fin[1];
     beta_d = a.nlcg ? fmax(gdelta / scp->g_norm2[a.sold], 0.0) : 0.0;
   };
+  auto nnoff = [&](int row) { return (unsigned)((row * P_ + t.ic) * (int)sizeof(uint16_t)); };
+  // The march is the only part that depends on the direction
+  // (marches_up).  Its first row is loaded before beta is totalled, so that
+  // its latency overlaps the round trip of the partials' loads (a second
+  // early row would cost the FP64 kernels half their occupancy).
+  const bool up = marches_up(s, it);
+  const bool bonds = !(it.flags & kItemNoBonds);
+  PRaw<T, NC> pre0;
+  if (bonds) {
+    const int jb0 = up ? t.j1 - 1 : t.j0, d0 = up ? -1 : 1;
+    p_load<T, NC, NSF>(pre0, b, clampr(jb0 - d0, N), P_, t, nnoff(clampr(jb0 - d0, N)));
+  }
   double gnorm2, gdelta, beta_d;
   totals(gnorm2, gdelta, beta_d);
   const T beta = uniform_value((T)beta_d);
@@ -444,13 +458,8 @@ __device__ __forceinline__ void poly_body(const StepArgs& a, const DevString& s,
   gptr<const double> __restrict__ hp = as_global(s.h.p);
   const T hv = (T)s.h.v;
   double acc_s[5] = {0, 0, 0, 0, 0}, acc_c[5] = {0, 0, 0, 0, 0};
-  auto nnoff = [&](int row) { return (unsigned)((row * P_ + t.ic) * (int)sizeof(uint16_t)); };
-
-  // The march is the only part that depends on the direction
-  // (marches_up); everything it needs from the string's descriptor is
-  // read above, before the branch and before any store, so it stays in
-  // scalar registers.
-  const bool up = marches_up(s, it);
+  // Everything the march needs from the string's descriptor is read above,
+  // before the branch and before any store, so it stays in scalar registers.
   using S0 = std::integral_constant<int, 0>;
   using S1 = std::integral_constant<int, 1>;
   using S2 = std::integral_constant<int, 2>;
@@ -465,8 +474,7 @@ __device__ __forceinline__ void poly_body(const StepArgs& a, const DevString& s,
     PRaw<T, NC> raw;
     PRow<T, NC> w[3];
     RowParams<T, NC, PL> rp[3];
-    p_load<T, NC, NSF>(raw, b, clampr(jb - D, N), P_, t, nnoff(clampr(jb - D, N)));
-    p_process<T, NC>(w[0], raw, beta);
+    p_process<T, NC>(w[0], pre0, beta);  // row jb - D
     p_load<T, NC, NSF>(raw, b, clampr(jb, N), P_, t, nnoff(clampr(jb, N)));
     p_process<T, NC>(w[1], raw, beta);
     p_load<T, NC, NSF>(raw, b, clampr(jb + D, N), P_, t, nnoff(clampr(jb + D, N)));
@@ -583,7 +591,7 @@ __device__ __forceinline__ void poly_body(const StepArgs& a, const DevString& s,
   };
   // Tiles without bonds: the new direction is -0 + beta 0 = 0, which the
   // other direction buffer already holds, and every quartic term is zero.
-  if (!(it.flags & kItemNoBonds)) {
+  if (bonds) {
     if (up)
       march(std::true_type());
     else
@@ -601,6 +609,9 @@ __device__ __forceinline__ void poly_body(const StepArgs& a, const DevString& s,
       scp->g_norm2[a.scur] = gnorm2;
       scp->g_delta_g = gdelta;
       scp->beta = beta_d;
+      // A gradient total that is not finite (NaN or +-Inf) marks the step;
+      // the host reads the flag once per outer iteration.
+      if (!__builtin_isfinite(gnorm2) || !__builtin_isfinite(gdelta)) *a.health = 1;
     }
   }
 }

@@ -29,13 +29,15 @@ class Solver:
     precision: "fp64" (the reference's DOUBLE_PRECISION arithmetic) or
     "mixed" (float fields and stencils, double reductions and scalars).
     poly_exact / cubic_textbook switch off the reference's line-search
-    quartic and spline-weight quirks.
+    quartic and spline-weight quirks.  rollback: loop() restores the state
+    from the start of an outer iteration that produced a NaN or Inf.
     """
 
     def __init__(self, strings: Sequence[GLString], precision: str = "fp64",
                  poly_exact: bool = False, cubic_textbook: bool = False,
                  use_graph: bool = True, device: int = 0, segment_rows: int = 0,
-                 gauge_align: bool = False):
+                 gauge_align: bool = False, rollback: bool = False,
+                 gauge_align_local: bool = False):
         self._L = _lib.load()
         self.strings: List[GLString] = [s.validate() for s in strings]
         self.precision = precision
@@ -43,7 +45,9 @@ class Solver:
         flags = (_lib.POLY_EXACT if poly_exact else 0) | \
                 (_lib.CUBIC_TEXTBOOK if cubic_textbook else 0) | \
                 (0 if use_graph else _lib.NO_GRAPH) | \
-                (_lib.GAUGE_ALIGN if gauge_align else 0)
+                (_lib.GAUGE_ALIGN if gauge_align else 0) | \
+                (_lib.ROLLBACK if rollback else 0) | \
+                (_lib.GAUGE_ALIGN_LOCAL if gauge_align_local else 0)
         self._keep = []
         descs = (_lib.StringDesc * len(self.strings))()
         for d, s in zip(descs, self.strings):
@@ -254,6 +258,12 @@ class Solver:
         applied to each string's images."""
         self._check(self._L.mepgl_gauge_align(self._ctx))
         return [self.scalars(i, "theta") for i in range(len(self.strings))]
+
+    def gauge_align_local(self):
+        """Local gauge alignment of neighbouring images (fields, and the psi
+        parts of the direction and gradients); strings with a fixed A are
+        left unchanged."""
+        self._check(self._L.mepgl_gauge_align_local(self._ctx))
 
     def reset_interior_beta(self):
         """mepgl.cu:231-234."""

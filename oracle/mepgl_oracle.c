@@ -41,6 +41,12 @@
  *                          closed-form offset on quadratic data
  *   or_gauge_align         removes a known constant phase between images;
  *                          energy, s_jb unchanged; no neighbour distance grows
+ *   or_gauge_align_local   the normal equations of its least-squares problem
+ *                          hold; xi equals a sparse direct solve of the
+ *                          Neumann Laplacian; a pure-gauge difference between
+ *                          images is removed; plaquette curls and |psi| are
+ *                          unchanged; no neighbour A distance grows; the
+ *                          energy changes at O(dx^2)
  */
 #include "mepgl_oracle.h"
 
@@ -933,4 +939,117 @@ void or_gauge_align(int N, int F, int multicomponent, double *psi_1, double *psi
             }
         }
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Local gauge alignment of neighbouring images (no reference function).
+ *
+ * For image n >= 1, xi_n minimises |A_n + D xi - A_{n-1}|^2 over the grid,
+ * where D is the forward difference (D_x xi(i,j) = (xi(i+1,j) - xi(i,j))/dx
+ * for i < N-1, 0 in the last column; D_y likewise).  The normal equations
+ * are L xi = dx^2 D^T (A_{n-1} - A_n) with L the Neumann graph Laplacian of
+ * the grid (unit spacing), solved exactly through its eigenbasis, the
+ * orthonormal DCT-II: C[i][k] = w_k cos(pi k (i + 1/2) / N), eigenvalues
+ * lam_k = 2 - 2 cos(pi k / N), xi = C ((C^T b C) / (lam_l + lam_k)) C^T with
+ * the constant mode set to zero.  The transform accumulates along the string,
+ * chi_n = chi_{n-1} + xi_n (chi_0 = 0), and is applied as
+ *   A_n <- A_n + D chi_n,  psi_a,n <- psi_a,n exp(i q_a chi_n),
+ * the directions and gradients of psi rotating with psi (those of A do not
+ * change).  A_n - A_{n-1} after the transform is the least-squares residual
+ * of the pair, so no neighbour A distance grows; the forward curl of every
+ * quadrant-0 plaquette is unchanged.  chi receives chi_n ([n][j][i]). */
+static void dct_matrix(int N, double *C, double *lam)
+{
+    const double pi = 3.14159265358979323846;
+    for (int i = 0; i < N; i++)
+        for (int k = 0; k < N; k++) {
+            const double w = k == 0 ? sqrt(1.0 / N) : sqrt(2.0 / N);
+            C[(size_t)i * N + k] = w * cos(pi * k * (i + 0.5) / N);
+        }
+    for (int k = 0; k < N; k++) lam[k] = 2.0 - 2.0 * cos(pi * k / N);
+}
+
+/* out[m][n] = sum_k A(m,k) B(k,n), with A(m,k) = a[m*ra + k*ca] and
+ * B(k,n) = b[k*rb + n*cb]; k in increasing order. */
+static void matmul(int N, const double *a, long ra, long ca, const double *b, long rb, long cb,
+                   double *out)
+{
+    for (int m = 0; m < N; m++)
+        for (int n = 0; n < N; n++) {
+            double acc = 0;
+            for (int k = 0; k < N; k++) acc += a[m * ra + k * ca] * b[k * rb + n * cb];
+            out[(size_t)m * N + n] = acc;
+        }
+}
+
+void or_gauge_align_local(int N, int F, int multicomponent, double dx, double q1, double q2,
+                          double *a, double *psi_1, double *psi_2, double *const *rot_1,
+                          double *const *rot_2, int nrot, double *chi)
+{
+    const size_t NN = (size_t)N * N;
+    double *C = (double *)malloc(NN * sizeof(double));
+    double *lam = (double *)malloc((size_t)N * sizeof(double));
+    dct_matrix(N, C, lam);
+    memset(chi, 0, NN * sizeof(double));
+#pragma omp parallel for schedule(static)
+    for (int n = 1; n < F; n++) {
+        double *b = (double *)malloc(NN * sizeof(double));
+        double *t = (double *)malloc(NN * sizeof(double));
+        const double *A0 = a + 2 * NN * (n - 1), *A1 = a + 2 * NN * n;
+        /* b = dx^2 D^T R, R = A_{n-1} - A_n: for a site, its x-bond to the
+         * left and its y-bond below enter with +R/dx, its own bonds (to the
+         * right, above) with -R/dx. */
+        for (int j = 0; j < N; j++)
+            for (int i = 0; i < N; i++) {
+                double v = 0;
+                const size_t k = (size_t)j * N + i;
+                if (i >= 1) v += A0[2 * (k - 1)] - A1[2 * (k - 1)];
+                if (i <= N - 2) v -= A0[2 * k] - A1[2 * k];
+                if (j >= 1) v += A0[2 * (k - N) + 1] - A1[2 * (k - N) + 1];
+                if (j <= N - 2) v -= A0[2 * k + 1] - A1[2 * k + 1];
+                b[k] = dx * v;
+            }
+        matmul(N, b, N, 1, C, N, 1, t);  /* b C */
+        matmul(N, C, 1, N, t, N, 1, b);  /* C^T (b C) */
+        for (int l = 0; l < N; l++)
+            for (int k = 0; k < N; k++) {
+                const size_t o = (size_t)l * N + k;
+                b[o] = (l == 0 && k == 0) ? 0.0 : b[o] / (lam[l] + lam[k]);
+            }
+        matmul(N, C, N, 1, b, N, 1, t);  /* C xi^ */
+        matmul(N, t, N, 1, C, 1, N, chi + NN * n);  /* (C xi^) C^T = xi_n */
+        free(b);
+        free(t);
+    }
+    for (int n = 1; n < F; n++)
+        for (size_t k = 0; k < NN; k++) chi[NN * n + k] += chi[NN * (n - 1) + k];
+#pragma omp parallel for schedule(static)
+    for (int n = 1; n < F; n++) {
+        const double *X = chi + NN * n;
+        double *An = a + 2 * NN * n;
+        for (int j = 0; j < N; j++)
+            for (int i = 0; i < N; i++) {
+                const size_t k = (size_t)j * N + i;
+                if (i <= N - 2) An[2 * k] += (X[k + 1] - X[k]) / dx;
+                if (j <= N - 2) An[2 * k + 1] += (X[k + N] - X[k]) / dx;
+            }
+        for (int c = 0; c < 1 + (multicomponent ? 1 : 0); c++) {
+            const double q = c == 0 ? q1 : q2;
+            for (int r = -1; r < nrot; r++) {
+                /* r = -1: the field; then the state arrays that rotate with
+                 * it (directions, gradients). */
+                double *p = r < 0 ? (c == 0 ? psi_1 : psi_2) : (c == 0 ? rot_1[r] : rot_2[r]);
+                if (!p) continue;
+                p += 2 * NN * n;
+                for (size_t k = 0; k < NN; k++) {
+                    const double th = q * X[k], cs = cos(th), sn = sin(th);
+                    const double u = p[2 * k], v = p[2 * k + 1];
+                    p[2 * k] = u * cs - v * sn;
+                    p[2 * k + 1] = u * sn + v * cs;
+                }
+            }
+        }
+    }
+    free(C);
+    free(lam);
 }

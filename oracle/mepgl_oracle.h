@@ -94,6 +94,11 @@ void or_gauge_align(int N, int F, int multicomponent, double *psi_1, double *psi
 void or_thomas(int n, const double *dl, const double *d, const double *du,
                double *rhs);
 
+void or_gauge_align_local(int N, int F, int multicomponent, double dx, double q1,
+                          double q2, double *a, double *psi_1, double *psi_2,
+                          double *const *rot_1, double *const *rot_2, int nrot,
+                          double *chi);
+
 #ifdef __cplusplus
 }
 #endif

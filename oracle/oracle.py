@@ -236,6 +236,26 @@ def gauge_align(N, F, multicomponent, p1, p2):
     return p1, p2, th
 
 
+def gauge_align_local(N, F, multicomponent, dx, q1, q2, a, p1, p2, rot_1=(), rot_2=()):
+    """Local gauge alignment (or_gauge_align_local) on copies; returns
+    (a, psi_1, psi_2, chi, rotated state arrays of component 1, of
+    component 2).  chi is (F, N, N)."""
+    a = _f64(a).copy()
+    p1 = _f64(p1).copy()
+    p2 = _f64(p2).copy() if multicomponent else None
+    r1 = [_f64(r).copy() for r in rot_1]
+    r2 = [_f64(r).copy() for r in rot_2] if multicomponent else [None] * len(r1)
+    chi = np.zeros((F, N, N))
+    n = len(r1)
+    arr1 = (_dp * max(n, 1))(*[_p(r) for r in r1])
+    arr2 = (_dp * max(n, 1))(*[(_p(r) if r is not None else None) for r in r2])
+    lib().or_gauge_align_local(C.c_int(N), C.c_int(F), C.c_int(int(multicomponent)),
+                               C.c_double(dx), C.c_double(q1), C.c_double(q2), _p(a), _p(p1),
+                               _p(p2) if p2 is not None else None, arr1, arr2, C.c_int(n),
+                               _p(chi))
+    return a, p1, p2, chi, r1, r2
+
+
 def thomas(dl, d, du, rhs):
     x = _f64(rhs).copy()
     lib().or_thomas(C.c_int(len(x)), _p(_f64(dl)), _p(_f64(d)), _p(_f64(du)), _p(x))
@@ -351,20 +371,44 @@ class OracleString:
             rotate(arr)
         return th
 
+    def gauge_align_local(self):
+        """Local gauge alignment of neighbouring images (or_gauge_align_local):
+        fields, and the psi parts of the direction and both gradient
+        buffers.  Strings with a fixed A (no self field) are left as they
+        are.  Returns chi (F, N, N)."""
+        if self.prob.no_self_field:
+            return np.zeros((self.F, self.N, self.N))
+        mc = self.multicomponent
+        r1 = [self.d_p1, self.g_p1[0], self.g_p1[1]]
+        r2 = [self.d_p2, self.g_p2[0], self.g_p2[1]] if mc else []
+        a, p1, p2, chi, r1, r2 = gauge_align_local(
+            self.N, self.F, mc, self.prob.dx, self.prob.q_1, self.prob.q_2, self.a, self.psi_1,
+            self.psi_2 if mc else None, r1, r2)
+        self.a, self.psi_1 = a, p1
+        self.d_p1, self.g_p1[0], self.g_p1[1] = r1
+        if mc:
+            self.psi_2 = p2
+            self.d_p2, self.g_p2[0], self.g_p2[1] = r2
+        return chi
+
     def reset_interior_beta(self):
         """mepgl.cu:231-234."""
         self.g_norm2[0][1:-1] = np.inf
         self.g_norm2[1][1:-1] = np.inf
 
-    def loop(self, iterations, relaxation_steps, mode="L", nlcg=True, align=False):
+    def loop(self, iterations, relaxation_steps, mode="L", nlcg=True, align=False,
+             align_local=False):
         """The outer loop of mepgl.cu:148-360 (without printing); align adds
-        the optional constant-phase alignment before each arclength."""
+        the optional constant-phase alignment before each arclength, and
+        align_local the local gauge alignment before that."""
         for _ in range(iterations):
             if mode == "S":
                 self.nlcg_steps(relaxation_steps, nlcg, False)
             elif mode == "M":
                 self.nlcg_steps(relaxation_steps, nlcg, True)
             elif mode in ("L", "C"):
+                if align_local:
+                    self.gauge_align_local()
                 if align:
                     self.gauge_align()
                 self.arc_lengths()

@@ -55,3 +55,33 @@ def test_bench_refuses_more_than_one_gpu():
     assert r.returncode != 0
     line = json.loads(r.stdout.strip().splitlines()[-1])
     assert "one GPU" in line["error"]
+
+
+@pytest.mark.parametrize("N,ncomp,precision", [(8200, 1, 0), (6700, 2, 0), (9500, 2, 1),
+                                               (11700, 1, 1)])
+def test_create_rejects_images_above_2GiB(N, ncomp, precision):
+    """An image's field set must fit the 32-bit byte offsets of the step
+    kernels' buffer accesses: mepgl_create rejects larger strings with
+    MEPGL_ERR_ARG before touching the device (so this runs without a GPU)."""
+    import numpy as np
+
+    from mepgl_amd import _lib
+
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libmepgl.so not built (run __graft_entry__.build())")
+    elem = 4 if precision == 1 else 8
+    assert (N + 1) * (2 + 2 * ncomp) * ((N + 63) // 64 * 64) * elem > 2 ** 31 - 1
+    L = _lib.load()
+    d = _lib.StringDesc()
+    d.N, d.F, d.ncomp, d.dx = N, 2, ncomp, 0.1
+    for name in ("a_1", "a_2"):
+        getattr(d, name).value = -1.0
+    for name in ("b_1", "m_xx_1", "m_yy_1", "b_2", "m_xx_2", "m_yy_2"):
+        getattr(d, name).value = 1.0
+    dom = np.ones(4, dtype=np.int32)  # never read: the size check comes first
+    d.comp_domain = dom.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    d.sc_domain = dom.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+    ctx = ctypes.c_void_p()
+    rc = L.mepgl_create(ctypes.byref(d), 1, precision, 0, 0, ctypes.byref(ctx))
+    assert rc == _lib.ERR_ARG
+    assert not ctx.value

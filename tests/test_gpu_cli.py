@@ -194,3 +194,56 @@ def test_cli_stdin_keys(tmp_path):
     assert i3 < i4 < out.rindex("relaxation_steps = 3")
     assert "NLCG set OFF" in out
     assert "I am quitting! Please wait." in out
+
+
+def test_run_py_workflow_matches_oracle(tmp_path, oracle):
+    """python -m mepgl_amd.run on a reference-style config.py: two stages
+    (5 images, 2 outer iterations of mode L, resampled to 7; then 7 images,
+    2 iterations), then a reload-and-continue run (-rc) of the second stage,
+    against the oracle running the same sequence; and the .npz summary."""
+    from test_run_workflow import write_config
+
+    from mepgl_amd import npyio, observables
+
+    cfgp = write_config(tmp_path, dp=True, mc=True)
+    say("[gpu] run.py workflow: 2 stages, then -rc")
+    r = subprocess.run([sys.executable, "-m", "mepgl_amd.run", "--config", cfgp, "--root",
+                        str(tmp_path)], cwd=ROOT, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    sim = os.path.join(tmp_path, "simulations", "wf")
+    assert os.path.exists(os.path.join(sim, "config.py"))
+    s0, _ = npyio.read_dir(os.path.join(sim, "input_data"))  # stage 2's input (reloaded)
+    assert s0.F == 7
+    out, extra = npyio.read_dir(os.path.join(sim, "output_data"))
+
+    # The oracle: stage 1 from the config's initial guess.
+    from mepgl_amd import run as runmod
+
+    cfg = runmod.load_config(cfgp)
+    init_dir = os.path.join(tmp_path, "init")
+    os.makedirs(init_dir)
+    runmod.write_input(cfg, init_dir)
+    s_init, _ = npyio.read_dir(os.path.join(init_dir, "simulations", "wf", "input_data"))
+    ref = oracle.OracleString(s_init)
+    ref.loop(2, 4, "L")
+    ref.arc_lengths()
+    ref.redistribute(F_new=7)
+    assert normwise(s0.psi_1, ref.psi_1) < 1e-10
+    # Stage 2 restarts the solver from the reloaded files.
+    ref2 = oracle.OracleString(s0)
+    ref2.loop(2, 4, "L")
+    assert normwise(out.psi_1, ref2.psi_1) < 1e-10
+    assert normwise(out.psi_2, ref2.psi_2) < 1e-10
+    assert normwise(out.a, ref2.a) < 1e-10
+    assert rel_err(extra["fenergy"], ref2.free_energy()) < 1e-11
+    z = np.load(os.path.join(sim, "wf.npz"), allow_pickle=True)
+    np.testing.assert_array_equal(z["fenergy"], extra["fenergy"])
+    b = observables.barriers(extra["fenergy"])
+    assert f"{b[0]:.8g}" in r.stdout
+
+    # -rc: the output becomes the input and every stage runs again from it.
+    r = subprocess.run([sys.executable, "-m", "mepgl_amd.run", "-rc", "--no-post", "--config",
+                        cfgp, "--root", str(tmp_path)], cwd=ROOT, capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stdout[-3000:] + r.stderr[-3000:]
+    assert "Reloading simulation data" in r.stdout

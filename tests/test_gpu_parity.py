@@ -221,6 +221,33 @@ def test_tiles_without_bonds(nsf, oracle):
     assert rel_err(fen, ref.fenergy) < 1e-12
 
 
+@pytest.mark.parametrize("mode,precision", [("L", "fp64"), ("C", "fp64"), ("L", "mixed")])
+def test_shared_fixed_a(mode, precision, oracle):
+    """A thin-film string whose fixed A is the same in every image keeps one
+    A for all images: the step kernels read image 0's A planes and the
+    reparametrisation does not interpolate it, so A stays exactly the input.
+    The reference interpolates the (identical) A as well, which moves it by
+    at most a few ulps; fields and energies match the oracle to 1e-10 after
+    three outer iterations (FP64), to 1e-5 in mixed precision."""
+    s = small_string(N=24, F=6, seed=94, multicomponent=True, masked=True, no_self_field=True)
+    s.a[:] = s.a[0]
+    say(f"[gpu] shared fixed A, mode {mode}, {precision}")
+    ref = oracle.OracleString(s)
+    ref.loop(3, 4, mode)
+    with solver_for([s], precision=precision) as sv:
+        sv.loop(3, 4, mode)
+        a, p1, p2 = sv.download(0)
+        e = sv.computeFreeEnergy()[0]
+    a_in = s.a if precision == "fp64" else s.a.astype(np.float32).astype(np.float64)
+    np.testing.assert_array_equal(a, a_in)
+    tol = 1e-10 if precision == "fp64" else 1e-5
+    if precision == "fp64":
+        assert normwise(a, ref.a) < 1e-14
+    assert normwise(p1, ref.psi_1) < tol
+    assert normwise(p2, ref.psi_2) < tol
+    assert rel_err(e, ref.free_energy()) < tol
+
+
 @pytest.mark.parametrize("textbook", [False, True])
 def test_cubic_redistribution(textbook, oracle):
     s = small_string(N=20, F=7, seed=10, multicomponent=True)
@@ -318,6 +345,55 @@ def test_loop_with_alignment(oracle):
         a, p1, p2 = sv.download(0)
     assert normwise(p1, ref.psi_1) < 1e-10
     assert normwise(p2, ref.psi_2) < 1e-10
+
+
+@pytest.mark.parametrize("case", ["one_comp", "two_comp_masked", "two_strings"])
+def test_local_gauge_alignment(case, oracle):
+    """mepgl_gauge_align_local (rhs, four DCT products, prefix, apply) against
+    the oracle's restatement, after two NLCG steps so that the direction and
+    gradients carry psi parts that must rotate; FP64, 1e-11.  Two strings of
+    different sizes share the context in the last case."""
+    if case == "two_strings":
+        strings = [small_string(N=20, F=5, seed=95, multicomponent=True),
+                   small_string(N=70, F=4, seed=96)]
+    else:
+        strings = [small_string(N=24, F=6, seed=97, **CASES[case])]
+    for st in strings:
+        st.q_2 = 0.6
+    say(f"[gpu] local gauge alignment case={case}")
+    refs = [oracle.OracleString(st) for st in strings]
+    for r in refs:
+        r.nlcg_steps(2)
+        r.gauge_align_local()
+        r.nlcg_steps(2)
+    with solver_for(strings) as sv:
+        sv.nlcgSteps(2)
+        sv.gauge_align_local()
+        sv.nlcgSteps(2)
+        got = [sv.download(i) for i in range(len(strings))]
+    for (a, p1, p2), r, st in zip(got, refs, strings):
+        assert normwise(a, r.a) < 1e-11
+        assert normwise(p1, r.psi_1) < 1e-11
+        if st.multicomponent:
+            assert normwise(p2, r.psi_2) < 1e-11
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_loop_with_local_alignment(precision, oracle):
+    """Mode L with the local and the constant-phase alignment before every
+    arclength, graph replay included, against the oracle's loop."""
+    s = small_string(N=20, F=6, seed=98, multicomponent=True)
+    say(f"[gpu] loop with local alignment, {precision}")
+    ref = oracle.OracleString(s)
+    ref.loop(3, 4, "L", align=True, align_local=True)
+    with solver_for([s], precision=precision, gauge_align=True,
+                    gauge_align_local=True) as sv:
+        sv.loop(3, 4, "L")
+        a, p1, p2 = sv.download(0)
+    tol = 1e-10 if precision == "fp64" else 1e-5
+    assert normwise(a, ref.a) < tol
+    assert normwise(p1, ref.psi_1) < tol
+    assert normwise(p2, ref.psi_2) < tol
 
 
 # N = 40: the last segment has 1 (13 rows), 2 (19), 3 (37), 4 (6), 5 (7)

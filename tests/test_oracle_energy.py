@@ -155,6 +155,43 @@ def test_two_component_global_phase_and_relative_phase(oracle):
     del rng
 
 
+@pytest.mark.parametrize("N,c1,c2,theta", [(9, 0.8, 0.6, 0.0), (12, 1.1, 0.4, 0.7),
+                                             (7, 0.5, 0.9, np.pi / 2), (10, 0.3, 1.2, 2.9)])
+def test_two_component_uniform_closed_form(oracle, N, c1, c2, theta):
+    """Uniform two-component state on the full box with A = 0 and h = 0: no
+    kinetic or magnetic term survives, and every one of the (N - 1)^2
+    plaquettes carries area weight 1 (interior sites 4 x 1/4, edges 2 x 1/4,
+    corners 1/4), so
+
+      F = dx^2 (N - 1)^2 [ sum_a b_a/2 (a_a/b_a + c_a^2)^2 + eta c1 c2 cos t
+                           + gamma/2 c1^2 c2^2 + delta/2 c1^2 c2^2 cos 2t ].
+
+    Each coupling is also switched on alone, so a scale error in eta, gamma
+    or delta (or a wrong sign of the cos 2t term) fails on its own."""
+    from mepgl_amd.problem import GLString
+
+    dx = 0.3
+    prm = dict(a_1=-1.0, a_2=-0.6, b_1=1.0, b_2=0.8, m_xx_1=1.0, m_yy_1=1.3, m_xx_2=0.5,
+               m_yy_2=2.0, q_1=1.0, q_2=0.7)
+    p1 = np.zeros((1, N, N, 2))
+    p2 = np.zeros((1, N, N, 2))
+    p1[..., 0] = c1
+    p2[..., 0], p2[..., 1] = c2 * np.cos(theta), c2 * np.sin(theta)
+    box = full_box(N)
+    for eta, gamma, delta in ((-0.2, 0.3, 0.05), (0.7, 0.0, 0.0), (0.0, 0.45, 0.0),
+                              (0.0, 0.0, 0.35)):
+        s = GLString(N=N, dx=dx, F=1, multicomponent=True, comp_domain=box,
+                     sc_domain=box.copy(), a=np.zeros((1, N, N, 2)), psi_1=p1, psi_2=p2, h=0.0,
+                     eta=eta, gamma=gamma, delta=delta, **prm).validate()
+        cond = sum(b / 2 * (a / b + c * c) ** 2
+                   for a, b, c in ((prm["a_1"], prm["b_1"], c1), (prm["a_2"], prm["b_2"], c2)))
+        coup = (eta * c1 * c2 * np.cos(theta) + gamma / 2 * c1 ** 2 * c2 ** 2
+                + delta / 2 * c1 ** 2 * c2 ** 2 * np.cos(2 * theta))
+        expect = dx ** 2 * (N - 1) ** 2 * (cond + coup)
+        e = oracle.free_energy(s)[0]
+        np.testing.assert_allclose(e, expect, rtol=1e-12, atol=1e-14)
+
+
 @pytest.mark.parametrize("case", [dict(multicomponent=False, planes=True),
                                   dict(multicomponent=True),
                                   dict(multicomponent=True, no_self_field=True)])
