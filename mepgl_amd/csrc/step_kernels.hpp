@@ -434,16 +434,11 @@ __device__ __forceinline__ void poly_body(const StepArgs& a, const DevString& s,
   };
   auto nnoff = [&](int row) { return (unsigned)((row * P_ + t.ic) * (int)sizeof(uint16_t)); };
   // The march is the only part that depends on the direction
-  // (marches_up).  Its first row is loaded before beta is totalled, so that
-  // its latency overlaps the round trip of the partials' loads (a second
-  // early row would cost the FP64 kernels half their occupancy).
+  // (marches_up).  Its rows are loaded after beta is totalled: holding a row
+  // across the partials' reduction costs the mixed kernels a wave per SIMD
+  // (129 VGPRs), more than the overlap gains.
   const bool up = marches_up(s, it);
   const bool bonds = !(it.flags & kItemNoBonds);
-  PRaw<T, NC> pre0;
-  if (bonds) {
-    const int jb0 = up ? t.j1 - 1 : t.j0, d0 = up ? -1 : 1;
-    p_load<T, NC, NSF>(pre0, b, clampr(jb0 - d0, N), P_, t, nnoff(clampr(jb0 - d0, N)));
-  }
   double gnorm2, gdelta, beta_d;
   totals(gnorm2, gdelta, beta_d);
   const T beta = uniform_value((T)beta_d);
@@ -474,7 +469,8 @@ __device__ __forceinline__ void poly_body(const StepArgs& a, const DevString& s,
     PRaw<T, NC> raw;
     PRow<T, NC> w[3];
     RowParams<T, NC, PL> rp[3];
-    p_process<T, NC>(w[0], pre0, beta);  // row jb - D
+    p_load<T, NC, NSF>(raw, b, clampr(jb - D, N), P_, t, nnoff(clampr(jb - D, N)));
+    p_process<T, NC>(w[0], raw, beta);
     p_load<T, NC, NSF>(raw, b, clampr(jb, N), P_, t, nnoff(clampr(jb, N)));
     p_process<T, NC>(w[1], raw, beta);
     p_load<T, NC, NSF>(raw, b, clampr(jb + D, N), P_, t, nnoff(clampr(jb + D, N)));

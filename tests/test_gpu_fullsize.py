@@ -11,8 +11,9 @@ not depend on size:
   (same schedule), and a tuned schedule stays within 1e-12 of the default;
   tuned mixed within 1e-5 of tuned FP64 on every string, FP64 within 1e-10
   of the oracle on four strings.
-* Barriers at convergence (vortex_entry_small, vortex_entry_256,
-  two_band_512): mixed within 1e-6 of FP64.
+* Barriers at convergence (vortex_entry_small, vortex_entry_256): mixed
+  within 1e-6 of FP64.  two_band_512 converges in neither precision: mixed
+  evaluation of a relaxed FP64 state within 1e-7 on barriers.
 * thin_film_disk_1024 as specified (mixed, 4 replicas in one context):
   batched equals per-replica bit for bit; energies within 1e-5 of the
   oracle on every replica.
@@ -164,12 +165,15 @@ def relax_to_convergence(sv, cap, tol=1e-10, chunk=5):
     return it, sv.computeFreeEnergy()[0]
 
 
-@pytest.mark.parametrize("name,cap", [("vortex_entry_small", 400), ("vortex_entry_256", 1500),
-                                      ("two_band_512", 600)])
+@pytest.mark.parametrize("name,cap", [("vortex_entry_small", 1500), ("vortex_entry_256", 1500)])
 def test_mixed_barrier_at_convergence(name, cap):
     """Mixed vs FP64 barriers (post.py:188-189) at convergence, once an
     outer iteration changes the energies by less than 1e-10 relative:
-    within 1e-6 relative of each other on the benchmark strings."""
+    within 1e-6 relative of each other.  FP64 creeps there slowly
+    (vortex_entry_small about 800 outer iterations, 1.3e-9 per iteration
+    after 15 and 1e-10 after 790; vortex_entry_256 about 100); the mixed
+    path stops changing at float resolution after some 20.  two_band_512
+    does not converge in either precision (next test)."""
     from mepgl_amd import configs, observables
 
     s = configs.WORKLOADS[name]()[0]
@@ -185,6 +189,42 @@ def test_mixed_barrier_at_convergence(name, cap):
         ref = bars["fp64"][k]
         if ref:
             assert abs(bars["mixed"][k] - ref) <= 1e-6 * abs(ref), (k, bars)
+
+
+def test_two_band_mixed_at_relaxed_state():
+    """two_band_512 does not converge within 2000 outer iterations in either
+    precision: after some 200 every image sits within 1.5 of 2937.3, and the
+    last image's component-1 vortex keeps leaving through the edge down a
+    nearly flat valley (FP64 changes 6e-11 per iteration after 1900, mixed
+    1.2e-8, and mixed goes further down it: last-image energy 2935.912
+    against 2935.942 after 1000; FP64 continued from the mixed state keeps
+    the lower energy).  So the two precisions are compared on one state:
+    FP64 relaxed for 200 outer iterations, then evaluated by a mixed
+    context.  Barriers agree within 1e-7 relative (measured 2.7e-9) and
+    energies within 1e-9 relative per image; 20 mixed outer iterations from
+    there raise no image's energy by more than the float floor (2e-7
+    relative, the first image's measured offset being 8e-8)."""
+    import dataclasses
+
+    from mepgl_amd import configs, observables
+
+    s = configs.two_band_512()
+    say("[gpu] two_band_512: FP64 relaxed 200 outer iterations, evaluated and continued in mixed")
+    with solver([s], precision="fp64") as sv:
+        sv.loop(200, 10, "L")
+        e64 = sv.computeFreeEnergy()[0]
+        a, p1, p2 = sv.download(0)
+    relaxed = dataclasses.replace(s, a=a, psi_1=p1, psi_2=p2)
+    with solver([relaxed], precision="mixed") as sv:
+        e32 = sv.computeFreeEnergy()[0]
+        sv.loop(20, 10, "L")
+        e32_after = sv.computeFreeEnergy()[0]
+    b64, b32 = observables.barriers(e64), observables.barriers(e32)
+    say(f"[gpu] two_band_512 barriers fp64 {b64} mixed {b32}")
+    assert np.max(np.abs(e32 - e64) / np.abs(e64)) < 1e-9
+    for k in range(2):
+        assert abs(b32[k] - b64[k]) <= 1e-7 * abs(b64[k]), (k, b64, b32)
+    assert np.all(e32_after <= e64 + 2e-7 * np.abs(e64)), np.max((e32_after - e64) / np.abs(e64))
 
 
 def test_thin_film_mixed_replicas(oracle):
