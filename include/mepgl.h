@@ -246,6 +246,29 @@ int mepgl_get_control(mepgl_ctx *ctx, int key, int *value);
 int mepgl_profile(mepgl_ctx *ctx, int enable);
 int mepgl_profile_read(mepgl_ctx *ctx, int kernel, double *avg_ms, int *count);
 
+/* computeFreeEnergy (nlcg.cu:217-240) of one string held in caller-owned
+ * device memory, allocating nothing.  No counterpart in the reference (its
+ * computeFreeEnergy allocates a density buffer, nlcg.cu:226, and its sums
+ * their partials, parallel_sum.cu:52-66): this entry point is for callers
+ * that cannot allocate device memory, such as code under HIP graph capture,
+ * serving loops and harnesses that forbid hipMalloc.  It needs no context.
+ * For this call every pointer is a device pointer: comp_domain and
+ * sc_domain in the descriptor (N*N int32, [j][i]) and any parameter plane
+ * (N*N double, [j][i]); a, psi_1, psi_2 (psi_2 only for two components) are
+ * (F, N, N, 2) double arrays in the layout above; energy receives F doubles.
+ * stream is a hipStream_t (NULL: the default stream); the call is
+ * asynchronous on it.  The per-tile partial sums go to one static device
+ * array per GPU of 65,536 (sum, compensation) pairs, so
+ * F * ceil(N / 64) * ceil(N / rows) may not exceed 65,536, where rows is
+ * desc->segment_rows (16 when 0); larger inputs return MEPGL_ERR_ARG before
+ * any launch.  The array keeps the last call's partials (each call
+ * overwrites every partial it reads), and calls that overlap on one GPU
+ * race on it.  With the same segment height the energies equal those of
+ * mepgl_free_energy on an FP64 context bit for bit. */
+int mepgl_free_energy_noalloc(const mepgl_string_desc *desc, const double *a,
+                              const double *psi_1, const double *psi_2, double *energy,
+                              void *stream);
+
 /* Waits for all work of the context. */
 int mepgl_synchronize(mepgl_ctx *ctx);
 

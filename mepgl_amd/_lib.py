@@ -30,6 +30,7 @@ EXPORTS = [
     "mepgl_reset_interior_beta", "mepgl_gauge_align", "mepgl_gauge_align_local", "mepgl_loop", "mepgl_get_scalars", "mepgl_download_state",
     "mepgl_set_control", "mepgl_get_control", "mepgl_profile", "mepgl_profile_read",
     "mepgl_synchronize", "mepgl_info", "mepgl_set_segment_rows", "mepgl_autotune", "mepgl_upload_state", "mepgl_set_scalars", "mepgl_winding",
+    "mepgl_free_energy_noalloc",
 ]
 
 _dp = C.POINTER(C.c_double)

@@ -27,7 +27,7 @@ HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
 # The step kernels are split by storage type and component count so that they
 # compile concurrently.
 SOURCES = ["step_f64_2.hip", "step_f64_1.hip", "step_f32_2.hip", "step_f32_1.hip", "kernels.hip",
-           "context.cpp", "gauge_local.hip"]
+           "context.cpp", "gauge_local.hip", "noalloc.hip"]
 CLI_SOURCES = ["cli.cpp", "npyio.cpp"]
 
 
