@@ -233,6 +233,15 @@ int mepgl_set_scalars(mepgl_ctx *ctx, int string, int which, const double *value
 #define MEPGL_CONTROL_STOP 0
 #define MEPGL_CONTROL_RELAXATION_STEPS 1
 #define MEPGL_CONTROL_NLCG 2
+/* No counterpart in the reference: where P's line search runs.  1 (the
+ * default): in the G of the next step of the same call or outer iteration
+ * (every G wavefront scans the table from P's partials), so only the last
+ * step's P has a finishing launch, when every string has at most 64 tiles
+ * per image (ceil(N / 64) * ceil(N / segment rows)); 0, or more tiles: in a
+ * finishing launch after every P.  Results are identical bit for bit.  Set
+ * it between calls, from the calling thread (a change drops the captured
+ * graphs). */
+#define MEPGL_CONTROL_FOLD_LINE_SEARCH 3
 int mepgl_set_control(mepgl_ctx *ctx, int key, int value);
 int mepgl_get_control(mepgl_ctx *ctx, int key, int *value);
 
@@ -242,7 +251,9 @@ int mepgl_get_control(mepgl_ctx *ctx, int key, int *value);
  * 0 = G with the pending update, 1 = P (direction + quartic),
  * 2 = G without an update, 3 = reserved (P reduces beta itself, so there
  * is no finishing launch after G; reads a count of 0),
- * 4 = finishing launch after P (quartic totals + line search). */
+ * 4 = finishing launch after P (quartic totals + line search; with the
+ * line search folded into G, only after the last step of a call or outer
+ * iteration). */
 int mepgl_profile(mepgl_ctx *ctx, int enable);
 int mepgl_profile_read(mepgl_ctx *ctx, int kernel, double *avg_ms, int *count);
 

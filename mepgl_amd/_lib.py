@@ -20,7 +20,7 @@ SCALARS = {
     "poly": 7, "energy": 8, "g_norm2_old": 9, "ls_index": 10, "theta": 11,
     "winding_1": 12, "winding_2": 13,
 }
-CONTROL_STOP, CONTROL_RELAXATION_STEPS, CONTROL_NLCG = 0, 1, 2
+CONTROL_STOP, CONTROL_RELAXATION_STEPS, CONTROL_NLCG, CONTROL_FOLD_LINE_SEARCH = 0, 1, 2, 3
 KERNEL_GRAD, KERNEL_POLY = 0, 1
 
 EXPORTS = [

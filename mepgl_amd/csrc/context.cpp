@@ -75,6 +75,10 @@ struct mepgl_ctx {
   std::string err;
   int xp = 0, dp = 0;        // current field / direction buffers
   bool pending = false;      // an update x += alpha d is owed
+  // The last P's line search is left to the next G (its partials hold the
+  // quartic; alpha and the line-search scalars are not yet stored).
+  bool ls_deferred = false;
+  bool fold_ls = true;       // MEPGL_CONTROL_FOLD_LINE_SEARCH
   std::atomic<int> stop{0}, relax_steps{25}, nlcg{1};
   Timing timing;
   bool dry = false;          // advance the parity state without launching
@@ -247,7 +251,12 @@ int run_step_group(mepgl_ctx* c, const StepArgs& a, int which) {
   return MEPGL_OK;
 }
 
+int settle_ls(mepgl_ctx* c);
+
 int run_aux(mepgl_ctx* c, const AuxArgs& a, int which, int arg) {
+  if (c->ls_deferred) {
+    if (int rc = settle_ls(c)) return rc;
+  }
   if (c->dry) return MEPGL_OK;
   hipError_t e = launch_aux(a, which, arg, c->fp32(), c->stream);
   if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, std::string("launch: ") + hipGetErrorString(e));
@@ -284,8 +293,18 @@ int run_finish(mepgl_ctx* c, const FinishArgs& f, int which) {
   return MEPGL_OK;
 }
 
+// Runs a line search left to the next G as a finishing launch, for anything
+// other than that G (nothing is left pending across calls; this covers a
+// sequence of steps cut short by an error).
+int settle_ls(mepgl_ctx* c) {
+  if (!c->ls_deferred) return MEPGL_OK;
+  c->ls_deferred = false;
+  return run_finish(c, finish_args(c), kKernelPoly);
+}
+
 // Applies an owed update in place.
 int flush(mepgl_ctx* c) {
+  if (int rc = settle_ls(c)) return rc;
   if (!c->pending) return MEPGL_OK;
   StepArgs a = step_args(c);
   a.xin = c->xp;
@@ -299,7 +318,23 @@ int flush(mepgl_ctx* c) {
 // One NLCG step k of a call (nlcg.cu:1478-1583).  The reference swaps local
 // copies of the gradient pointers (nlcg.cu:1484-1498), so step k writes the
 // caller's g_old buffer when k is even and its g buffer when k is odd.
-int nlcg_step(mepgl_ctx* c, int k, int nlcg, int maxwell) {
+// Whether G takes over P's line search: only when every string has at most
+// one tile per lane of the partials' reduction.  With more tiles per image
+// every G wavefront waits for several rounds of partial loads before its
+// first row, which cost more than the finishing launch saves (two_band_512,
+// 256 tiles per image: 1 % slower in both precisions; vortex_entry_256, 64
+// tiles: 1.2-1.8 % faster).
+bool fold_line_search(const mepgl_ctx* c) {
+  if (!c->fold_ls) return false;
+  for (const auto& s : c->strings)
+    if (s.ntiles > kWave) return false;
+  return true;
+}
+
+// `more`: another step follows at once, so P's line search may be left to
+// that step's G (every G wavefront scans the table from P's partials), which
+// saves the finishing launch and one kernel boundary per step.
+int nlcg_step(mepgl_ctx* c, int k, int nlcg, int maxwell, bool more = false) {
   const int gcur = (k % 2 == 0) ? 1 : 0;
   StepArgs a = step_args(c);
   a.gcur = gcur;
@@ -311,14 +346,20 @@ int nlcg_step(mepgl_ctx* c, int k, int nlcg, int maxwell) {
   a.xin = c->xp;
   a.din = c->dp;
   a.upd = c->pending ? 1 : 0;
+  a.ls_fold = c->pending && c->ls_deferred ? 1 : 0;
   a.xout = 1 - c->xp;
+  if (!a.ls_fold) {
+    if (int rc = settle_ls(c)) return rc;
+  }
   int rc = run_step(c, a, kKernelGrad);
   if (rc) return rc;
+  c->ls_deferred = false;
   if (c->pending) {
     c->xp = 1 - c->xp;
     c->pending = false;
   }
   a.upd = 0;
+  a.ls_fold = 0;
   a.xin = c->xp;
   a.din = c->dp;
   a.dout = 1 - c->dp;
@@ -326,7 +367,10 @@ int nlcg_step(mepgl_ctx* c, int k, int nlcg, int maxwell) {
   // launch after G); the quartic totals and the line search follow P.
   rc = run_step(c, a, kKernelPoly);
   if (rc) return rc;
-  if ((rc = run_finish(c, finish_args(c), kKernelPoly))) return rc;
+  if (more && fold_line_search(c))
+    c->ls_deferred = true;
+  else if ((rc = run_finish(c, finish_args(c), kKernelPoly)))
+    return rc;
   c->dp = 1 - c->dp;
   c->pending = true;
   return MEPGL_OK;
@@ -770,7 +814,7 @@ int outer_iteration(mepgl_ctx* c, int relax, char mode, int nlcg) {
   }
   const bool maxwell = mode == 'M';
   for (int k = 0; k < relax; ++k)
-    if ((rc = nlcg_step(c, k, nlcg, maxwell ? 1 : 0))) return rc;
+    if ((rc = nlcg_step(c, k, nlcg, maxwell ? 1 : 0, k + 1 < relax))) return rc;
   return MEPGL_OK;
 }
 
@@ -798,7 +842,7 @@ struct Snapshot {
   std::vector<size_t> bytes;
   void* sc = nullptr;
   int xp = 0, dp = 0;
-  bool pending = false;
+  bool pending = false, ls_deferred = false;
 };
 
 void drop_snapshot(mepgl_ctx* c, Snapshot& sn) {
@@ -839,6 +883,7 @@ int take_snapshot(mepgl_ctx* c, Snapshot& sn) {
   sn.xp = c->xp;
   sn.dp = c->dp;
   sn.pending = c->pending;
+  sn.ls_deferred = c->ls_deferred;
   return MEPGL_OK;
 }
 
@@ -851,6 +896,7 @@ int restore_snapshot(mepgl_ctx* c, const Snapshot& sn) {
   c->xp = sn.xp;
   c->dp = sn.dp;
   c->pending = sn.pending;
+  c->ls_deferred = sn.ls_deferred;
   return MEPGL_OK;
 }
 
@@ -864,6 +910,7 @@ int refresh_snapshot(mepgl_ctx* c, Snapshot& sn) {
   sn.xp = c->xp;
   sn.dp = c->dp;
   sn.pending = c->pending;
+  sn.ls_deferred = c->ls_deferred;
   return MEPGL_OK;
 }
 
@@ -894,7 +941,8 @@ int time_schedule(mepgl_ctx* c, const Snapshot& sn, int steps, double* ms) {
   if (e == hipSuccess && reparam) {
     rc = outer_iteration(c, steps, 'L', 1);
   } else {
-    for (int k = 0; k < steps && !rc && e == hipSuccess; ++k) rc = nlcg_step(c, k + 1, 1, 0);
+    for (int k = 0; k < steps && !rc && e == hipSuccess; ++k)
+      rc = nlcg_step(c, k + 1, 1, 0, k + 1 < steps);
   }
   if (!rc && e == hipSuccess) e = hipEventRecord(e1, c->stream);
   if (!rc && e == hipSuccess) e = hipEventSynchronize(e1);
@@ -1218,7 +1266,7 @@ int mepgl_nlcg_steps(mepgl_ctx* c, int iter_n, int nlcg, int maxwell) {
   if (!c || iter_n < 0) return MEPGL_ERR_ARG;
   HIP_TRY(c, hipSetDevice(c->device));
   for (int k = 0; k < iter_n; ++k) {
-    int rc = nlcg_step(c, k, nlcg ? 1 : 0, maxwell ? 1 : 0);
+    int rc = nlcg_step(c, k, nlcg ? 1 : 0, maxwell ? 1 : 0, k + 1 < iter_n);
     if (rc) return rc;
   }
   return check_finite(c, " after nlcg_steps");
@@ -1451,6 +1499,7 @@ int mepgl_loop(mepgl_ctx* c, int iterations, int relax, char mode, int nlcg) {
         c->xp = xp0;
         c->dp = dp0;
         c->pending = pend0;
+        c->ls_deferred = false;
         if (rc) {
           if (g) (void)hipGraphDestroy(g);
           break;
@@ -1511,6 +1560,7 @@ int mepgl_loop(mepgl_ctx* c, int iterations, int relax, char mode, int nlcg) {
 int mepgl_get_scalars(mepgl_ctx* c, int si, int which, double* out) {
   if (!valid_string(c, si) || !out) return fail(c, MEPGL_ERR_ARG, "get_scalars: bad arguments");
   HIP_TRY(c, hipSetDevice(c->device));
+  if (int rc = settle_ls(c)) return rc;
   const DevString& d = c->strings[si].dev;
   std::vector<ImageScalars> h(d.F);
   HIP_TRY(c, hipMemcpyAsync(h.data(), c->d_sc + d.img0, sizeof(ImageScalars) * d.F,
@@ -1599,6 +1649,12 @@ int mepgl_set_control(mepgl_ctx* c, int key, int value) {
     case MEPGL_CONTROL_STOP: c->stop.store(value); break;
     case MEPGL_CONTROL_RELAXATION_STEPS: c->relax_steps.store(value); break;
     case MEPGL_CONTROL_NLCG: c->nlcg.store(value); break;
+    case MEPGL_CONTROL_FOLD_LINE_SEARCH:
+      if (c->fold_ls != (value != 0)) {
+        c->fold_ls = value != 0;
+        destroy_graphs(c);
+      }
+      break;
     default: return MEPGL_ERR_ARG;
   }
   return MEPGL_OK;
@@ -1610,6 +1666,7 @@ int mepgl_get_control(mepgl_ctx* c, int key, int* value) {
     case MEPGL_CONTROL_STOP: *value = c->stop.load(); break;
     case MEPGL_CONTROL_RELAXATION_STEPS: *value = c->relax_steps.load(); break;
     case MEPGL_CONTROL_NLCG: *value = c->nlcg.load(); break;
+    case MEPGL_CONTROL_FOLD_LINE_SEARCH: *value = c->fold_ls ? 1 : 0; break;
     default: return MEPGL_ERR_ARG;
   }
   return MEPGL_OK;

@@ -49,18 +49,11 @@ __global__ __launch_bounds__(kFinishPolyBlock) void k_finish_poly(FinishArgs f) 
       for (int k = 0; k < 5; ++k) s_q[k] = fin[k];
   }
   __syncthreads();
-  const double A = s_q[0], B = s_q[1], C = s_q[2], D = s_q[3], E = s_q[4];
-  // f evaluated without contraction, in the reference's operation order.
-  auto fq = [&](double xv, double x4v) {
-    double v = __dmul_rn(E, x4v);
-    v = __dadd_rn(v, __dmul_rn(__dmul_rn(__dmul_rn(D, xv), xv), xv));
-    v = __dadd_rn(v, __dmul_rn(__dmul_rn(C, xv), xv));
-    v = __dadd_rn(v, __dmul_rn(B, xv));
-    return __dadd_rn(v, A);
-  };
+  const double q[5] = {s_q[0], s_q[1], s_q[2], s_q[3], s_q[4]};
+  const double A = q[0], B = q[1], C = q[2], D = q[3], E = q[4];
   double fv[kLsPerLane + 1];
 #pragma unroll
-  for (int k = 0; k <= kLsPerLane; ++k) fv[k] = fq(x[k], x4[k]);
+  for (int k = 0; k <= kLsPerLane; ++k) fv[k] = ls_quartic(q, x[k], x4[k]);
   int first = kLsSteps;
 #pragma unroll
   for (int k = kLsPerLane - 1; k >= 0; --k)

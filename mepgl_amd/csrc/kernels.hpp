@@ -39,6 +39,8 @@ struct StepArgs {
   int gcur, gold;   // gradient buffers (current, previous)
   int scur, sold;   // g_norm2 slots (current, previous)
   int upd;          // G applies the pending update x += alpha d
+  int ls_fold;      // ... with alpha from the previous P's partials (its line
+                    // search done by G; no finishing launch after that P)
   int maxwell;      // Maxwell mode: psi gradient not recomputed
   int nlcg;         // Polak-Ribiere beta (1) or steepest descent (0)
   int scratch;      // G writes the diagnostic gradient buffer only

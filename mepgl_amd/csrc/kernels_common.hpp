@@ -118,18 +118,18 @@ __device__ __forceinline__ void publish_partial(double (&s)[K], double (&c)[K], 
   }
 }
 
-// Fixed-order total of the ntiles partials of image n (every lane).
-template <int K, bool COMP>
+// Fixed-order total of the ntiles partials of image n (every lane).  U: the
+// loads of U of a lane's tiles are issued before any is accumulated (fewer
+// waits for memory, more registers); the order of the sums does not depend
+// on U.
+template <int K, bool COMP, int U = 4>
 __device__ __forceinline__ void reduce_partials(const double* part, int n, int ntiles,
                                                 double (&fin)[K]) {
   double ss[K], cc[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) ss[k] = cc[k] = 0.0;
   const double* base = part + (long long)n * ntiles * (2 * K);
-  // Each lane takes tiles lane, lane + 64, ... in that order; the loads of
-  // four of its tiles are issued before any is accumulated, so a lane waits
-  // for memory once per four tiles instead of once per tile.
-  constexpr int U = 4;
+  // Each lane takes tiles lane, lane + 64, ... in that order.
   for (int t0 = threadIdx.x; t0 < ntiles; t0 += U * kWave) {
     double v[U][K], w[U][K];
 #pragma unroll
@@ -185,6 +185,56 @@ __device__ __forceinline__ double from_right(double v, double edge) {
   const int hi = __builtin_amdgcn_update_dpp(__double2hiint(edge), __double2hiint(v), 0x130, 0xF,
                                              0xF, false);
   return __hiloint2double(hi, lo);
+}
+
+// ---------------------------------------------------------------------------
+// Line search (lineSearch, nlcg.cu:1414-1436): the quartic
+// f(x) = A + B x + C x^2 + D x^3 + E x^4 of an image's step length, scanned
+// over the table x_k for the first rise.
+
+// f at table point (x, x^4), without contraction, in the reference's order.
+__device__ __forceinline__ double ls_quartic(const double (&q)[5], double x, double x4) {
+  double v = __dmul_rn(q[4], x4);
+  v = __dadd_rn(v, __dmul_rn(__dmul_rn(__dmul_rn(q[3], x), x), x));
+  v = __dadd_rn(v, __dmul_rn(__dmul_rn(q[2], x), x));
+  v = __dadd_rn(v, __dmul_rn(q[1], x));
+  return __dadd_rn(v, q[0]);
+}
+
+// The line search of image n by one wavefront: the fixed-order totals A..E
+// of its quartic partials (into q, every lane) and the first table index
+// k < kLsSteps with f(x_{k+1}) > f(x_k), or kLsSteps - 1 when f never rises.
+// Each pass evaluates 64 consecutive points, lane l at x_{c0+l}, and
+// compares each with its right neighbour, so a pass decides 63 indices; the
+// next pass's points are loaded before the current one is evaluated.  The
+// totals and the index equal those of the finishing launch (k_finish_poly).
+template <bool COMP>
+__device__ __forceinline__ int ls_scan_wave(const LineSearchTable* t, const double* part_p,
+                                            int n, int ntiles, double (&q)[5]) {
+  const int lane = threadIdx.x % kWave;
+  int k = min(lane, kLsSteps);
+  double xv = t->x[k], x4v = t->x4[k];
+  // One tile's loads at a time: this runs before the march, and a wider
+  // batch would raise the kernel's register count.
+  reduce_partials<5, COMP, 1>(part_p, n, ntiles, q);
+#pragma unroll
+  for (int m = 0; m < 5; ++m) q[m] = uniform_value(q[m]);
+  int first = kLsSteps - 1;
+  for (int c0 = 0; c0 < kLsSteps; c0 += kWave - 1) {
+    const int kn = min(c0 + (kWave - 1) + lane, kLsSteps);
+    const double xn = t->x[kn], x4n = t->x4[kn];
+    const double f = ls_quartic(q, xv, x4v);
+    const double fr = from_right(f, f);  // f(x_{c0+l+1}); lane 63 has no right neighbour
+    const bool rise = lane < kWave - 1 && c0 + lane < kLsSteps && fr > f;
+    const unsigned long long b = __ballot(rise);
+    if (b) {
+      first = c0 + (int)__builtin_ctzll(b);
+      break;
+    }
+    xv = xn;
+    x4v = x4n;
+  }
+  return first;
 }
 
 // Strip geometry of a work item.

@@ -98,8 +98,9 @@ __device__ __forceinline__ void grad_body(const StepArgs& a, const DevString& s,
   b.go = rsrc_of((const T*)s.g[a.gold] + ib, ibytes);
   b.nn = rsrc_of(s.nn, (long long)s.N * s.pitch * (long long)sizeof(uint16_t));
   b.ps = (unsigned)(s.plane_stride * (long long)sizeof(T));
-  const ImageScalars& sc = a.sc[s.img0 + t.n];
-  const T al = UPD ? (T)sc.alpha : (T)0;
+  ImageScalars& sc = a.sc[s.img0 + t.n];
+  T al = (T)0;
+  if (UPD && !a.ls_fold) al = (T)sc.alpha;
   constexpr bool do_a = !NSF;
   const bool want_go = !a.scratch;
   const bool store = t.i < N;
@@ -118,6 +119,28 @@ __device__ __forceinline__ void grad_body(const StepArgs& a, const DevString& s,
   // read above, before the branch and before any store, so it stays in
   // scalar registers.
   const bool up = marches_up(s, it);
+  if (UPD && a.ls_fold) {
+    // The previous P's line search, done here instead of in a finishing
+    // launch: every wavefront of the image scans the table itself, and the
+    // first tile's stores the results that launch would have stored (after
+    // every read of the descriptor, which thus stays in scalar registers).
+    double q[5];
+    const int first = ls_scan_wave<COMP>(a.ls, s.part_p, t.n, s.tiles_x * s.tiles_y, q);
+    const double alpha = a.ls->alpha[first];
+    al = (T)alpha;
+    if (it.tile == 0 && threadIdx.x == 0) {
+      sc.fenergy = q[0];
+      sc.poly[0] = q[1];
+      sc.poly[1] = q[2];
+      sc.poly[2] = q[3];
+      sc.poly[3] = q[4];
+      sc.alpha = alpha;
+      sc.ls_index = first;
+      if (!__builtin_isfinite(q[0]) || !__builtin_isfinite(q[1]) || !__builtin_isfinite(q[2]) ||
+          !__builtin_isfinite(q[3]) || !__builtin_isfinite(q[4]))
+        *a.health = 1;
+    }
+  }
   using S0 = std::integral_constant<int, 0>;
   using S1 = std::integral_constant<int, 1>;
   using S2 = std::integral_constant<int, 2>;
@@ -661,27 +684,33 @@ __global__ __launch_bounds__(kBlock) void k_update(StepArgs a) {
 
 // ---------------------------------------------------------------------------
 // Host launchers of the step kernels.
+
+// One wavefront per work item.
+template <class K>
+static void launch_items(K kernel, const StepArgs& a, hipStream_t st) {
+  hipLaunchKernelGGL(kernel, dim3((unsigned)a.nitems), dim3(kBlock), 0, st, a);
+}
+
 template <class T, int NC, bool NSF, bool PL>
-static void launch_gp_pl(const StepArgs& a, int which, const dim3& g, const dim3& b,
-                         hipStream_t st) {
+static void launch_gp_pl(const StepArgs& a, int which, hipStream_t st) {
   if (which == kKernelPoly)
-    hipLaunchKernelGGL((k_poly<T, NC, NSF, PL>), g, b, 0, st, a);
+    launch_items(k_poly<T, NC, NSF, PL>, a, st);
   else if (a.maxwell && a.upd)
-    hipLaunchKernelGGL((k_grad<T, NC, true, NSF, PL, true>), g, b, 0, st, a);
+    launch_items(k_grad<T, NC, true, NSF, PL, true>, a, st);
   else if (a.maxwell)
-    hipLaunchKernelGGL((k_grad<T, NC, false, NSF, PL, true>), g, b, 0, st, a);
+    launch_items(k_grad<T, NC, false, NSF, PL, true>, a, st);
   else if (a.upd)
-    hipLaunchKernelGGL((k_grad<T, NC, true, NSF, PL, false>), g, b, 0, st, a);
+    launch_items(k_grad<T, NC, true, NSF, PL, false>, a, st);
   else
-    hipLaunchKernelGGL((k_grad<T, NC, false, NSF, PL, false>), g, b, 0, st, a);
+    launch_items(k_grad<T, NC, false, NSF, PL, false>, a, st);
 }
 
 template <class T, int NC, bool NSF>
-static void launch_gp(const StepArgs& a, int which, const dim3& g, const dim3& b, hipStream_t st) {
+static void launch_gp(const StepArgs& a, int which, hipStream_t st) {
   if (a.pl)
-    launch_gp_pl<T, NC, NSF, true>(a, which, g, b, st);
+    launch_gp_pl<T, NC, NSF, true>(a, which, st);
   else
-    launch_gp_pl<T, NC, NSF, false>(a, which, g, b, st);
+    launch_gp_pl<T, NC, NSF, false>(a, which, st);
 }
 
 // Launches the step kernel `which` for items of NC components (the in-place
@@ -693,7 +722,7 @@ hipError_t launch_step_nc(const StepArgs& a, int which, hipStream_t st) {
   switch (which) {
     case kKernelGrad:
     case kKernelPoly:
-      a.nsf ? launch_gp<T, NC, true>(a, which, g, b, st) : launch_gp<T, NC, false>(a, which, g, b, st);
+      a.nsf ? launch_gp<T, NC, true>(a, which, st) : launch_gp<T, NC, false>(a, which, st);
       break;
     case kKernelUpdate:
       if constexpr (NC != 1) return hipErrorInvalidValue;

@@ -29,6 +29,7 @@ import glob
 import json
 import os
 import re
+import shlex
 import subprocess
 import sys
 import time
@@ -133,10 +134,13 @@ def _class_cells(strings):
     return out
 
 
-def run(a):
-    from . import configs
+def _portable(cmd):
+    """A recorded command with the repository root and interpreter written
+    portably (run from the repository root)."""
+    return shlex.join(cmd).replace(sys.executable, "python3").replace(ROOT + os.sep, "")
 
-    os.makedirs(a.out, exist_ok=True)
+
+def _passes(a):
     bench = _bench_cmd(a)
     calib = [os.path.join(HERE, "copy_calib"), str(CALIB_MIB), "3", "copy", "16"]
     passes = [("trace", ["--kernel-trace", "--stats"], bench),
@@ -147,17 +151,29 @@ def run(a):
     if a.passes:
         keep = set(a.passes.split(","))
         passes = [p for p in passes if p[0] in keep or p[0] in ("cfetch", "cwrite")]
-    cmds = {}
-    for name, opts, prog in passes:
-        d = os.path.join(a.out, name)
-        cmd = _rocprof(opts, d, name, a.timeout) + prog
-        cmds[name] = " ".join(cmd[4:])
+    return [(name, _rocprof(opts, os.path.join(a.out, name), name, a.timeout) + prog)
+            for name, opts, prog in passes]
+
+
+def run(a):
+    os.makedirs(a.out, exist_ok=True)
+    for name, cmd in _passes(a):
         rc = _run(cmd, os.path.join(a.out, name + ".log"))
         # rocprofv3 itself may end with a segmentation fault after a complete
         # run; the outputs are checked below instead of its status.
         print(f"[profiling] pass {name}: rc {rc}", flush=True)
         if rc in (124, 137):
             raise SystemExit(f"pass {name} hit its time limit")
+    summarize(a)
+
+
+def summarize(a):
+    """summary.json and summary.md from the pass outputs under a.out."""
+    from . import configs
+
+    cmds = {name: _portable(cmd[4:]) for name, cmd in _passes(a)
+            if os.path.isdir(os.path.join(a.out, name))}
+
     def opt(fn, *args):
         try:
             return fn(*args)
@@ -200,8 +216,9 @@ def run(a):
         if fv and f_fac:
             rec["read_bytes"] = sum(fv) / len(fv) * 1024 * f_fac
             rec["dispatches"] = len(fv)
-            # Reads: x, d and g_old (G with update), x (G first), x, g, d (P).
-            reads = {"grad_update": 3, "grad_first": 1, "poly": 3}[kid]
+            # Reads: x, d and g_old (G with update), x and g_old (G first),
+            # x, g and d (P).
+            reads = {"grad_update": 3, "grad_first": 2, "poly": 3}[kid]
             rec["read_over_algorithmic"] = rec["read_bytes"] / (cells * reads * c_f * sbytes)
         if wv and w_fac:
             rec["write_bytes"] = sum(wv) / len(wv) * 1024 * w_fac
@@ -224,35 +241,32 @@ def run(a):
 def markdown(sm):
     lines = [f"### {sm['workload']}, {sm['precision']}, {sm['segment_rows']}-row segments",
              "", "| Kernel | Calls | Average (us) | Algorithmic MB | GB/s (algorithmic) | "
-             "of 8 TB/s | HBM read MB | HBM write MB | traffic / algorithmic |",
-             "|---|---|---|---|---|---|---|---|---|"]
+             "of 8 TB/s | HBM read MB | read / algorithmic read | HBM write MB | "
+             "traffic / algorithmic |",
+             "|---|---|---|---|---|---|---|---|---|---|"]
 
     def mb(r, k):
         return f"{r[k] / 1e6:.1f}" if r.get(k) is not None else "-"
 
+    def num(r, k, fmt):
+        return format(r[k], fmt) if r.get(k) is not None else "-"
+
     for recs in sm["kernels"].values():
         for r in recs:
-            name = r["name"]
-            if not r["avg_us"]:
-                rr = r.get("read_over_algorithmic")
-                lines.append(f"| `{name}` | {r['calls']} | - | {r['algorithmic_bytes'] / 1e6:.1f} | "
-                             f"- | - | {mb(r, 'read_bytes')} | {mb(r, 'write_bytes')} | "
-                             f"reads {rr:.3f} |" if rr else f"| `{name}` | - |")
-                continue
-            ratio = r.get("traffic_over_algorithmic")
+            timed = bool(r["avg_us"])
             lines.append(
-                f"| `{name}` | {r['calls']} | {r['avg_us']:.1f} | "
-                f"{r['algorithmic_bytes'] / 1e6:.1f} | {r['achieved_GBs']:.0f} | "
-                f"{100 * r['frac']:.1f} % | {mb(r, 'read_bytes')} | {mb(r, 'write_bytes')} | "
-                f"{ratio:.3f} |" if ratio is not None else
-                f"| `{name}` | {r['calls']} | {r['avg_us']:.1f} | "
-                f"{r['algorithmic_bytes'] / 1e6:.1f} | {r['achieved_GBs']:.0f} | "
-                f"{100 * r['frac']:.1f} % | - | - | - |")
+                f"| `{r['name']}` | {r['calls']} | {r['avg_us']:.1f} | " if timed else
+                f"| `{r['name']}` | {r['calls']} | - | ")
+            lines[-1] += (f"{r['algorithmic_bytes'] / 1e6:.1f} | "
+                          + (f"{r['achieved_GBs']:.0f} | {100 * r['frac']:.1f} % | " if timed
+                             else "- | - | ")
+                          + f"{mb(r, 'read_bytes')} | {num(r, 'read_over_algorithmic', '.3f')} | "
+                          f"{mb(r, 'write_bytes')} | {num(r, 'traffic_over_algorithmic', '.3f')} |")
     c = sm["calibration"]
     if c["fetch_factor"] and c["write_factor"]:
         lines += ["", f"Counter calibration on the {c['copy_kib'] / 1024:.0f} MiB copy kernel: "
                   f"FETCH_SIZE x {c['fetch_factor']:.4f}, WRITE_SIZE x {c['write_factor']:.4f}."]
-    lines += ["", "Commands:", ""]
+    lines += ["", "Commands (from the repository root):", ""]
     lines += [f"    {v}" for v in sm["commands"].values()]
     return "\n".join(lines) + "\n"
 
@@ -271,14 +285,22 @@ def merge(path):
                        "kernel; keyed workload -> precision -> segment height -> kernel; "
                        "written by python -m mepgl_amd.profiling merge")
     ent = table.setdefault(sm["workload"], {}).setdefault(sm["precision"], {})
+    for k in [k for k in ent if not k.isdigit()]:
+        del ent[k]
     rows = ent.setdefault(str(sm["segment_rows"]), {})
     for kid, recs in sm["kernels"].items():
-        if len(recs) != 1 or "bytes" not in recs[0]:
+        if len(recs) != 1 or "read_bytes" not in recs[0]:
             continue  # several classes: no single per-launch figure
         r = recs[0]
-        rows[kid] = {"bytes": round(r["bytes"]), "read": round(r["read_bytes"]),
-                     "write": round(r["write_bytes"]), "algorithmic": r["algorithmic_bytes"],
-                     "avg_us": round(r["avg_us"], 2), "dispatches": r["dispatches"]}
+        rec = {"read": round(r["read_bytes"]), "algorithmic": r["algorithmic_bytes"],
+               "read_over_algorithmic": round(r["read_over_algorithmic"], 4),
+               "dispatches": r["dispatches"]}
+        if "write_bytes" in r:
+            rec.update({"bytes": round(r["bytes"]), "write": round(r["write_bytes"]),
+                        "traffic_over_algorithmic": round(r["traffic_over_algorithmic"], 4)})
+        if r["avg_us"]:
+            rec["avg_us"] = round(r["avg_us"], 2)
+        rows[kid] = rec
     rows["calibration"] = sm["calibration"]
     with open(TRAFFIC, "w") as f:
         json.dump(table, f, indent=1, sort_keys=True)
@@ -299,11 +321,16 @@ def main(argv=None):
     r.add_argument("--passes", default="",
                    help="comma-separated subset of trace,fetch,write (default: all)")
     r.add_argument("--out", required=True)
+    sm = sub.add_parser("summarize", help="recompute a summary from the outputs of `run`")
+    for act in r._actions[1:]:
+        sm._add_action(act)
     m = sub.add_parser("merge")
     m.add_argument("summary", nargs="+")
     a = ap.parse_args(argv)
     if a.cmd == "run":
         run(a)
+    elif a.cmd == "summarize":
+        summarize(a)
     else:
         for p in a.summary:
             merge(p)

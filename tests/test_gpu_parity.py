@@ -538,3 +538,37 @@ def test_winding_kernel_matches_summary(precision):
     np.testing.assert_allclose(w2, r2, rtol=0, atol=1e-12)
     np.testing.assert_allclose(w1, [2 * np.pi, -2 * np.pi, 0.0], atol=0.05)
     np.testing.assert_allclose(w2, [4 * np.pi] * 3, atol=0.1)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("N,rows", [(70, 16), (130, 7)])
+def test_folded_line_search_equals_finishing_launch(precision, N, rows):
+    """P's line search done by the next G (the default, with at most 64
+    tiles per image) and by a finishing launch after every P
+    (MEPGL_CONTROL_FOLD_LINE_SEARCH = 0) give the same fields, direction and
+    per-image scalars bit for bit, through graph replays of mode L and an
+    nlcgSteps call.  130 columns at 7-row segments: 57 tiles per image, the
+    last strip two columns wide.  The folded run makes one finishing launch
+    per call, the other one per step."""
+    from mepgl_amd import _lib
+
+    s = small_string(N=N, F=5, seed=41, multicomponent=True, masked=True)
+    say(f"[gpu] folded line search vs finishing launch: {precision} N={N} rows={rows}")
+    out = []
+    for fold in (1, 0):
+        with solver_for([s], precision=precision, segment_rows=rows) as sv:
+            sv.set_control(_lib.CONTROL_FOLD_LINE_SEARCH, fold)
+            sv.loop(2, 6, "L")
+            sv.profile(True)
+            sv.nlcgSteps(5)
+            finishes = sv.profile_read(4)[1]
+            sv.profile(False)
+            assert finishes == (1 if fold else 5), (fold, finishes)
+            names = ("fenergy", "alpha", "ls_index", "g_norm2", "beta", "poly")
+            sc = [sv.scalars(0, k) for k in names]
+            out.append((sv.download(0), sv.download_state(0, "d"), sc))
+    (f1, d1, s1), (f0, d0, s0) = out
+    for x, y in zip(f1 + d1, f0 + d0):
+        np.testing.assert_array_equal(x, y)
+    for x, y in zip(s1, s0):
+        np.testing.assert_array_equal(x, y)
