@@ -1217,7 +1217,20 @@ int mepgl_reset_solver(mepgl_ctx* c) {
   HIP_TRY(c, hipSetDevice(c->device));
   int rc = flush(c);
   if (rc) return rc;
-  return reset_solver(c);
+  if ((rc = reset_solver(c))) return rc;
+  // The current fields and direction become buffer 0 (pointer swaps in the
+  // descriptors; captured graphs reach the buffers through them), so that
+  // every reset leaves the same buffer parity and the loop after it replays
+  // the same graphs, whatever ran before.
+  if (c->xp || c->dp) {
+    for (auto& s : c->strings) {
+      if (c->xp) std::swap(s.dev.x[0], s.dev.x[1]);
+      if (c->dp) std::swap(s.dev.d[0], s.dev.d[1]);
+    }
+    c->xp = c->dp = 0;
+    if ((rc = upload_descriptors(c))) return rc;
+  }
+  return MEPGL_OK;
 }
 
 int mepgl_winding(mepgl_ctx* c) {

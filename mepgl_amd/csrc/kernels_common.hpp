@@ -36,6 +36,10 @@ __device__ __forceinline__ WorkItem load_item(const WorkItem* items, int k) {
   return u;
 }
 
+// Lane of this thread in its wavefront (workgroups hold one or more whole
+// wavefronts).
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & (kWave - 1)); }
+
 // A value that every lane of the wavefront holds, moved to scalar registers.
 __device__ __forceinline__ double uniform_value(double v) {
   const unsigned long long b = (unsigned long long)__double_as_longlong(v);
@@ -81,7 +85,7 @@ template <int K, bool COMP>
 __device__ __forceinline__ void wave_reduce(double (&s)[K], double (&c)[K]) {
 #pragma unroll
   for (int off = kWave / 2; off > 0; off >>= 1) {
-    const bool lo = (threadIdx.x & off) == 0;
+    const bool lo = (lane_id() & off) == 0;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const double so = __shfl_xor(s[k], off);
@@ -108,7 +112,7 @@ template <int K, bool COMP>
 __device__ __forceinline__ void publish_partial(double (&s)[K], double (&c)[K], double* part,
                                                 int n, int tile, int ntiles) {
   wave_reduce<K, COMP>(s, c);
-  if (threadIdx.x == 0) {
+  if (lane_id() == 0) {
     double* mine = part + ((long long)n * ntiles + tile) * (2 * K);
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -130,7 +134,7 @@ __device__ __forceinline__ void reduce_partials(const double* part, int n, int n
   for (int k = 0; k < K; ++k) ss[k] = cc[k] = 0.0;
   const double* base = part + (long long)n * ntiles * (2 * K);
   // Each lane takes tiles lane, lane + 64, ... in that order.
-  for (int t0 = threadIdx.x; t0 < ntiles; t0 += U * kWave) {
+  for (int t0 = lane_id(); t0 < ntiles; t0 += U * kWave) {
     double v[U][K], w[U][K];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
@@ -204,35 +208,58 @@ __device__ __forceinline__ double ls_quartic(const double (&q)[5], double x, dou
 // The line search of image n by one wavefront: the fixed-order totals A..E
 // of its quartic partials (into q, every lane) and the first table index
 // k < kLsSteps with f(x_{k+1}) > f(x_k), or kLsSteps - 1 when f never rises.
-// Each pass evaluates 64 consecutive points, lane l at x_{c0+l}, and
-// compares each with its right neighbour, so a pass decides 63 indices; the
-// next pass's points are loaded before the current one is evaluated.  The
-// totals and the index equal those of the finishing launch (k_finish_poly).
+// Each pass evaluates 4 x 64 consecutive points, lane l at x_{c0+4l+i}, and
+// compares each with the next (the last one of a lane with the first of
+// the lane to its right), so a pass decides 255 indices; the next pass's
+// points are loaded before the current one is evaluated.  The totals and
+// the index equal those of the finishing launch (k_finish_poly).
 template <bool COMP>
 __device__ __forceinline__ int ls_scan_wave(const LineSearchTable* t, const double* part_p,
                                             int n, int ntiles, double (&q)[5]) {
-  const int lane = threadIdx.x % kWave;
-  int k = min(lane, kLsSteps);
-  double xv = t->x[k], x4v = t->x4[k];
+  constexpr int L = 4;                  // table points per lane and pass
+  constexpr int kPass = L * kWave - 1;  // indices decided per pass
+  const int lane = lane_id();
+  auto load = [&](int c0, double (&xs)[L], double (&x4s)[L]) {
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+      const int k = min(c0 + L * lane + i, kLsSteps);
+      xs[i] = t->x[k];
+      x4s[i] = t->x4[k];
+    }
+  };
+  double xv[L], x4v[L];
+  load(0, xv, x4v);
   // One tile's loads at a time: this runs before the march, and a wider
   // batch would raise the kernel's register count.
   reduce_partials<5, COMP, 1>(part_p, n, ntiles, q);
 #pragma unroll
   for (int m = 0; m < 5; ++m) q[m] = uniform_value(q[m]);
   int first = kLsSteps - 1;
-  for (int c0 = 0; c0 < kLsSteps; c0 += kWave - 1) {
-    const int kn = min(c0 + (kWave - 1) + lane, kLsSteps);
-    const double xn = t->x[kn], x4n = t->x4[kn];
-    const double f = ls_quartic(q, xv, x4v);
-    const double fr = from_right(f, f);  // f(x_{c0+l+1}); lane 63 has no right neighbour
-    const bool rise = lane < kWave - 1 && c0 + lane < kLsSteps && fr > f;
-    const unsigned long long b = __ballot(rise);
+  for (int c0 = 0; c0 < kLsSteps; c0 += kPass) {
+    double xn[L], x4n[L];
+    load(c0 + kPass, xn, x4n);
+    double f[L];
+#pragma unroll
+    for (int i = 0; i < L; ++i) f[i] = ls_quartic(q, xv[i], x4v[i]);
+    const double fr = from_right(f[0], f[0]);  // f(x_{c0+4l+4}), lanes 0..62
+    int mine = L;                              // this lane's first rise
+#pragma unroll
+    for (int i = L - 1; i >= 0; --i) {
+      const double next = i + 1 < L ? f[i + 1] : fr;
+      const bool decided = c0 + L * lane + i < kLsSteps && (i + 1 < L || lane < kWave - 1);
+      if (decided && next > f[i]) mine = i;
+    }
+    const unsigned long long b = __ballot(mine < L);
     if (b) {
-      first = c0 + (int)__builtin_ctzll(b);
+      const int l0 = (int)__builtin_ctzll(b);
+      first = c0 + L * l0 + __builtin_amdgcn_readlane(mine, l0);
       break;
     }
-    xv = xn;
-    x4v = x4n;
+#pragma unroll
+    for (int i = 0; i < L; ++i) {
+      xv[i] = xn[i];
+      x4v[i] = x4n[i];
+    }
   }
   return first;
 }
@@ -251,7 +278,7 @@ __device__ __forceinline__ Strip strip_of(const DevString& s, const WorkItem& it
   t.n = it.image;
   t.tile = it.tile;
   const int sx = it.tile % s.tiles_x, sy = it.tile / s.tiles_x;
-  t.lane = threadIdx.x;
+  t.lane = lane_id();
   const int x0 = sx * kTileX;
   t.i = x0 + t.lane;
   t.ic = min(t.i, s.N - 1);

@@ -128,7 +128,7 @@ __device__ __forceinline__ void grad_body(const StepArgs& a, const DevString& s,
     const int first = ls_scan_wave<COMP>(a.ls, s.part_p, t.n, s.tiles_x * s.tiles_y, q);
     const double alpha = a.ls->alpha[first];
     al = (T)alpha;
-    if (it.tile == 0 && threadIdx.x == 0) {
+    if (it.tile == 0 && lane_id() == 0) {
       sc.fenergy = q[0];
       sc.poly[0] = q[1];
       sc.poly[1] = q[2];
@@ -331,9 +331,38 @@ __device__ __forceinline__ void grad_body(const StepArgs& a, const DevString& s,
   publish_partial<2, COMP>(acc_s, acc_c, s.part_g, t.n, t.tile, s.tiles_x * s.tiles_y);
 }
 
+// In mixed precision G and P run as workgroups of kStepWaves wavefronts on
+// consecutive work items, which are horizontally adjacent strips of one
+// segment (the work list runs strip by strip, then segment by segment).  A
+// workgroup starts on one CU as a unit, so neighbouring strips march
+// together in every dispatch generation, and each strip's halo columns are
+// read while its neighbour's wavefront still has those cache lines in L2.
+// With one wavefront per workgroup, neighbours dispatched in later
+// generations start at unrelated times and the halo lines are fetched
+// twice: two_band_512 mixed G read 1.35x its algorithmic bytes, 1.06x
+// grouped.  FP64 keeps one wavefront per workgroup: at two wavefronts per
+// SIMD, four-wavefront groups leave slots idle while a group drains
+// (two_band_512 FP64 2.3 % and thin_film_disk_1024 FP64 5.5 % slower).
+constexpr int kStepWaves = 4;
+
+template <class T>
+constexpr int step_waves() {
+  return sizeof(T) == 4 ? kStepWaves : 1;
+}
+
+// Work item of this wavefront (possibly past the end of the list); groups of
+// W items are spread over the XCDs as single items are elsewhere.
+template <int W>
+__device__ __forceinline__ int step_item(int nitems) {
+  const int groups = (nitems + W - 1) / W;
+  return xcd_item(blockIdx.x, groups) * W + (W > 1 ? (int)(threadIdx.x / kWave) : 0);
+}
+
 template <class T, int NC, bool UPD, bool NSF, bool PL, bool MW>
-__global__ __launch_bounds__(kBlock) void k_grad(StepArgs a) {
-  const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
+__global__ __launch_bounds__(kWave * step_waves<T>()) void k_grad(StepArgs a) {
+  const int k = step_item<step_waves<T>()>(a.nitems);
+  if (k >= a.nitems) return;
+  const WorkItem it = load_item(a.items, k);
   grad_body<T, NC, UPD, NSF, PL, MW>(a, a.strings[it.string], it);
 }
 
@@ -624,7 +653,7 @@ __device__ __forceinline__ void poly_body(const StepArgs& a, const DevString& s,
   // read the descriptor with vector loads.
   if (it.tile == 0) {
     totals(gnorm2, gdelta, beta_d);
-    if (threadIdx.x == 0) {
+    if (lane_id() == 0) {
       scp->g_norm2[a.scur] = gnorm2;
       scp->g_delta_g = gdelta;
       scp->beta = beta_d;
@@ -636,8 +665,10 @@ __device__ __forceinline__ void poly_body(const StepArgs& a, const DevString& s,
 }
 
 template <class T, int NC, bool NSF, bool PL>
-__global__ __launch_bounds__(kBlock) void k_poly(StepArgs a) {
-  const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
+__global__ __launch_bounds__(kWave * step_waves<T>()) void k_poly(StepArgs a) {
+  const int k = step_item<step_waves<T>()>(a.nitems);
+  if (k >= a.nitems) return;
+  const WorkItem it = load_item(a.items, k);
   poly_body<T, NC, NSF, PL>(a, a.strings[it.string], it);
 }
 
@@ -685,24 +716,26 @@ __global__ __launch_bounds__(kBlock) void k_update(StepArgs a) {
 // ---------------------------------------------------------------------------
 // Host launchers of the step kernels.
 
-// One wavefront per work item.
-template <class K>
+// One wavefront per work item, step_waves<T>() per workgroup.
+template <class T, class K>
 static void launch_items(K kernel, const StepArgs& a, hipStream_t st) {
-  hipLaunchKernelGGL(kernel, dim3((unsigned)a.nitems), dim3(kBlock), 0, st, a);
+  constexpr int W = step_waves<T>();
+  const unsigned groups = (unsigned)((a.nitems + W - 1) / W);
+  hipLaunchKernelGGL(kernel, dim3(groups), dim3(kWave * W), 0, st, a);
 }
 
 template <class T, int NC, bool NSF, bool PL>
 static void launch_gp_pl(const StepArgs& a, int which, hipStream_t st) {
   if (which == kKernelPoly)
-    launch_items(k_poly<T, NC, NSF, PL>, a, st);
+    launch_items<T>(k_poly<T, NC, NSF, PL>, a, st);
   else if (a.maxwell && a.upd)
-    launch_items(k_grad<T, NC, true, NSF, PL, true>, a, st);
+    launch_items<T>(k_grad<T, NC, true, NSF, PL, true>, a, st);
   else if (a.maxwell)
-    launch_items(k_grad<T, NC, false, NSF, PL, true>, a, st);
+    launch_items<T>(k_grad<T, NC, false, NSF, PL, true>, a, st);
   else if (a.upd)
-    launch_items(k_grad<T, NC, true, NSF, PL, false>, a, st);
+    launch_items<T>(k_grad<T, NC, true, NSF, PL, false>, a, st);
   else
-    launch_items(k_grad<T, NC, false, NSF, PL, false>, a, st);
+    launch_items<T>(k_grad<T, NC, false, NSF, PL, false>, a, st);
 }
 
 template <class T, int NC, bool NSF>
