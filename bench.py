@@ -11,8 +11,10 @@ path is timed on identical inputs and reported beside it.
 Further workloads (--legs, default two_band_512, sweep_48 and
 thin_film_disk_1024) run after it for min(K, 50) steps each and are
 reported under "legs": both precisions, the dominant kernel's roofline
-(mask-active cells only for the thin film), and for the 48-string sweep the
-batched context against one context per string in both precisions.
+(mask-active cells only for the thin film) with its calibrated traffic at
+the leg's segment height (fixed for the single-class legs), and for the
+48-string sweep the batched context against one context per string in both
+precisions.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
                     [--legs a,b,c] [--leg-steps K2]
@@ -191,7 +193,8 @@ def leg(name, steps, warmup, tune_steps=4, separate=False):
            "active_cell_fraction": act / cells}
     res = {}
     for prec in ("mixed", "fp64"):
-        r = run_path(strings, prec, steps, warmup, tune_steps=tune_steps)
+        r = run_path(strings, prec, steps, warmup, tune_steps=tune_steps,
+                     segment_rows=LEG_ROWS.get(name, 0))
         res[prec] = r
         v = cells * r["steps"] / r["seconds"]
         roof = roofline(strings, r, 4 if prec == "mixed" else 8)
@@ -305,6 +308,11 @@ def measured_traffic(workload, precision, kernel, rows):
 
 
 LEGS = ("two_band_512", "sweep_48", "thin_film_disk_1024")
+# Segment heights of the single-class legs, fixed at heights whose HBM
+# traffic profiles/traffic.json records (the autotuner's picks for these
+# workloads vary from run to run within about 1 % of these heights' times);
+# the sweep's classes are autotuned.
+LEG_ROWS = {"two_band_512": 16, "thin_film_disk_1024": 25}
 
 
 def main(argv=None):
