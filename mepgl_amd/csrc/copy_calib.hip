@@ -4,7 +4,13 @@
 // at a given access width per lane.
 //   copy_calib [MiB] [repeats] [copy|read|write|triad] [bytes per lane]
 // copy: 1 read + 1 write; read: reads only; write: writes only; triad:
-// 3 reads + 1 write per element (the read/write mix of the P kernel).
+// 3 reads + 1 write per element (the read/write mix of the P kernel), one
+// element per thread; stream: the same mix from a grid of [waves per SIMD]
+// wavefronts per SIMD (default 4, as the mixed stencil kernels run) that
+// loops over the arrays, each wavefront issuing the loads of 4 elements of
+// every array before their stores (the stencil kernels' way of keeping
+// loads in flight).
+//   copy_calib [MiB] [repeats] [mode] [bytes per lane] [waves per SIMD]
 // Bytes per lane: 4 (one dword: the mixed-precision stencil kernels' load
 // width with one column per lane), 8 (dwordx2: FP64 with one column per
 // lane, or mixed with two), 16 (dwordx4, the default).
@@ -77,11 +83,40 @@ __global__ __launch_bounds__(256) void triad_kernel(const typename Elem<W>::T* _
 }
 
 template <int W>
-static int run(long long mib, int reps, const char* mode) {
+__global__ __launch_bounds__(256) void stream_kernel(const typename Elem<W>::T* __restrict__ a,
+                                                     const typename Elem<W>::T* __restrict__ b,
+                                                     const typename Elem<W>::T* __restrict__ c,
+                                                     typename Elem<W>::T* __restrict__ d,
+                                                     long long n) {
+  using T = typename Elem<W>::T;
+  constexpr int U = 4;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i0 = blockIdx.x * (long long)blockDim.x + threadIdx.x; i0 < n; i0 += U * stride) {
+    T x[U], y[U], z[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long i = i0 + u * stride;
+      if (i < n) {
+        x[u] = a[i];
+        y[u] = b[i];
+        z[u] = c[i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const long long i = i0 + u * stride;
+      if (i < n) d[i] = Elem<W>::mix(x[u], y[u], z[u]);
+    }
+  }
+}
+
+template <int W>
+static int run(long long mib, int reps, const char* mode, int waves) {
   using T = typename Elem<W>::T;
   const long long bytes = mib << 20;
   const long long n = bytes / (long long)sizeof(T);
-  const bool triad = !strcmp(mode, "triad");
+  const bool stream = !strcmp(mode, "stream");
+  const bool triad = !strcmp(mode, "triad") || stream;
   T *a = nullptr, *b = nullptr, *c = nullptr, *d = nullptr;
   if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&d, bytes) != hipSuccess) return 1;
   if (triad && (hipMalloc(&b, bytes) != hipSuccess || hipMalloc(&c, bytes) != hipSuccess)) return 1;
@@ -91,6 +126,9 @@ static int run(long long mib, int reps, const char* mode) {
   hipEvent_t e0, e1;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return 1;
   const unsigned blocks = (unsigned)((n + 255) / 256);
+  int cus = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, 0) != hipSuccess) return 1;
+  const unsigned sblocks = (unsigned)(cus * waves);  // 4-wavefront blocks: `waves` per SIMD
   double moved = 0;
   auto launch = [&]() {
     if (!strcmp(mode, "read")) {
@@ -99,6 +137,9 @@ static int run(long long mib, int reps, const char* mode) {
     } else if (!strcmp(mode, "write")) {
       hipLaunchKernelGGL(write_kernel<W>, dim3(blocks), dim3(256), 0, 0, d, n);
       moved = (double)bytes;
+    } else if (stream) {
+      hipLaunchKernelGGL(stream_kernel<W>, dim3(sblocks), dim3(256), 0, 0, a, b, c, d, n);
+      moved = 4.0 * bytes;
     } else if (triad) {
       hipLaunchKernelGGL(triad_kernel<W>, dim3(blocks), dim3(256), 0, 0, a, b, c, d, n);
       moved = 4.0 * bytes;
@@ -115,8 +156,12 @@ static int run(long long mib, int reps, const char* mode) {
   float ms = 0;
   hipEventElapsedTime(&ms, e0, e1);
   const double per = ms / reps;
-  printf("%s %lld MiB, %d bytes per lane: %.4f ms per pass, %.1f GB/s\n", mode, mib, W, per,
-         moved / (per * 1e-3) / 1e9);
+  if (stream)
+    printf("%s %lld MiB, %d bytes per lane, %d wavefronts per SIMD: %.4f ms per pass, %.1f GB/s\n",
+           mode, mib, W, waves, per, moved / (per * 1e-3) / 1e9);
+  else
+    printf("%s %lld MiB, %d bytes per lane: %.4f ms per pass, %.1f GB/s\n", mode, mib, W, per,
+           moved / (per * 1e-3) / 1e9);
   hipFree(a);
   hipFree(d);
   if (b) hipFree(b);
@@ -129,14 +174,15 @@ int main(int argc, char** argv) {
   const int reps = argc > 2 ? atoi(argv[2]) : 20;
   const char* mode = argc > 3 ? argv[3] : "copy";
   const int width = argc > 4 ? atoi(argv[4]) : 16;
-  if (mib <= 0 || mib > 16384 || reps <= 0) {
-    fprintf(stderr, "copy_calib: MiB in 1..16384 and repeats > 0\n");
+  const int waves = argc > 5 ? atoi(argv[5]) : 4;
+  if (mib <= 0 || mib > 16384 || reps <= 0 || waves < 1 || waves > 8) {
+    fprintf(stderr, "copy_calib: MiB in 1..16384, repeats > 0, 1..8 wavefronts per SIMD\n");
     return 2;
   }
   switch (width) {
-    case 4: return run<4>(mib, reps, mode);
-    case 8: return run<8>(mib, reps, mode);
-    case 16: return run<16>(mib, reps, mode);
+    case 4: return run<4>(mib, reps, mode, waves);
+    case 8: return run<8>(mib, reps, mode, waves);
+    case 16: return run<16>(mib, reps, mode, waves);
     default:
       fprintf(stderr, "copy_calib: bytes per lane must be 4, 8 or 16\n");
       return 2;
