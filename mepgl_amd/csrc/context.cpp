@@ -258,6 +258,20 @@ int run_aux(mepgl_ctx* c, const AuxArgs& a, int which, int arg) {
     if (int rc = settle_ls(c)) return rc;
   }
   if (c->dry) return MEPGL_OK;
+  if (which == kKernelArclen) {
+    // Launched once per item class, like the step kernels.
+    for (const auto& g : c->groups) {
+      AuxArgs ag = a;
+      ag.items = c->d_items + g.begin;
+      ag.nitems = g.count;
+      ag.nc = g.nc;
+      ag.pl = g.pl;
+      hipError_t e = launch_aux(ag, which, arg, c->fp32(), c->stream);
+      if (e != hipSuccess)
+        return fail(c, MEPGL_ERR_HIP, std::string("launch: ") + hipGetErrorString(e));
+    }
+    return MEPGL_OK;
+  }
   hipError_t e = launch_aux(a, which, arg, c->fp32(), c->stream);
   if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, std::string("launch: ") + hipGetErrorString(e));
   return MEPGL_OK;

@@ -192,7 +192,7 @@ __device__ void jb_body(const AuxArgs& a, const DevString& s, const WorkItem& it
     load_comp_const<double, NC>(s, off, k);
     const uint16_t nnv = s.nn[off];
     double b, jj[NC][2];
-    site_jb<NC>(nb, um, vm, uym, vym, k, s.inv_dx, nnv & 0xff, nnv >> 8, b, jj);
+    site_jb<double, NC>(nb, um, vm, uym, vym, k, s.inv_dx, nnv & 0xff, nnv >> 8, b, jj);
     O[off] = b;
 #pragma unroll
     for (int c = 0; c < NC; ++c) {
@@ -261,6 +261,11 @@ __device__ __forceinline__ ArcBufs arc_bufs(const AuxArgs& a, const DevString& s
 // updated image n to the other field buffer: the update and the arclength
 // pass share one sweep over the fields.  As in the G kernel, the fixed A of
 // a NO_SELF_FIELD string takes no update and is not stored.
+//
+// b and j are computed in double from the stored fields in both precisions:
+// on a nearly degenerate string, neighbouring images differ by less than
+// float rounding of b and j (two_band_512 in mixed precision moved energies
+// by 2.5e-5 relative after three outer iterations with float b and j).
 template <class T, int NC, bool PL, bool UPD>
 __device__ __forceinline__ void arclen_body(const DevString& s, const Strip& t, const ArcBufs& ab) {
   double acc_s[2] = {0, 0}, acc_c[2] = {0, 0};
@@ -325,7 +330,7 @@ __device__ __forceinline__ void arclen_body(const DevString& s, const Strip& t, 
         for (int q = 0; q < 3; ++q) nb.du[c][q] = nb.dv[c][q] = 0;
       }
       nb.h0 = 0;
-      site_jb<NC>(nb, um, vm, uym, vym, k, idx, wc.nn & 0xff, wc.nn >> 8, b, jj);
+      site_jb<double, NC>(nb, um, vm, uym, vym, k, idx, wc.nn & 0xff, wc.nn >> 8, b, jj);
     };
     // Row j: loads of row j + 2 for both images, the row's distances, the
     // loads into the previous row's slots (see the G kernel's loop).
@@ -423,37 +428,45 @@ __device__ __forceinline__ void arclen_body(const DevString& s, const Strip& t, 
   publish_partial<2, true>(acc_s, acc_c, s.part_a, t.n, t.tile, s.tiles_x * s.tiles_y);
 }
 
-template <class T, bool UPD>
-__device__ __forceinline__ void arclen_dispatch(const DevString& s, const Strip& t, const ArcBufs& ab) {
-  if (s.nc == 2) {
-    if (s.planes)
-      arclen_body<T, 2, true, UPD>(s, t, ab);
-    else
-      arclen_body<T, 2, false, UPD>(s, t, ab);
-  } else {
-    if (s.planes)
-      arclen_body<T, 1, true, UPD>(s, t, ab);
-    else
-      arclen_body<T, 1, false, UPD>(s, t, ab);
-  }
-}
-
 // The fused update is built for the mixed-precision fields only: in FP64
 // the update's direction loads would push the kernel past 256 VGPRs (one
-// wavefront per SIMD), so FP64 applies the update with k_update first.
-template <class T>
+// wavefront per SIMD), so FP64 applies the update with k_update first.  One
+// launch per item class (components, parameter planes), specialised like
+// the step kernels, so that each class runs at its own register count.
+template <class T, int NC, bool PL, bool UPD>
 __global__ __launch_bounds__(kBlock) void k_arclen(AuxArgs a) {
   const WorkItem it = load_item(a.items, xcd_item(blockIdx.x, a.nitems));
   const DevString& s = a.strings[it.string];
   const Strip t = strip_of(s, it);
   const ArcBufs ab = arc_bufs<T>(a, s, t);
+  arclen_body<T, NC, PL, UPD>(s, t, ab);
+}
+
+template <class T, bool UPD>
+static void launch_arclen_upd(const AuxArgs& a, hipStream_t st) {
+  const dim3 g((unsigned)a.nitems), b(kBlock);
+  if (a.nc == 2) {
+    if (a.pl)
+      hipLaunchKernelGGL((k_arclen<T, 2, true, UPD>), g, b, 0, st, a);
+    else
+      hipLaunchKernelGGL((k_arclen<T, 2, false, UPD>), g, b, 0, st, a);
+  } else {
+    if (a.pl)
+      hipLaunchKernelGGL((k_arclen<T, 1, true, UPD>), g, b, 0, st, a);
+    else
+      hipLaunchKernelGGL((k_arclen<T, 1, false, UPD>), g, b, 0, st, a);
+  }
+}
+
+template <class T>
+static void launch_arclen(const AuxArgs& a, hipStream_t st) {
   if constexpr (sizeof(T) == 4) {
     if (a.upd) {
-      arclen_dispatch<T, true>(s, t, ab);
+      launch_arclen_upd<T, true>(a, st);
       return;
     }
   }
-  arclen_dispatch<T, false>(s, t, ab);
+  launch_arclen_upd<T, false>(a, st);
 }
 
 // computeArcLength (interpolation.cu:126-140) for both metrics, then the
@@ -895,7 +908,7 @@ static hipError_t launch_aux_impl(const AuxArgs& a, int which, int arg, hipStrea
       if (a.nitems > 0) hipLaunchKernelGGL(k_jb<T>, g, b, 0, st, a);
       break;
     case kKernelArclen:
-      if (a.nitems > 0) hipLaunchKernelGGL(k_arclen<T>, g, b, 0, st, a);
+      if (a.nitems > 0) launch_arclen<T>(a, st);
       break;
     case kKernelArcFinalize:
       hipLaunchKernelGGL(k_arc_finalize, dim3(arg), dim3(kArcBlock), 0, st, a, arg);

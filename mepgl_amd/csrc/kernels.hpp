@@ -66,6 +66,8 @@ struct AuxArgs {
   double* out;          // per global image (energy)
   double* const* jb;    // per string JB planes (device array of pointers)
   int only_string;      // scalar maintenance: string index, or -1 for all
+  int nc, pl;           // arclength: class of every item in `items`
+                        // (components, parameter planes)
 };
 
 // Arguments of the per-image finishing launches (one wavefront per image).

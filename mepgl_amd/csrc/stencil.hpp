@@ -506,42 +506,44 @@ __device__ __forceinline__ double site_energy(const PolyNbhd<double, NC>& s,
 }
 
 // Magnetic field and supercurrents of one site, computeJBKernel
-// nlcg.cu:243-364, in double.  Uses own-site A and no bond weights.
-template <int NC>
-__device__ __forceinline__ void site_jb(const PolyNbhd<double, NC>& s, const double (&um)[NC],
-                                        const double (&vm)[NC], const double (&uym)[NC],
-                                        const double (&vym)[NC], const CompConst<double> (&k)[NC],
-                                        double idx, int sc, int cm, double& b,
-                                        double (&j)[NC][2]) {
-  const double yf = (s.ay[1] - s.ay[0]) * idx, yb = (s.ay[0] - s.ay[2]) * idx;
-  const double xf = (s.ax[3] - s.ax[0]) * idx, xb = (s.ax[0] - s.ax[4]) * idx;
-  double bb = 0;
-  if (bitof(cm, 0) && bitof(cm, 1) && bitof(cm, 2)) bb += 0.25 * (yf - xf);
-  if (bitof(cm, 2) && bitof(cm, 3) && bitof(cm, 4)) bb += 0.25 * (yb - xf);
-  if (bitof(cm, 4) && bitof(cm, 5) && bitof(cm, 6)) bb += 0.25 * (yb - xb);
-  if (bitof(cm, 6) && bitof(cm, 7) && bitof(cm, 0)) bb += 0.25 * (yf - xb);
-  if (bb != 0) bb /= quad_count(cm) * 0.25;
+// nlcg.cu:243-364, in R (double for the JB output and the FP64 arclength;
+// float in the mixed-precision arclength, as the mixed step kernels' own
+// stencils).  Uses own-site A and no bond weights.
+template <class R, int NC>
+__device__ __forceinline__ void site_jb(const PolyNbhd<R, NC>& s, const R (&um)[NC],
+                                        const R (&vm)[NC], const R (&uym)[NC],
+                                        const R (&vym)[NC], const CompConst<R> (&k)[NC], R idx,
+                                        int sc, int cm, R& b, R (&j)[NC][2]) {
+  const R yf = (s.ay[1] - s.ay[0]) * idx, yb = (s.ay[0] - s.ay[2]) * idx;
+  const R xf = (s.ax[3] - s.ax[0]) * idx, xb = (s.ax[0] - s.ax[4]) * idx;
+  const R qr = (R)0.25;
+  R bb = 0;
+  if (bitof(cm, 0) && bitof(cm, 1) && bitof(cm, 2)) bb += qr * (yf - xf);
+  if (bitof(cm, 2) && bitof(cm, 3) && bitof(cm, 4)) bb += qr * (yb - xf);
+  if (bitof(cm, 4) && bitof(cm, 5) && bitof(cm, 6)) bb += qr * (yb - xb);
+  if (bitof(cm, 6) && bitof(cm, 7) && bitof(cm, 0)) bb += qr * (yf - xb);
+  if (bb != 0) bb /= (R)quad_count(cm) * qr;
   b = bb;
 #pragma unroll
   for (int c = 0; c < NC; ++c) {
-    const double q = k[c].q, u0 = s.u[c][0], v0 = s.v[c][0];
-    const double ax0 = s.ax[0], ay0 = s.ay[0];
-    double jx = 0, jy = 0;
-    const double fx = q * k[c].hx, fy = q * k[c].hy;  // 0.5 q / m
+    const R q = k[c].q, u0 = s.u[c][0], v0 = s.v[c][0];
+    const R ax0 = s.ax[0], ay0 = s.ay[0];
+    R jx = 0, jy = 0;
+    const R fx = q * k[c].hx, fy = q * k[c].hy;  // 0.5 q / m
     if (bitof(sc, 0)) {
-      const double Du = (s.u[c][1] - u0) * idx, Dv = (s.v[c][1] - v0) * idx;
+      const R Du = (s.u[c][1] - u0) * idx, Dv = (s.v[c][1] - v0) * idx;
       jx += fx * (v0 * (-Du - q * ax0 * v0) + u0 * (Dv - q * ax0 * u0));
     }
     if (bitof(sc, 4)) {
-      const double Du = (u0 - um[c]) * idx, Dv = (v0 - vm[c]) * idx;
+      const R Du = (u0 - um[c]) * idx, Dv = (v0 - vm[c]) * idx;
       jx += fx * (v0 * (-Du - q * ax0 * v0) + u0 * (Dv - q * ax0 * u0));
     }
     if (bitof(sc, 2)) {
-      const double Du = (s.u[c][2] - u0) * idx, Dv = (s.v[c][2] - v0) * idx;
+      const R Du = (s.u[c][2] - u0) * idx, Dv = (s.v[c][2] - v0) * idx;
       jy += fy * (v0 * (-Du - q * ay0 * v0) + u0 * (Dv - q * ay0 * u0));
     }
     if (bitof(sc, 6)) {
-      const double Du = (u0 - uym[c]) * idx, Dv = (v0 - vym[c]) * idx;
+      const R Du = (u0 - uym[c]) * idx, Dv = (v0 - vym[c]) * idx;
       jy += fy * (v0 * (-Du - q * ay0 * v0) + u0 * (Dv - q * ay0 * u0));
     }
     j[c][0] = jx;
