@@ -411,14 +411,17 @@ std::string first_nonfinite_image(mepgl_ctx* c) {
   return "";
 }
 
-// Reads and clears the non-finite flag once the stream is idle.
+// Reads and clears the non-finite flag once the stream is idle.  Every
+// value first_nonfinite_image checks is also checked by the kernel that
+// writes it (the gradient totals by P, the energy and line-search
+// coefficients by the line search), so the scalars are read back only to
+// name the image once the flag is set.
 int check_finite(mepgl_ctx* c, const std::string& where = "") {
   HIP_TRY(c, hipStreamSynchronize(c->stream));
-  const int flag = *(volatile int*)c->health;
-  const std::string img = first_nonfinite_image(c);
-  if (flag || !img.empty()) {
+  if (*(volatile int*)c->health) {
     *(volatile int*)c->health = 0;
-    return fail(c, MEPGL_ERR_NONFINITE, "non-finite energy or gradient norm" + where + img);
+    return fail(c, MEPGL_ERR_NONFINITE,
+                "non-finite energy or gradient norm" + where + first_nonfinite_image(c));
   }
   return MEPGL_OK;
 }
