@@ -572,3 +572,32 @@ def test_folded_line_search_equals_finishing_launch(precision, N, rows):
         np.testing.assert_array_equal(x, y)
     for x, y in zip(s1, s0):
         np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_reset_makes_the_run_independent_of_history(precision):
+    """mepgl_reset_solver makes the current field and direction buffers the
+    context's buffer 0 (pointer swaps in the descriptors).  A context that
+    ran outer iterations and NLCG steps of both parities before an upload
+    and a reset then continues exactly as a fresh context from the same
+    upload: fields, direction and scalars bit for bit, through graph
+    replays."""
+    s = small_string(N=40, F=5, seed=43, multicomponent=True, masked=True)
+    say(f"[gpu] reset after history vs fresh context: {precision}")
+    out = []
+    for history in (False, True):
+        with solver_for([s], precision=precision) as sv:
+            if history:
+                sv.loop(3, 5, "L")
+                sv.nlcgSteps(3)
+                sv.upload(0, s)
+                sv.reset_solver()
+            sv.loop(3, 5, "L")
+            out.append((sv.download(0), sv.download_state(0, "d"),
+                        [sv.scalars(0, k) for k in ("fenergy", "alpha", "s_jb")]))
+    (f1, d1, s1), (f0, d0, s0) = out
+    for x, y in zip(f1 + d1, f0 + d0):
+        if x is not None:
+            np.testing.assert_array_equal(x, y)
+    for x, y in zip(s1, s0):
+        np.testing.assert_array_equal(x, y)
