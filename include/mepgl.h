@@ -287,7 +287,7 @@ int mepgl_synchronize(mepgl_ctx *ctx);
  * size (config.cuh:25-31, blockSizeX); here one wavefront marches a
  * 64-column strip down `rows` rows (heights of 3k + 1 rows waste no work).
  * mepgl_set_segment_rows sets one string's height (clamped to
- * [min(N, 4), N]).  mepgl_autotune times three rounds per candidate height
+ * [min(N, 4), N]).  mepgl_autotune times five rounds per candidate height
  * (4 .. 127) of an outer iteration of mode L with `steps` NLCG steps
  * (arclength, linear redistribution, the steps; only the steps when a string
  * has one image), each round visiting every candidate in turn, for each

@@ -932,7 +932,7 @@ int refresh_snapshot(mepgl_ctx* c, Snapshot& sn) {
 }
 
 // Timing rounds per candidate height.
-constexpr int kTuneRounds = 3;
+constexpr int kTuneRounds = 5;
 
 // Milliseconds per NLCG step of the current schedule, starting from the
 // snapshot: one untimed step, then an outer iteration of mode L with `steps`

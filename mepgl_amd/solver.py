@@ -297,7 +297,7 @@ class Solver:
         self._check(self._L.mepgl_set_segment_rows(self._ctx, i, int(rows)))
 
     def autotune(self, steps: int = 5):
-        """Pick each string class's segment height by timing three rounds per
+        """Pick each string class's segment height by timing five rounds per
         candidate of an outer iteration of mode L with `steps` NLCG steps
         (each round visits every candidate in turn; the fastest round counts);
         the solver state is left unchanged."""
