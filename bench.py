@@ -7,8 +7,8 @@ A "step" is one NLCG iteration over all images; the timed region runs
 `--steps` of them (default 1000 = 100 outer iterations) on fields already
 resident in HBM.  Before it: `--warmup` steps, the autotuner's trial
 rounds (the state is restored), a restart from the initial guess and at
-least 4 untimed outer iterations from there, after which the timed steps
-follow at once.  `value` is the mixed-precision path's throughput; the FP64
+least 4 untimed outer iterations and 50 ms from there, after which the
+timed steps follow at once.  `value` is the mixed-precision path's throughput; the FP64
 path is timed on identical inputs and reported beside it.
 
 Further workloads (--legs, default two_band_512, sweep_48 and
@@ -50,6 +50,23 @@ def barrier(e):
 KERNEL_NAMES = ((0, "grad_update"), (1, "poly"), (2, "grad_first"), (4, "finish_poly"))
 
 
+WARM_S = 0.05  # least untimed device time right before a timed region
+
+
+def warm_up(sv, iters):
+    """Untimed outer iterations right before a timed region: at least
+    `iters`, in pairs, and at least WARM_S seconds of them.  The device
+    reaches its steady speed only after some 30 ms of load (vortex_entry_256
+    mixed: 0.143-0.151 ms per step after 4 outer iterations, 0.136 after 20
+    or more); the outer iterations' buffer parity repeats within two, so
+    every graph the timed steps replay has been captured by the end."""
+    t0, done = time.perf_counter(), 0
+    while done < iters or time.perf_counter() - t0 < WARM_S:
+        sv.loop(2, R, "L")
+        sv.synchronize()
+        done += 2
+
+
 def run_path(strings, precision, steps, warmup, profile_iters=3, segment_rows=0, tune=True,
              tune_steps=10):
     from mepgl_amd.solver import Solver
@@ -57,22 +74,19 @@ def run_path(strings, precision, steps, warmup, profile_iters=3, segment_rows=0,
     sv = Solver(strings, precision=precision, segment_rows=segment_rows)
     iters = max(steps // R, 1)
     w_iters = max(warmup // R, 1)
-    sv.loop(w_iters, R, "L")
-    sv.synchronize()
+    # The warm-up steps, on a device brought to its steady speed before the
+    # autotuner times its candidates.
+    warm_up(sv, w_iters)
     # The library's autotuner picks each string class's segment height on
     # the live problem (the state is restored afterwards).
     rows = sv.autotune(steps=tune_steps) if (tune and not segment_rows) else None
     # Re-start from the initial guess, so that both precisions run the same
-    # trajectory, and run the warm-up steps from there: at least four outer
-    # iterations (a new schedule drops the captured graphs, and the outer
-    # iterations' buffer parity repeats within two, so every graph the timed
-    # steps replay is captured by then).  The timed steps follow at once on
-    # a busy device.
+    # trajectory, and warm up from there (a new schedule also drops the
+    # captured graphs); the timed steps follow at once.
     for i, s in enumerate(strings):
         sv.upload(i, s)
     sv.reset_solver()
-    sv.loop(max(w_iters, 4), R, "L")
-    sv.synchronize()
+    warm_up(sv, max(w_iters, 4))
     t0 = time.perf_counter()
     sv.loop(iters, R, "L")
     sv.synchronize()
@@ -111,8 +125,7 @@ def run_separately(strings, precision, steps, warmup, tune_steps=2):
         # As in run_path: restart, warm up, then time at once.
         sv.upload(0, s)
         sv.reset_solver()
-        sv.loop(max(warmup // R, 4), R, "L")
-        sv.synchronize()
+        warm_up(sv, max(warmup // R, 4))
         t0 = time.perf_counter()
         sv.loop(iters, R, "L")
         sv.synchronize()
