@@ -329,9 +329,6 @@ int flush(mepgl_ctx* c) {
   return MEPGL_OK;
 }
 
-// One NLCG step k of a call (nlcg.cu:1478-1583).  The reference swaps local
-// copies of the gradient pointers (nlcg.cu:1484-1498), so step k writes the
-// caller's g_old buffer when k is even and its g buffer when k is odd.
 // Whether G takes over P's line search: only when every string has at most
 // one tile per lane of the partials' reduction.  With more tiles per image
 // every G wavefront waits for several rounds of partial loads before its
@@ -345,6 +342,9 @@ bool fold_line_search(const mepgl_ctx* c) {
   return true;
 }
 
+// One NLCG step k of a call (nlcg.cu:1478-1583).  The reference swaps local
+// copies of the gradient pointers (nlcg.cu:1484-1498), so step k writes the
+// caller's g_old buffer when k is even and its g buffer when k is odd.
 // `more`: another step follows at once, so P's line search may be left to
 // that step's G (every G wavefront scans the table from P's partials), which
 // saves the finishing launch and one kernel boundary per step.
