@@ -163,7 +163,7 @@ def roofline(strings, r, sbytes):
     the number of item classes (each class is one launch per step)."""
     ncls = kernel_classes(strings)
     act = [active_cells(s) for s in strings]
-    out = None
+    out, others = None, {}
     for name, per_cf in (("grad_update", 5), ("poly", 4)):
         ms, n = r["times"][name]
         if not n:
@@ -180,7 +180,12 @@ def roofline(strings, r, sbytes):
         share = ms * n
         if out is None or share > out[0]:
             out = (share, cand)
-    return out[1] if out else None
+        others[name] = {"achieved": cand["achieved"], "frac": cand["frac"], "avg_ms": ms}
+    if out is None:
+        return None
+    # Both step kernels' figures, the dominant one included.
+    out[1]["step_kernels"] = others
+    return out[1]
 
 
 def barrier_errors(e_mixed, e_fp64):
@@ -215,10 +220,12 @@ def leg(name, steps, warmup, tune_steps=4, separate=False):
         roof = roofline(strings, r, 4 if prec == "mixed" else 8)
         rows = sorted(set(r["info"]["rows"]))
         if roof is not None:
-            # Calibrated HBM bytes per launch at the height the autotuner
-            # picked, when a single class ran at one recorded height.
-            roof["traffic"] = (measured_traffic(name, prec, roof["kernel"], rows[0])
-                               if len(rows) == 1 and roof["launches_per_step"] == 1 else None)
+            # Calibrated HBM bytes per launch at the leg's height, when a
+            # single class ran at one recorded height.
+            single = len(rows) == 1 and roof["launches_per_step"] == 1
+            roof["traffic"] = measured_traffic(name, prec, roof["kernel"], rows[0]) if single else None
+            for k, v in roof["step_kernels"].items():
+                v["traffic"] = measured_traffic(name, prec, k, rows[0]) if single else None
         out[prec] = {"value": v, "value_active_cells": v * act / cells,
                      "ms_per_step": 1e3 * r["seconds"] / r["steps"], "steps": r["steps"],
                      "segment_rows": rows, "roofline": roof}
@@ -399,6 +406,9 @@ def main(argv=None):
         "frac_of_triad_ceiling": roof["achieved"] / ceil["triad"] if ceil.get("triad") else None})
     roof64["traffic"] = measured_traffic(args.workload, "fp64", roof64["kernel"],
                                          fp64["info"]["tile_rows"])
+    for rf, prec, rows in ((roof, "mixed", rows_m), (roof64, "fp64", fp64["info"]["tile_rows"])):
+        for k, v in rf["step_kernels"].items():
+            v["traffic"] = measured_traffic(args.workload, prec, k, rows)
 
     out = {
         "metric": "image*cells/s per NLCG step (mixed precision)",
