@@ -144,20 +144,24 @@ __device__ __forceinline__ void grad_body(const StepArgs& a, const DevString& s,
   using S0 = std::integral_constant<int, 0>;
   using S1 = std::integral_constant<int, 1>;
   using S2 = std::integral_constant<int, 2>;
-  // Tiles without bonds (kItemNoBonds): g, d and every partial stay exact
-  // zeros (the loop never makes them non-zero there), so only the pending
-  // update of the fields is carried over, in the expression the window uses.
-  auto carry_update = [&]() {
-    if constexpr (UPD) {
-      if (!store) return;
-      constexpr int P0 = NSF ? 2 : 0;  // fixed A is not stored
-      for (int j = t.j0; j < t.j1; ++j) {
-        const unsigned o = (unsigned)((j * P_ + t.ic) * (int)sizeof(T));
+  // Sites without bonds (an empty neighbour map: outside both domains)
+  // have exact zeros for g and d, so an update leaves their fields as they
+  // are.  Both field buffers hold the same values there: only uploads (which
+  // write both buffers), the reparametrisation and the gauge alignments
+  // change such a site's fields, the last two leave no update pending, and
+  // the next G, one without the update, copies those sites' fields to the
+  // other buffer (sync_sites).  G with the update then skips tiles without
+  // bonds (kItemNoBonds) altogether, and writes x' = x at the other such
+  // sites with the rest of their row.
+  const bool sync_sites = !UPD && !a.scratch;
+  auto copy_tile = [&]() {
+    if (!sync_sites || !store) return;
+    constexpr int P0 = NSF ? 2 : 0;  // fixed A is not stored
+    for (int j = t.j0; j < t.j1; ++j) {
+      const unsigned o = (unsigned)((j * P_ + t.ic) * (int)sizeof(T));
 #pragma unroll
-        for (int p = P0; p < GRaw<T, NC>::P; ++p)
-          bstore(fma(al, bload<T>(b.d, o, p * b.ps), bload<T>(b.x, o, p * b.ps)), b.xo, o,
-                 p * b.ps);
-      }
+      for (int p = P0; p < GRaw<T, NC>::P; ++p)
+        bstore(bload<T>(b.x, o, p * b.ps), b.xo, o, p * b.ps);
     }
   };
   auto march = [&](auto up_tag) {
@@ -294,7 +298,7 @@ __device__ __forceinline__ void grad_body(const StepArgs& a, const DevString& s,
           bstore(ga[0], b.g, o, PAX * b.ps);
           bstore(ga[1], b.g, o, PAY * b.ps);
         }
-        if (UPD) {
+        if (UPD || (sync_sites && wc.nn == 0)) {
           if (do_a) {
             bstore(wc.ax, b.xo, o, PAX * b.ps);
             bstore(wc.ay, b.xo, o, PAY * b.ps);
@@ -321,7 +325,7 @@ __device__ __forceinline__ void grad_body(const StepArgs& a, const DevString& s,
     row(j, S0(), S1(), S2(), std::false_type());
   };
   if (it.flags & kItemNoBonds)
-    carry_update();
+    copy_tile();
   else if (up)
     march(std::true_type());
   else

@@ -601,3 +601,40 @@ def test_reset_makes_the_run_independent_of_history(precision):
             np.testing.assert_array_equal(x, y)
     for x, y in zip(s1, s0):
         np.testing.assert_array_equal(x, y)
+
+
+def test_sites_without_bonds_across_height_changes(oracle):
+    """G with the pending update skips tiles without bonds, relying on both
+    field buffers holding the same values at every site without bonds (G
+    without the update copies them after a reparametrisation).  Changing the
+    segment height between calls moves sites between tiles with and without
+    bonds; fields everywhere, outside the domain included (where the
+    redistribution moves the non-zero initial values), must still follow the
+    oracle through linear and cubic reparametrisations and plain steps."""
+    s = bondless_string()
+    say("[gpu] sites without bonds across segment-height changes (fp64)")
+    ref = oracle.OracleString(s)
+    ref.loop(1, 3, "L")
+    ref.nlcg_steps(2)
+    ref.loop(1, 3, "C")
+    ref.nlcg_steps(3)
+    with solver_for([s]) as sv:
+        sv.set_segment_rows(0, 7)
+        sv.loop(1, 3, "L")
+        sv.set_segment_rows(0, 16)
+        sv.nlcgSteps(2)
+        sv.set_segment_rows(0, 4)
+        sv.loop(1, 3, "C")
+        sv.set_segment_rows(0, 7)
+        sv.nlcgSteps(3)
+        a, p1, p2 = sv.download(0)
+    assert normwise(a, ref.a) < 1e-10
+    assert normwise(p1, ref.psi_1) < 1e-10
+    assert normwise(p2, ref.psi_2) < 1e-10
+    # Sites without bonds (both domains are the same disk), [i][j] as the
+    # fields' file layout [n][i][j].
+    bondless = oracle.build_nn_map(s.comp_domain) == 0
+    assert bondless.any()
+    for got, want in ((p1, ref.psi_1), (p2, ref.psi_2)):
+        g, w = got[:, bondless], want[:, bondless]
+        assert np.max(np.abs(g - w)) <= 1e-10 * np.max(np.abs(w))
