@@ -224,8 +224,8 @@ def leg(name, steps, warmup, tune_steps=4, separate=False):
             # single class ran at one recorded height.
             single = len(rows) == 1 and roof["launches_per_step"] == 1
             roof["traffic"] = measured_traffic(name, prec, roof["kernel"], rows[0]) if single else None
-            for k, v in roof["step_kernels"].items():
-                v["traffic"] = measured_traffic(name, prec, k, rows[0]) if single else None
+            for kname, kfig in roof["step_kernels"].items():
+                kfig["traffic"] = measured_traffic(name, prec, kname, rows[0]) if single else None
         out[prec] = {"value": v, "value_active_cells": v * act / cells,
                      "ms_per_step": 1e3 * r["seconds"] / r["steps"], "steps": r["steps"],
                      "segment_rows": rows, "roofline": roof}
@@ -407,8 +407,8 @@ def main(argv=None):
     roof64["traffic"] = measured_traffic(args.workload, "fp64", roof64["kernel"],
                                          fp64["info"]["tile_rows"])
     for rf, prec, rows in ((roof, "mixed", rows_m), (roof64, "fp64", fp64["info"]["tile_rows"])):
-        for k, v in rf["step_kernels"].items():
-            v["traffic"] = measured_traffic(args.workload, prec, k, rows)
+        for kname, kfig in rf["step_kernels"].items():
+            kfig["traffic"] = measured_traffic(args.workload, prec, kname, rows)
 
     out = {
         "metric": "image*cells/s per NLCG step (mixed precision)",

@@ -128,3 +128,52 @@ def test_bench_roofline_and_traffic_lookup():
             assert 1.0 <= b / alg < 1.2
     assert bench.measured_traffic("vortex_entry_256", "mixed", "poly", 5) is None
     assert np.isfinite(bench.barrier([3.0, 4.0, 2.5])[0])
+
+
+def test_bench_json_line_contract(monkeypatch, capsys):
+    """bench.main's JSON line, with the GPU runs replaced by synthetic
+    results: the keys the benchmark contract names, a leg with its roofline
+    and traffic fields, and a refusal for more than one GPU."""
+    import bench
+
+    def fake_run_path(strings, precision, steps, warmup, segment_rows=0, **kw):
+        iters = max(steps // bench.R, 1)
+        ms = {"mixed": 0.135, "fp64": 0.265}[precision]
+        times = {"grad_update": (0.068, 27), "poly": (0.060, 30), "grad_first": (0.044, 3),
+                 "finish_poly": (0.008, 3)}
+        rows = segment_rows or 16
+        return dict(seconds=iters * bench.R * ms * 1e-3, steps=iters * bench.R,
+                    energies=[np.linspace(1.0, 2.0, s.F) for s in strings], times=times,
+                    info={"tile_rows": rows, "rows": [rows] * len(strings)}, profile_steps=30)
+
+    monkeypatch.setattr(bench, "run_path", fake_run_path)
+    monkeypatch.setitem(configs.WORKLOADS, "two_band_small",
+                        lambda: [configs.two_band_entry(64, 0.25, 5, 0.8, seed=3)])
+    monkeypatch.setattr(bench, "streaming_ceilings",
+                        lambda: {"copy": 6200.0, "triad": 5650.0, "triad_4B": 6150.0,
+                                 "triad_8B": 5900.0})
+    rc = bench.main(["--steps", "20", "--warmup", "5", "--no-cpu", "--workload",
+                     "vortex_entry_small", "--legs", "vortex_entry_small,two_band_small"])
+    assert rc == 0
+    line = [l for l in capsys.readouterr().out.splitlines() if l.startswith("{")][-1]
+    d = json.loads(line)
+    for key in ("metric", "value", "unit", "n_gpus", "steps", "warmup", "ms_per_step",
+                "higher_is_better", "scaling", "vs_baseline", "dtype", "data", "config",
+                "roofline", "fp64", "mixed_over_fp64"):
+        assert key in d, key
+    assert d["n_gpus"] == 1 and d["steps"] == 20 and d["warmup"] == 5
+    assert d["ms_per_step"] == pytest.approx(0.135)
+    roof = d["roofline"]
+    for key in ("bound", "achieved", "peak", "unit", "frac", "traffic", "step_kernels"):
+        assert key in roof, key
+    assert set(roof["step_kernels"]) == {"grad_update", "poly"}
+    assert d["config"]["workload"] == "vortex_entry_small"
+    # The headline workload is not repeated as a leg.
+    assert list(d["legs"]) == ["two_band_small"]
+    leg = d["legs"]["two_band_small"]
+    for prec in ("mixed", "fp64"):
+        assert leg[prec]["value"] > 0 and leg[prec]["roofline"]["frac"] > 0
+        assert "traffic" in leg[prec]["roofline"]
+        assert "traffic" in leg[prec]["roofline"]["step_kernels"]["poly"]
+    assert leg["mixed_over_fp64"] == pytest.approx(0.265 / 0.135)
+    assert bench.main(["--gpus", "2"]) == 2
