@@ -30,6 +30,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <utility>
 #include <vector>
 
 #include "mepgl.h"
@@ -57,6 +58,11 @@ struct Options {
   double dx = 0.0;  // 0 = from input
   bool stdin_keys = true;
   bool dry_run = false;
+  // The FP64 statement of a mixed-precision run's error: 'c' continues the
+  // end state in FP64 for fp64_check_iterations outer iterations, 's'
+  // repeats the whole run in FP64 from the input, 0 skips it.
+  char fp64_check = 'c';
+  int fp64_check_iterations = 1;
   std::string root = ".";
 };
 
@@ -86,6 +92,9 @@ void usage() {
       "  --gauge-align              align neighbouring image phases in modes L, C\n"
       "  --no-graph                 launch eagerly instead of replaying a hipGraph\n"
       "  --segment-rows arg         rows per wavefront (default: chosen by the library)\n"
+      "  --fp64-check arg           with --precision mixed: error against the FP64 path,\n"
+      "                             continue (default), shadow or off\n"
+      "  --fp64-check-iterations arg  outer iterations of the FP64 continuation (default 1)\n"
       "  --no-stdin                 do not read the q + - C keys\n"
       "  --dry-run                  load and validate the input, print its summary, exit\n"
       "  --root arg                 directory holding simulations/ (default .)\n");
@@ -103,7 +112,8 @@ Options parse(int argc, char** argv) {
   Options o;
   static const char* with_value[] = {"simname", "mode", "Fin", "Fout", "iterations", "device",
                                      "dx", "ncomp", "relaxation-steps", "precision",
-                                     "segment-rows", "root"};
+                                     "segment-rows", "root", "fp64-check",
+                                     "fp64-check-iterations"};
   for (int i = 1; i < argc; ++i) {
     std::string a = argv[i];
     if (a == "-h" || a == "--help") {
@@ -137,6 +147,12 @@ Options parse(int argc, char** argv) {
     else if (key == "relaxation-steps") o.relaxation_steps = to_int(key, val);
     else if (key == "segment-rows") o.segment_rows = to_int(key, val);
     else if (key == "root") o.root = val;
+    else if (key == "fp64-check") {
+      if (val == "continue") o.fp64_check = 'c';
+      else if (val == "shadow") o.fp64_check = 's';
+      else if (val == "off") o.fp64_check = 0;
+      else die("--fp64-check expects continue, shadow or off");
+    } else if (key == "fp64-check-iterations") o.fp64_check_iterations = to_int(key, val);
     else if (key == "dx") {
       char* end = nullptr;
       o.dx = std::strtod(val.c_str(), &end);
@@ -163,6 +179,7 @@ Options parse(int argc, char** argv) {
   if (o.iterations < 0) die("--iterations is required and must be >= 0");
   if (o.relaxation_steps < 1) die("--relaxation-steps must be >= 1");
   if (o.ncomp != 0 && o.ncomp != 1 && o.ncomp != 2) die("--ncomp must be 1 or 2");
+  if (o.fp64_check_iterations < 1) die("--fp64-check-iterations must be >= 1");
   return o;
 }
 
@@ -430,6 +447,86 @@ void save_output(const Options& o, const Problem& p, int F, const std::vector<do
   }
 }
 
+// Relative difference, absolute where the reference value is zero.
+double rel_diff(double x, double ref) {
+  return ref != 0.0 ? std::fabs(x - ref) / std::fabs(ref) : std::fabs(x - ref);
+}
+
+// Entry and exit barriers of a string (post.py:188-189): max(F - F_0) and
+// max(F - F_last).
+std::pair<double, double> barriers(const std::vector<double>& e) {
+  double in = -INFINITY, out = -INFINITY;
+  for (double v : e) {
+    in = std::max(in, v - e.front());
+    out = std::max(out, v - e.back());
+  }
+  return {in, out};
+}
+
+// The error of a mixed-precision run against the FP64 path, per image and
+// for the barriers, printed and written to output_data/fp64_check.npy as
+// (F_out, 2) float64: the mixed run's energies, then the FP64 path's.
+//  'c'  relaxes the mixed end state further in FP64, for
+//       o.fp64_check_iterations outer iterations with the run's last
+//       relaxation-step count and NLCG setting: on a relaxed string the
+//       difference is what float arithmetic left in it; on a string still
+//       relaxing it also holds that further relaxation, so it bounds the
+//       precision error from above.
+//  's'  repeats the run's iterations in FP64 from the input (the same
+//       relaxation-step counts and NLCG settings, and the same resampling to
+//       F_out), which also counts where the two trajectories part.
+void fp64_check(const Options& o, mepgl_string_desc d, const Problem& p, int F_out,
+                const std::vector<double>& e_mixed, const std::vector<double>& a,
+                const std::vector<double>& psi_1, const std::vector<double>& psi_2,
+                const std::vector<std::pair<int, int>>& schedule, int relax, int nlcg) {
+  Lib chk;
+  const bool two = p.ncomp == 2;
+  char what[160];
+  if (o.fp64_check == 'c') {
+    d.F = F_out;
+    chk.check(mepgl_create(&d, 1, MEPGL_FP64, o.flags, o.device, &chk.ctx), "mepgl_create");
+    chk.check(mepgl_upload(chk.ctx, 0, a.data(), psi_1.data(), two ? psi_2.data() : nullptr),
+              "mepgl_upload");
+    chk.check(mepgl_loop(chk.ctx, o.fp64_check_iterations, relax, o.mode, nlcg), "mepgl_loop");
+    std::snprintf(what, sizeof what, "the end state relaxed further in FP64, %d iteration(s) of %d steps",
+                  o.fp64_check_iterations, relax);
+  } else {
+    d.F = p.F;
+    chk.check(mepgl_create(&d, 1, MEPGL_FP64, o.flags, o.device, &chk.ctx), "mepgl_create");
+    chk.check(mepgl_upload(chk.ctx, 0, p.a.data(), p.psi_1.data(), two ? p.psi_2.data() : nullptr),
+              "mepgl_upload");
+    for (const auto& it : schedule) chk.check(mepgl_loop(chk.ctx, 1, it.first, o.mode, it.second), "mepgl_loop");
+    if (F_out != p.F) chk.check(mepgl_resample(chk.ctx, 0, F_out), "mepgl_resample");
+    std::snprintf(what, sizeof what, "the same %zu iteration(s) run in FP64 from the input",
+                  schedule.size());
+  }
+  chk.check(mepgl_free_energy(chk.ctx), "mepgl_free_energy");
+  const std::vector<double> e64 = chk.scalars(MEPGL_SCALAR_ENERGY, F_out);
+
+  std::printf("\nMixed precision against the FP64 path (%s):\n", what);
+  std::printf("┌──────┬─────────────────┬─────────────────┬────────────┐\n");
+  std::printf("│ Sys# │  Mixed energy   │   FP64 energy   │ rel. error │\n");
+  std::printf("├──────┼─────────────────┼─────────────────┼────────────┤\n");
+  for (int n = 0; n < F_out; n += std::max((F_out - 1) / 10, 1))
+    std::printf("│  %3.3d │ %+.8e │ %+.8e │ %.4e │\n", n, e_mixed[n], e64[n], rel_diff(e_mixed[n], e64[n]));
+  std::printf("└──────┴─────────────────┴─────────────────┴────────────┘\n");
+  double worst = 0.0;
+  for (int n = 0; n < F_out; ++n) worst = std::max(worst, rel_diff(e_mixed[n], e64[n]));
+  const auto bm = barriers(e_mixed), bd = barriers(e64);
+  std::printf("energy, largest relative error over images   %.4e\n", worst);
+  std::printf("entry barrier  mixed %+.10e  FP64 %+.10e  relative error %.4e\n", bm.first, bd.first,
+              rel_diff(bm.first, bd.first));
+  std::printf("exit barrier   mixed %+.10e  FP64 %+.10e  relative error %.4e\n", bm.second, bd.second,
+              rel_diff(bm.second, bd.second));
+  std::vector<double> both((size_t)F_out * 2);
+  for (int n = 0; n < F_out; ++n) {
+    both[2 * (size_t)n] = e_mixed[n];
+    both[2 * (size_t)n + 1] = e64[n];
+  }
+  npy_save(o.root + "/simulations/" + o.simname + "/output_data/fp64_check.npy", both.data(),
+           {(size_t)F_out, 2});
+}
+
 int run(const Options& o) {
   setvbuf(stdout, nullptr, _IONBF, 0);
   if (o.dry_run) {
@@ -497,6 +594,8 @@ int run(const Options& o) {
   const int F = p.F;
   const int N = p.N;
   std::vector<double> fenergy_old((size_t)F, INFINITY);
+  std::vector<std::pair<int, int>> schedule;  // (relaxation steps, NLCG) per iteration
+  int relax_last = o.relaxation_steps, nlcg_last = 1;
   int rc_loop = MEPGL_OK;
   for (int counter = 0; counter < o.iterations; ++counter) {
     int relax = 0, nlcg = 1, stop = 0;
@@ -508,6 +607,9 @@ int run(const Options& o) {
     if (o.mode == 'C') std::printf("=> Cubic interpolation step \n");
     std::printf("=> Relaxation steps: ");
     rc_loop = mepgl_loop(lib.ctx, 1, relax, o.mode, nlcg);
+    schedule.emplace_back(relax, nlcg);
+    relax_last = relax;
+    nlcg_last = nlcg;
     for (int k = 0; k < relax; ++k) std::printf(" %d ", k + 1);
     std::printf("\n");
     if (rc_loop != MEPGL_OK) break;
@@ -571,6 +673,19 @@ int run(const Options& o) {
   std::printf("Saving results...  ");
   save_output(o, p, F_out, a, psi_1, psi_2, b, j_1, j_2, fenergy, s, s_jb);
   std::printf("done.\n");
+
+  if (o.precision == MEPGL_MIXED && o.fp64_check) {
+    // The mixed context goes first: the FP64 one needs twice its field memory.
+    mepgl_destroy(lib.ctx);
+    lib.ctx = nullptr;
+    try {
+      fp64_check(o, d, p, F_out, fenergy, a, psi_1, psi_2, schedule, relax_last, nlcg_last);
+    } catch (const std::exception& e) {
+      // The run's outputs are saved; the check's failure is reported, not fatal.
+      std::fflush(stdout);
+      std::fprintf(stderr, "\nmepgl: FP64 check not completed: %s\n", e.what());
+    }
+  }
   return 0;
 }
 

@@ -25,6 +25,23 @@ def barriers(fenergy):
     return float(np.max(e - e[0])), float(np.max(e - e[-1]))
 
 
+def precision_error(fenergy_mixed, fenergy_fp64):
+    """Error of a mixed-precision string against the FP64 path: the
+    per-image relative energy error and the relative errors of the entry and
+    exit barriers (absolute where the FP64 value is zero)."""
+    em = np.asarray(fenergy_mixed, dtype=np.float64)
+    ed = np.asarray(fenergy_fp64, dtype=np.float64)
+
+    def rel(x, ref):
+        ref = np.asarray(ref, dtype=np.float64)
+        return np.abs(np.asarray(x) - ref) / np.where(ref != 0.0, np.abs(ref), 1.0)
+
+    bm, bd = barriers(em), barriers(ed)
+    return {"fenergy_mixed": em, "fenergy_fp64": ed, "fenergy_rel_err": rel(em, ed),
+            "barriers_mixed": np.array(bm), "barriers_fp64": np.array(bd),
+            "barrier_rel_err": rel(bm, bd)}
+
+
 def winding(u, v):
     """Phase winding around the grid boundary per image (gl_utils.py:10-41).
 
@@ -77,6 +94,12 @@ def summary(output_dir):
         u2 = np.swapaxes(s.psi_2[..., 0], 1, 2)
         v2 = np.swapaxes(s.psi_2[..., 1], 1, 2)
         out["w_2"] = winding(u2, v2)
+    # A mixed-precision run of the mepgl program writes its energies and the
+    # FP64 path's as the columns of fp64_check.npy.
+    chk = os.path.join(output_dir, "fp64_check.npy")
+    if os.path.exists(chk):
+        e = np.load(chk, allow_pickle=False)
+        out.update(precision_error(e[:, 0], e[:, 1]))
     return out
 
 
@@ -100,6 +123,10 @@ def main(argv=None):
     if e is not None:
         dn, de = barriers(e)
         print(f"Delta F_n = {dn:.6}   Delta F_e = {de:.6}")
+    if "fenergy_rel_err" in out:
+        b = out["barrier_rel_err"]
+        print(f"mixed vs FP64: energy rel. error <= {out['fenergy_rel_err'].max():.3e}   "
+              f"barrier rel. errors {b[0]:.3e} (entry) {b[1]:.3e} (exit)")
     print(path)
     return 0
 

@@ -48,6 +48,9 @@ class Solver:
                 (_lib.GAUGE_ALIGN if gauge_align else 0) | \
                 (_lib.ROLLBACK if rollback else 0) | \
                 (_lib.GAUGE_ALIGN_LOCAL if gauge_align_local else 0)
+        self._options = dict(poly_exact=poly_exact, cubic_textbook=cubic_textbook,
+                             use_graph=use_graph, device=device, gauge_align=gauge_align,
+                             gauge_align_local=gauge_align_local)
         self._keep = []
         descs = (_lib.StringDesc * len(self.strings))()
         for d, s in zip(descs, self.strings):
@@ -277,6 +280,30 @@ class Solver:
         """loop() (mepgl.cu:148-306) without its printing."""
         self._check(self._L.mepgl_loop(self._ctx, int(iterations), int(relaxation_steps),
                                        mode.encode()[:1], int(bool(nlcg))))
+
+    def fp64_check(self, iterations: int = 1, relaxation_steps: int = 25, mode: str = "L",
+                   nlcg: bool = True) -> List[dict]:
+        """Error of the strings' current state against the FP64 path, per
+        string (observables.precision_error): the state is relaxed further in
+        FP64 for `iterations` outer iterations of loop() and the energies and
+        barriers are compared with this solver's.  On a relaxed string the
+        difference is what float arithmetic left in it; on one still relaxing
+        it also holds that further relaxation, an upper bound.  The mepgl
+        program states the same after every mixed-precision run."""
+        from . import observables
+
+        e_mixed = self.computeFreeEnergy()
+        strings = []
+        for i, s in enumerate(self.strings):
+            t = s.copy()
+            t.F = self.F[i]
+            t.a, t.psi_1, p2 = self.download(i)
+            t.psi_2 = p2 if s.multicomponent else None
+            strings.append(t)
+        with Solver(strings, "fp64", **self._options) as ref:
+            ref.loop(iterations, relaxation_steps, mode, nlcg)
+            e64 = ref.computeFreeEnergy()
+        return [observables.precision_error(m, d) for m, d in zip(e_mixed, e64)]
 
     # -- control, timing -------------------------------------------------------
     def set_control(self, key: int, value: int):

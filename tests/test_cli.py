@@ -61,6 +61,10 @@ def test_help(exe):
     (["--simname", "x", "--mode", "L", "--Fin", "3", "--iterations", "1", "--bogus"], "bogus"),
     (["--simname", "x", "--mode", "L", "--Fin", "3", "--iterations", "1", "--precision", "fp16"],
      "precision"),
+    (["--simname", "x", "--mode", "L", "--Fin", "3", "--iterations", "1", "--fp64-check", "all"],
+     "--fp64-check"),
+    (["--simname", "x", "--mode", "L", "--Fin", "3", "--iterations", "1",
+      "--fp64-check-iterations", "0"], "--fp64-check-iterations"),
 ])
 def test_bad_arguments(exe, args, msg):
     r = run(exe, *args)

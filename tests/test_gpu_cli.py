@@ -113,6 +113,56 @@ def test_cli_mixed_and_float32_input(tmp_path, oracle):
     assert rel_err(extra["fenergy"], ref.free_energy()) < 1e-4
 
 
+@pytest.mark.parametrize("check", ["continue", "shadow"])
+def test_cli_mixed_fp64_check(tmp_path, oracle, check):
+    """A mixed-precision run states its error against the FP64 path, printed
+    and in output_data/fp64_check.npy.  Its first column is the run's final
+    energies; its second is the oracle's, after (continue) two further
+    iterations from the saved end state or (shadow) the run's iterations
+    from the input, resampled to --Fout.  Energies relative 1e-11."""
+    from mepgl_amd import npyio, observables
+
+    s = small_string(N=20, F=5, seed=25, multicomponent=True)
+    say(f"[gpu] cli mixed --fp64-check {check}")
+    npyio.write_input(s, "t", str(tmp_path), double_precision=False)
+    r = cli(str(tmp_path), "--simname", "t", "--mode", "L", "--Fin", 5, "--Fout", 7,
+            "--iterations", 2, "--relaxation-steps", 3, "--precision", "mixed",
+            "--fp64-check", check, "--fp64-check-iterations", 2)
+    assert "Mixed precision against the FP64 path" in r.stdout
+    assert "exit barrier" in r.stdout
+    d = os.path.join(tmp_path, "simulations", "t", "output_data")
+    chk = np.load(os.path.join(d, "fp64_check.npy"))
+    assert chk.shape == (7, 2) and chk.dtype == np.float64
+    out, extra = npyio.read_dir(d)
+    np.testing.assert_allclose(chk[:, 0], extra["fenergy"], rtol=1e-7)  # fenergy.npy is float32
+    if check == "continue":
+        ref = oracle.OracleString(out)
+        ref.loop(2, 3, "L")
+    else:
+        s32, _ = npyio.read_dir(os.path.join(tmp_path, "simulations", "t", "input_data"))
+        ref = oracle.OracleString(s32)
+        ref.loop(2, 3, "L")
+        ref.arc_lengths()
+        ref.redistribute(F_new=7)
+    assert rel_err(chk[:, 1], ref.free_energy()) < 1e-11
+    z = np.load(observables.write_summary("t", str(tmp_path)), allow_pickle=True)
+    np.testing.assert_array_equal(z["fenergy_fp64"], chk[:, 1])
+    assert z["barrier_rel_err"].shape == (2,)
+
+
+def test_cli_fp64_check_off(tmp_path):
+    from mepgl_amd import npyio
+
+    s = small_string(N=16, F=4, seed=27)
+    say("[gpu] cli mixed --fp64-check off")
+    npyio.write_input(s, "t", str(tmp_path))
+    r = cli(str(tmp_path), "--simname", "t", "--mode", "S", "--Fin", 4, "--iterations", 1,
+            "--relaxation-steps", 2, "--precision", "mixed", "--fp64-check", "off")
+    assert "FP64 path" not in r.stdout
+    assert not os.path.exists(os.path.join(tmp_path, "simulations", "t", "output_data",
+                                           "fp64_check.npy"))
+
+
 def test_cli_stop_key(tmp_path):
     from mepgl_amd import npyio
 

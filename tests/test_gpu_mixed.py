@@ -152,3 +152,30 @@ def test_mixed_loop_graph_and_eager(mode, oracle):
     ref.loop(3, 4, mode)
     assert rel_err(fen[0], ref.fenergy) < 1e-5
     assert normwise(out[0][1], ref.psi_1) < 1e-4
+
+
+def test_solver_fp64_check(oracle):
+    """Solver.fp64_check relaxes the mixed state further in FP64: its FP64
+    energies are the oracle's after the same iterations from the downloaded
+    state (1e-11), its mixed energies are the solver's own, and the
+    barriers' errors follow from the two."""
+    from mepgl_amd import observables
+    from mepgl_amd.solver import Solver
+
+    s = small_string(N=20, F=5, seed=26, multicomponent=True, masked=True)
+    say("[gpu] Solver.fp64_check")
+    with Solver([s], precision="mixed") as sv:
+        sv.loop(2, 3, "L")
+        r = sv.fp64_check(iterations=2, relaxation_steps=3, mode="L")[0]
+        a, p1, p2 = sv.download(0)
+        e_mixed = sv.computeFreeEnergy()[0]
+    np.testing.assert_array_equal(r["fenergy_mixed"], e_mixed)
+    t = s.copy()
+    t.a, t.psi_1, t.psi_2 = a, p1, p2
+    ref = oracle.OracleString(t)
+    ref.loop(2, 3, "L")
+    e64 = ref.free_energy()
+    assert rel_err(r["fenergy_fp64"], e64) < 1e-11
+    np.testing.assert_allclose(r["barriers_fp64"], observables.barriers(e64), rtol=1e-9,
+                               atol=1e-11 * np.max(np.abs(e64)))
+    assert np.all(r["fenergy_rel_err"] < 1e-3)

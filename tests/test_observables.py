@@ -82,6 +82,25 @@ def test_barriers():
     assert observables.barriers([3.0, 2.0, 1.0]) == (0.0, 2.0)
 
 
+def test_precision_error():
+    """Energies off by known amounts: per-image errors against the FP64
+    values, and barriers whose errors follow from the closed forms
+    (entry 2.0 vs 2.5, exit 2.5 vs 2.5)."""
+    from mepgl_amd import observables
+
+    e64 = np.array([1.0, 3.5, 2.0, 1.0])
+    em = np.array([1.0, 3.5, 2.0, 0.5])
+    r = observables.precision_error(em, e64)
+    np.testing.assert_allclose(r["fenergy_rel_err"], [0.0, 0.0, 0.0, 0.5])
+    np.testing.assert_allclose(r["barriers_mixed"], [2.5, 3.0])
+    np.testing.assert_allclose(r["barriers_fp64"], [2.5, 2.5])
+    np.testing.assert_allclose(r["barrier_rel_err"], [0.0, 0.2])
+    # A zero FP64 value gives the absolute difference.
+    z = observables.precision_error([0.25, 0.0], [0.0, 0.0])
+    np.testing.assert_allclose(z["fenergy_rel_err"], [0.25, 0.0])
+    np.testing.assert_allclose(z["barrier_rel_err"], [0.0, 0.25])
+
+
 @pytest.mark.parametrize("mc", [False, True])
 def test_summary_file(tmp_path, mc):
     """output_data fabricated with the layout writer, then summarised."""
@@ -112,3 +131,10 @@ def test_summary_file(tmp_path, mc):
         assert z["w_2"].shape == (3,)
     else:
         assert z["w_2"].item() is None
+    assert "fenergy_rel_err" not in z
+    # A mixed-precision run adds its energies against the FP64 path's.
+    e64 = e * (1 + np.array([1e-7, -2e-7, 0.0]))
+    np.save(os.path.join(out, "fp64_check.npy"), np.stack([e, e64], axis=1))
+    z = np.load(observables.write_summary("t", str(tmp_path)), allow_pickle=True)
+    np.testing.assert_allclose(z["fenergy_rel_err"], np.abs(e - e64) / np.abs(e64), rtol=1e-12)
+    np.testing.assert_array_equal(z["barriers_fp64"], observables.barriers(e64))
