@@ -9,7 +9,9 @@ resident in HBM.  Before it: `--warmup` steps, the autotuner's trial
 rounds (the state is restored), a restart from the initial guess and at
 least 4 untimed outer iterations and 50 ms from there, after which the
 timed steps follow at once.  `value` is the mixed-precision path's throughput; the FP64
-path is timed on identical inputs and reported beside it.
+path is timed on identical inputs and reported beside it.  The energies and barriers of
+the two are compared after exactly the timed number of steps from the initial guess, in
+a run of their own after the timing.
 
 Further workloads (--legs, default two_band_512, sweep_48 and
 thin_film_disk_1024) run after it for min(K, 50) steps each and are
@@ -91,7 +93,6 @@ def run_path(strings, precision, steps, warmup, profile_iters=3, segment_rows=0,
     sv.loop(iters, R, "L")
     sv.synchronize()
     dt = time.perf_counter() - t0
-    energies = sv.computeFreeEnergy()
     # Per-kernel timing with HIP events on the library's stream (eager
     # launches, same sequence), for the roofline of the dominant kernel.
     sv.profile(True)
@@ -103,6 +104,15 @@ def run_path(strings, precision, steps, warmup, profile_iters=3, segment_rows=0,
         times[name] = (ms, cnt)
         print(f"# {precision} {name}: {1e3 * ms:.2f} us x {cnt}", flush=True)
     sv.profile(False)
+    # The energies the two precisions are compared on: exactly the timed
+    # number of steps from the initial guess.  The warm-up is time-based, so
+    # it runs about twice as many iterations in mixed precision as in FP64,
+    # and the state after the timed steps is not comparable between them.
+    for i, s in enumerate(strings):
+        sv.upload(i, s)
+    sv.reset_solver()
+    sv.loop(iters, R, "L")
+    energies = sv.computeFreeEnergy()
     info = sv.info(0)
     info["rows"] = rows if rows is not None else [sv.info(i)["tile_rows"]
                                                   for i in range(len(strings))]
