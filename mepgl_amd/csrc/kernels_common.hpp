@@ -79,27 +79,73 @@ __device__ __forceinline__ void acc(double& s, double& c, double v) {
     s += v;
 }
 
+// Lane exchanges by DPP.  Every lane of the wavefront must be active.
+// row_ror:R: lane l of each 16-lane row receives lane (l - R) mod 16 of the
+// same row.
+template <int R>
+__device__ __forceinline__ double row_rotate(double v) {
+  static_assert(R > 0 && R < 16, "row_ror takes 1..15");
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x120 + R, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x120 + R, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+// row_bcast:15 (CTRL 0x142, rows 1 and 3 receive lane 15 of rows 0 and 2)
+// and row_bcast:31 (CTRL 0x143, rows 2 and 3 receive lane 31).  The other
+// rows receive no defined value: only lane 63's result is used.
+template <int CTRL, int ROWS>
+__device__ __forceinline__ double row_broadcast(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, ROWS, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, ROWS, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+
+// (s, c) += (so, co).  The rounded sum is commutative and the two-sum error
+// is the exact rounding error whatever the operand order, so the result has
+// the same bits with the operands swapped.
+template <bool COMP>
+__device__ __forceinline__ void combine(double& s, double& c, double so, double co) {
+  if (COMP) {
+    const double t = s + so;
+    const double bp = t - s;
+    const double e = (s - (t - bp)) + (so - bp);
+    s = t;
+    c = (c + co) + e;
+  } else {
+    s = s + so;
+  }
+}
+
+template <int R, int K, bool COMP>
+__device__ __forceinline__ void row_level(double (&s)[K], double (&c)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    combine<COMP>(s[k], c[k], row_rotate<R>(s[k]), COMP ? row_rotate<R>(c[k]) : 0.0);
+}
+
 // Wavefront reduction of K (sum, compensation) pairs; every lane ends with
-// the same result (butterfly, operands combined in lane order).
+// the same bits.  Within each 16-lane row, rotations by 8, 4, 2 and 1 lanes:
+// each adds two values that the levels before made equal R lanes apart, so
+// the sum is symmetric and all 16 lanes of a row hold the row's total.  The
+// four row totals R0..R3 then combine as (R3 + R2) + (R1 + R0) in lane 63,
+// which every lane reads.  The order depends only on the lanes' values.
 template <int K, bool COMP>
 __device__ __forceinline__ void wave_reduce(double (&s)[K], double (&c)[K]) {
+  row_level<8, K, COMP>(s, c);
+  row_level<4, K, COMP>(s, c);
+  row_level<2, K, COMP>(s, c);
+  row_level<1, K, COMP>(s, c);
 #pragma unroll
-  for (int off = kWave / 2; off > 0; off >>= 1) {
-    const bool lo = (lane_id() & off) == 0;
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const double so = __shfl_xor(s[k], off);
-      if (COMP) {
-        const double co = __shfl_xor(c[k], off);
-        double x = lo ? s[k] : so, y = lo ? so : s[k];
-        double cx = lo ? c[k] : co, cy = lo ? co : c[k];
-        two_sum_acc(x, cx, y);
-        s[k] = x;
-        c[k] = cx + cy;
-      } else {
-        s[k] = lo ? s[k] + so : so + s[k];
-      }
-    }
+  for (int k = 0; k < K; ++k) {
+    combine<COMP>(s[k], c[k], row_broadcast<0x142, 0xA>(s[k]),
+                  COMP ? row_broadcast<0x142, 0xA>(c[k]) : 0.0);
+    combine<COMP>(s[k], c[k], row_broadcast<0x143, 0xC>(s[k]),
+                  COMP ? row_broadcast<0x143, 0xC>(c[k]) : 0.0);
+    const double sf = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(s[k]), kWave - 1),
+                                       __builtin_amdgcn_readlane(__double2loint(s[k]), kWave - 1));
+    s[k] = sf;
+    if (COMP)
+      c[k] = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(c[k]), kWave - 1),
+                              __builtin_amdgcn_readlane(__double2loint(c[k]), kWave - 1));
   }
 }
 
