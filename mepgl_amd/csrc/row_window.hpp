@@ -22,6 +22,7 @@ struct GRow {
   T ax, ay, axl, axr, ayl, ayr;  // A and its x-neighbours
   T u[NC], v[NC];                // psi
   T eu[NC], ev[NC];              // psi outside the strip (lanes 0 / 63)
+  T eax, eay;                    // A outside the strip, until g_finish
   uint16_t nn;
 };
 
@@ -35,7 +36,7 @@ struct GBufs {
 
 template <class T, int NC, bool UPD, bool NSF>
 __device__ __forceinline__ void g_load(GRaw<T, NC>& r, const GBufs& b, int row, int P_,
-                                       const Strip& t, unsigned nnoff) {
+                                       const Strip& t, unsigned nnoff, bool halo) {
   const RowOff o = row_off<T>(row, P_, t);
 #pragma unroll
   for (int p = 0; p < GRaw<T, NC>::P; ++p) {
@@ -46,7 +47,7 @@ __device__ __forceinline__ void g_load(GRaw<T, NC>& r, const GBufs& b, int row, 
   // (exec-masked loads); the other lanes' halo values are never used.
 #pragma unroll
   for (int p = 0; p < GRaw<T, NC>::P; ++p) r.hx[p] = r.hd[p] = (T)0;
-  if (t.edge) {
+  if (halo) {
 #pragma unroll
     for (int p = 0; p < GRaw<T, NC>::P; ++p) {
       r.hx[p] = bload<T>(p < 2 ? b.xa : b.x, o.halo, p * b.ps);
@@ -54,6 +55,12 @@ __device__ __forceinline__ void g_load(GRaw<T, NC>& r, const GBufs& b, int row, 
     }
   }
   r.nn = bload_u16(b.nn, nnoff);
+}
+
+template <class T, int NC, bool UPD, bool NSF>
+__device__ __forceinline__ void g_load(GRaw<T, NC>& r, const GBufs& b, int row, int P_,
+                                       const Strip& t, unsigned nnoff) {
+  g_load<T, NC, UPD, NSF>(r, b, row, P_, t, nnoff, t.edge);
 }
 
 template <class T, int NC>
@@ -86,6 +93,49 @@ __device__ __forceinline__ void g_process(GRow<T, NC>& w, const GRaw<T, NC>& r, 
     w.ev[q] = e[PV1 + 2 * q];
   }
   w.nn = r.nn;
+}
+
+// The same in two parts, for kernels whose halo columns may arrive later
+// (from the neighbouring strip's wavefront): g_process_own keeps A's values
+// outside the strip, and g_finish forms A's x-neighbours once they are final.
+// Together they give g_process's values bit for bit.
+template <class T, int NC>
+__device__ __forceinline__ void g_process_own(GRow<T, NC>& w, const GRaw<T, NC>& r, T al,
+                                              bool upd) {
+  T c[GRaw<T, NC>::P], e[GRaw<T, NC>::P];
+#pragma unroll
+  for (int p = 0; p < GRaw<T, NC>::P; ++p) {
+    c[p] = upd ? fma(al, r.d[p], r.x[p]) : r.x[p];
+    e[p] = upd ? fma(al, r.hd[p], r.hx[p]) : r.hx[p];
+  }
+  w.ax = c[PAX];
+  w.ay = c[PAY];
+  w.eax = e[PAX];
+  w.eay = e[PAY];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    w.u[q] = c[PU1 + 2 * q];
+    w.v[q] = c[PV1 + 2 * q];
+    w.eu[q] = e[PU1 + 2 * q];
+    w.ev[q] = e[PV1 + 2 * q];
+  }
+  w.nn = r.nn;
+}
+
+template <class T, int NC>
+__device__ __forceinline__ void pin_row_own(GRow<T, NC>& w) {
+  pin(w.ax), pin(w.ay), pin(w.eax), pin(w.eay);
+#pragma unroll
+  for (int q = 0; q < NC; ++q) pin(w.u[q]), pin(w.v[q]), pin(w.eu[q]), pin(w.ev[q]);
+  pin(w.nn);
+}
+
+template <class T, int NC>
+__device__ __forceinline__ void g_finish(GRow<T, NC>& w) {
+  w.axl = from_left(w.ax, w.eax);
+  w.axr = from_right(w.ax, w.eax);
+  w.ayl = from_left(w.ay, w.eay);
+  w.ayr = from_right(w.ay, w.eay);
 }
 
 }  // namespace mepgl

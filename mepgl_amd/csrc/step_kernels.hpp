@@ -82,6 +82,102 @@ __device__ __forceinline__ void pin_side(GSide<T, NC, PL>& sd) {
   }
 }
 
+// In mixed precision G and P run as workgroups of kStepWaves wavefronts on
+// consecutive work items, which are horizontally adjacent strips of one
+// segment (the work list runs strip by strip, then segment by segment).  A
+// workgroup starts on one CU as a unit, so neighbouring strips march
+// together in every dispatch generation, and each strip's halo columns are
+// read while its neighbour's wavefront still has those cache lines in L2.
+// With one wavefront per workgroup, neighbours dispatched in later
+// generations start at unrelated times and the halo lines are fetched
+// twice: two_band_512 mixed G read 1.35x its algorithmic bytes, 1.06x
+// grouped.  FP64 keeps one wavefront per workgroup: at two wavefronts per
+// SIMD, four-wavefront groups leave slots idle while a group drains
+// (two_band_512 FP64 2.3 % and thin_film_disk_1024 FP64 5.5 % slower).
+constexpr int kStepWaves = 4;
+
+template <class T>
+constexpr int step_waves() {
+  return sizeof(T) == 4 ? kStepWaves : 1;
+}
+
+// Work item of this wavefront (possibly past the end of the list); groups of
+// W items are spread over the XCDs as single items are elsewhere.
+template <int W>
+__device__ __forceinline__ int step_item(int nitems) {
+  const int groups = (nitems + W - 1) / W;
+  return xcd_item(blockIdx.x, groups) * W + (W > 1 ? (int)(threadIdx.x / kWave) : 0);
+}
+
+// Edge columns traded inside a workgroup.  The W items of a workgroup are
+// consecutive in the work list: horizontally adjacent strips of one segment
+// (or of two, when a row has fewer than W strips).  A strip's halo columns
+// are then its neighbours' edge columns, which the neighbours have loaded
+// and processed already: they pass through LDS, with one barrier per row.
+// That needs every wavefront of the workgroup to have an item and to march
+// (the same rows, so the same barriers).  A halo column that is the strip's
+// own column (lane 0 of the first strip, lane 63 of the last, where the
+// halo index is clamped into the grid) takes the lane's own values.  Other
+// halo columns are loaded as before.
+struct EdgePlan {
+  bool lds = false;                            // the workgroup trades edges
+  bool nb_left = false, nb_right = false;      // neighbour strip in the group
+  bool own_left = false, own_right = false;    // halo column is the own one
+  int wave = 0;                                // wavefront in the group
+};
+
+template <int W>
+__device__ __forceinline__ EdgePlan edge_plan(const StepArgs& a, const DevString& s,
+                                              const WorkItem& it) {
+  EdgePlan e;
+  if constexpr (W > 1) {
+    const int groups = (a.nitems + W - 1) / W;
+    const int kb = xcd_item(blockIdx.x, groups) * W;
+    if (kb + W > a.nitems) return e;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    // Decided from the group's items alone, so every wavefront of the group
+    // takes the same decision: all items march (they have bonds) over the
+    // same rows in the same direction (one segment of one image).
+    const WorkItem o0 = load_item(a.items, kb);
+    const int seg0 = o0.tile / s.tiles_x;
+    bool all = !(o0.flags & kItemNoBonds), left = false, right = false;
+#pragma unroll
+    for (int q = 0; q < W; ++q) {
+      const WorkItem o = q == 0 ? o0 : load_item(a.items, kb + q);
+      all = all && !(o.flags & kItemNoBonds) && o.string == o0.string && o.image == o0.image &&
+            o.tile / s.tiles_x == seg0;
+      left = left || (q == wave - 1 && o.tile == it.tile - 1);
+      right = right || (q == wave + 1 && o.tile == it.tile + 1);
+    }
+    if (!all) return e;
+    const int sx = it.tile % s.tiles_x;
+    e.lds = true;
+    e.wave = wave;
+    e.own_left = sx == 0;
+    e.own_right = (sx + 1) * kTileX >= s.N;
+    e.nb_left = !e.own_left && left;
+    e.nb_right = !e.own_right && right;
+  }
+  return e;
+}
+
+// Whether this lane loads its halo column from memory.
+__device__ __forceinline__ bool halo_from_memory(const Strip& t, const EdgePlan& e) {
+  if (!t.edge) return false;
+  if (!e.lds) return true;
+  return t.lane == 0 ? !(e.nb_left || e.own_left) : !(e.nb_right || e.own_right);
+}
+
+// The barrier of the edge exchange: LDS writes before it are visible to the
+// workgroup after it.  Only LDS operations are waited for, so loads issued
+// ahead stay in flight.
+__device__ __forceinline__ void edge_barrier() {
+  memory_fence_compiler();
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt and expcnt left as they are
+  __builtin_amdgcn_s_barrier();
+  memory_fence_compiler();
+}
+
 template <class T, int NC, bool UPD, bool NSF, bool PL, bool MW>
 __device__ __forceinline__ void grad_body(const StepArgs& a, const DevString& s, const WorkItem& it) {
   constexpr bool COMP = sizeof(T) == 4;
@@ -113,6 +209,63 @@ __device__ __forceinline__ void grad_body(const StepArgs& a, const DevString& s,
   const T hv = (T)s.h.v;
   double acc_s[2] = {0, 0}, acc_c[2] = {0, 0};
   auto nnoff = [&](int row) { return (unsigned)((row * P_ + t.ic) * (int)sizeof(uint16_t)); };
+  // Edge exchange (EdgePlan): in mixed precision a row's A values outside
+  // the strip are final only once the neighbours' edges are in, so A's
+  // x-neighbours are formed one row later (g_finish), when the row becomes
+  // the next row of the march.
+  constexpr int W = step_waves<T>();
+  const EdgePlan ep = edge_plan<W>(a, s, it);
+  const bool hmem = halo_from_memory(t, ep);
+  constexpr int EV = 2 + 2 * NC;  // A and psi
+  __shared__ T edges[W > 1 ? 3 : 1][W][2][EV];
+  auto put_edges = [&](const GRow<T, NC>& w, auto slot_tag) {
+    constexpr int SL = decltype(slot_tag)::value;
+    if constexpr (W > 1) {
+      if (ep.lds && t.edge) {
+        T* dst = edges[SL][ep.wave][t.lane == 0 ? 0 : 1];
+        dst[0] = w.ax;
+        dst[1] = w.ay;
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+          dst[2 + 2 * q] = w.u[q];
+          dst[3 + 2 * q] = w.v[q];
+        }
+      }
+    }
+  };
+  auto finish = [&](GRow<T, NC>& w, auto slot_tag) {
+    constexpr int SL = decltype(slot_tag)::value;
+    if constexpr (W > 1) {
+      if (ep.lds && t.edge) {
+        const bool left = t.lane == 0;
+        if (left ? ep.nb_left : ep.nb_right) {
+          const T* src = edges[SL][left ? ep.wave - 1 : ep.wave + 1][left ? 1 : 0];
+          w.eax = src[0];
+          w.eay = src[1];
+#pragma unroll
+          for (int q = 0; q < NC; ++q) {
+            w.eu[q] = src[2 + 2 * q];
+            w.ev[q] = src[3 + 2 * q];
+          }
+        } else if (left ? ep.own_left : ep.own_right) {
+          w.eax = w.ax;
+          w.eay = w.ay;
+#pragma unroll
+          for (int q = 0; q < NC; ++q) {
+            w.eu[q] = w.u[q];
+            w.ev[q] = w.v[q];
+          }
+        }
+      }
+      g_finish<T, NC>(w);
+    }
+  };
+  auto process = [&](GRow<T, NC>& w, const GRaw<T, NC>& r) {
+    if constexpr (W > 1)
+      g_process_own<T, NC>(w, r, al, UPD);
+    else
+      g_process<T, NC>(w, r, al, UPD);
+  };
 
   // The march is the only part that depends on the direction
   // (marches_up); everything it needs from the string's descriptor is
@@ -179,13 +332,20 @@ __device__ __forceinline__ void grad_body(const StepArgs& a, const DevString& s,
     GRaw<T, NC> raw;
     GRow<T, NC> w[3];
     GSide<T, NC, PL> sd[3];
-    g_load<T, NC, UPD, NSF>(raw, b, clampr(jb - D, N), P_, t, nnoff(clampr(jb - D, N)));
-    g_process<T, NC>(w[0], raw, al, UPD);
-    g_load<T, NC, UPD, NSF>(raw, b, clampr(jb, N), P_, t, nnoff(clampr(jb, N)));
-    g_process<T, NC>(w[1], raw, al, UPD);
-    g_load<T, NC, UPD, NSF>(raw, b, clampr(jb + D, N), P_, t, nnoff(clampr(jb + D, N)));
-    g_process<T, NC>(w[2], raw, al, UPD);
+    g_load<T, NC, UPD, NSF>(raw, b, clampr(jb - D, N), P_, t, nnoff(clampr(jb - D, N)), hmem);
+    process(w[0], raw);
+    g_load<T, NC, UPD, NSF>(raw, b, clampr(jb, N), P_, t, nnoff(clampr(jb, N)), hmem);
+    process(w[1], raw);
+    g_load<T, NC, UPD, NSF>(raw, b, clampr(jb + D, N), P_, t, nnoff(clampr(jb + D, N)), hmem);
+    process(w[2], raw);
     g_side_load<T, NC, NSF, PL, MW>(sd[1], b, jb, P_, t, pv, hp, N);
+    // The first rows' edges; row jb + D is finished at the first row's start.
+    put_edges(w[0], S0());
+    put_edges(w[1], S1());
+    put_edges(w[2], S2());
+    if (ep.lds) edge_barrier();
+    finish(w[0], S0());
+    finish(w[1], S1());
 
     // Row j: issue one batch of loads (the fields of row j + 2D, the own-site
     // values of row j + D), compute row j from registers, turn the batch into
@@ -204,9 +364,12 @@ __device__ __forceinline__ void grad_body(const StepArgs& a, const DevString& s,
       const bool live = store && (UP ? j >= t.j0 : j < t.j1);
       if constexpr (BATCH) {
         g_load<T, NC, UPD, NSF>(raw, b, clampr(j + 2 * D, N), P_, t,
-                                nnoff(clampr(j + 2 * D, N)));
+                                nnoff(clampr(j + 2 * D, N)), hmem);
         g_side_load<T, NC, NSF, PL, MW>(sd[IN], b, clampr(j + D, N), P_, t, pv, hp, N);
       }
+      // Row j + D, processed one row ago, takes its neighbours' edges.
+      if (ep.lds) edge_barrier();
+      finish(w[IN], in_tag);
       __builtin_amdgcn_sched_barrier(0);
 
       Nbhd<T, NC> nb;
@@ -279,8 +442,12 @@ __device__ __forceinline__ void grad_body(const StepArgs& a, const DevString& s,
       }
       __builtin_amdgcn_sched_barrier(0);
       if constexpr (BATCH) {
-        g_process<T, NC>(w[IP], raw, al, UPD);
-        pin_row<T, NC>(w[IP]);
+        process(w[IP], raw);
+        put_edges(w[IP], ip_tag);
+        if constexpr (W > 1)
+          pin_row_own<T, NC>(w[IP]);
+        else
+          pin_row<T, NC>(w[IP]);
         pin_side<T, NC, PL>(sd[IN]);
         memory_fence_compiler();
       }
@@ -335,33 +502,6 @@ __device__ __forceinline__ void grad_body(const StepArgs& a, const DevString& s,
   publish_partial<2, COMP>(acc_s, acc_c, s.part_g, t.n, t.tile, s.tiles_x * s.tiles_y);
 }
 
-// In mixed precision G and P run as workgroups of kStepWaves wavefronts on
-// consecutive work items, which are horizontally adjacent strips of one
-// segment (the work list runs strip by strip, then segment by segment).  A
-// workgroup starts on one CU as a unit, so neighbouring strips march
-// together in every dispatch generation, and each strip's halo columns are
-// read while its neighbour's wavefront still has those cache lines in L2.
-// With one wavefront per workgroup, neighbours dispatched in later
-// generations start at unrelated times and the halo lines are fetched
-// twice: two_band_512 mixed G read 1.35x its algorithmic bytes, 1.06x
-// grouped.  FP64 keeps one wavefront per workgroup: at two wavefronts per
-// SIMD, four-wavefront groups leave slots idle while a group drains
-// (two_band_512 FP64 2.3 % and thin_film_disk_1024 FP64 5.5 % slower).
-constexpr int kStepWaves = 4;
-
-template <class T>
-constexpr int step_waves() {
-  return sizeof(T) == 4 ? kStepWaves : 1;
-}
-
-// Work item of this wavefront (possibly past the end of the list); groups of
-// W items are spread over the XCDs as single items are elsewhere.
-template <int W>
-__device__ __forceinline__ int step_item(int nitems) {
-  const int groups = (nitems + W - 1) / W;
-  return xcd_item(blockIdx.x, groups) * W + (W > 1 ? (int)(threadIdx.x / kWave) : 0);
-}
-
 template <class T, int NC, bool UPD, bool NSF, bool PL, bool MW>
 __global__ __launch_bounds__(kWave * step_waves<T>()) void k_grad(StepArgs a) {
   const int k = step_item<step_waves<T>()>(a.nitems);
@@ -395,7 +535,7 @@ struct PBufs {
 
 template <class T, int NC, bool NSF>
 __device__ __forceinline__ void p_load(PRaw<T, NC>& r, const PBufs& b, int row, int P_,
-                                       const Strip& t, unsigned nnoff) {
+                                       const Strip& t, unsigned nnoff, bool halo) {
   const RowOff o = row_off<T>(row, P_, t);
 #pragma unroll
   for (int p = 0; p < PRaw<T, NC>::P; ++p) {
@@ -406,7 +546,7 @@ __device__ __forceinline__ void p_load(PRaw<T, NC>& r, const PBufs& b, int row, 
   }
 #pragma unroll
   for (int p = 0; p < PRaw<T, NC>::P; ++p) r.hx[p] = r.hg[p] = r.hd[p] = (T)0;
-  if (t.edge) {
+  if (halo) {
 #pragma unroll
     for (int p = 0; p < PRaw<T, NC>::P; ++p) {
       const bool dir = !(NSF && p < 2);
@@ -460,6 +600,58 @@ __device__ __forceinline__ void p_process(PRow<T, NC>& w, const PRaw<T, NC>& r, 
   w.nn = r.nn;
 }
 
+// A P row's values traded at the strip edges: A, its new direction, psi and
+// its new direction.
+template <int NC>
+constexpr int p_edge_values() {
+  return 4 + 4 * NC;
+}
+
+template <class T, int NC>
+__device__ __forceinline__ void p_put_edges(const PRow<T, NC>& w, T* dst) {
+  dst[0] = w.ax;
+  dst[1] = w.ay;
+  dst[2] = w.dax;
+  dst[3] = w.day;
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    dst[4 + 4 * q] = w.u[q];
+    dst[5 + 4 * q] = w.v[q];
+    dst[6 + 4 * q] = w.du[q];
+    dst[7 + 4 * q] = w.dv[q];
+  }
+}
+
+template <class T, int NC>
+__device__ __forceinline__ void p_get_edges(PRow<T, NC>& w, const T* src) {
+  w.eax = src[0];
+  w.eay = src[1];
+  w.edax = src[2];
+  w.eday = src[3];
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    w.eu[q] = src[4 + 4 * q];
+    w.ev[q] = src[5 + 4 * q];
+    w.edu[q] = src[6 + 4 * q];
+    w.edv[q] = src[7 + 4 * q];
+  }
+}
+
+template <class T, int NC>
+__device__ __forceinline__ void p_own_edges(PRow<T, NC>& w) {
+  w.eax = w.ax;
+  w.eay = w.ay;
+  w.edax = w.dax;
+  w.eday = w.day;
+#pragma unroll
+  for (int q = 0; q < NC; ++q) {
+    w.eu[q] = w.u[q];
+    w.ev[q] = w.v[q];
+    w.edu[q] = w.du[q];
+    w.edv[q] = w.dv[q];
+  }
+}
+
 template <class T, int NC, bool NSF, bool PL>
 __device__ __forceinline__ void poly_body(const StepArgs& a, const DevString& s, const WorkItem& it) {
   constexpr bool COMP = sizeof(T) == 4;
@@ -489,6 +681,31 @@ __device__ __forceinline__ void poly_body(const StepArgs& a, const DevString& s,
     beta_d = a.nlcg ? fmax(gdelta / scp->g_norm2[a.sold], 0.0) : 0.0;
   };
   auto nnoff = [&](int row) { return (unsigned)((row * P_ + t.ic) * (int)sizeof(uint16_t)); };
+  constexpr int W = step_waves<T>();
+  const EdgePlan ep = edge_plan<W>(a, s, it);
+  const bool hmem = halo_from_memory(t, ep);
+  // Edge values per row slot of the window, wavefront and side (lane 0,
+  // lane 63).
+  constexpr int EV = p_edge_values<NC>();
+  __shared__ T edges[W > 1 ? 3 : 1][W][2][EV];
+  auto put_edges = [&](const PRow<T, NC>& w, auto slot_tag) {
+    constexpr int SL = decltype(slot_tag)::value;
+    if constexpr (W > 1) {
+      if (ep.lds && t.edge) p_put_edges<T, NC>(w, edges[SL][ep.wave][t.lane == 0 ? 0 : 1]);
+    }
+  };
+  auto get_edges = [&](PRow<T, NC>& w, auto slot_tag) {
+    constexpr int SL = decltype(slot_tag)::value;
+    if constexpr (W > 1) {
+      if (ep.lds && t.edge) {
+        const bool left = t.lane == 0;
+        if (left ? ep.nb_left : ep.nb_right)
+          p_get_edges<T, NC>(w, edges[SL][left ? ep.wave - 1 : ep.wave + 1][left ? 1 : 0]);
+        else if (left ? ep.own_left : ep.own_right)
+          p_own_edges<T, NC>(w);
+      }
+    }
+  };
   // The march is the only part that depends on the direction
   // (marches_up).  Its rows are loaded after beta is totalled: holding a row
   // across the partials' reduction costs the mixed kernels a wave per SIMD
@@ -525,13 +742,20 @@ __device__ __forceinline__ void poly_body(const StepArgs& a, const DevString& s,
     PRaw<T, NC> raw;
     PRow<T, NC> w[3];
     RowParams<T, NC, PL> rp[3];
-    p_load<T, NC, NSF>(raw, b, clampr(jb - D, N), P_, t, nnoff(clampr(jb - D, N)));
+    p_load<T, NC, NSF>(raw, b, clampr(jb - D, N), P_, t, nnoff(clampr(jb - D, N)), hmem);
     p_process<T, NC>(w[0], raw, beta);
-    p_load<T, NC, NSF>(raw, b, clampr(jb, N), P_, t, nnoff(clampr(jb, N)));
+    p_load<T, NC, NSF>(raw, b, clampr(jb, N), P_, t, nnoff(clampr(jb, N)), hmem);
     p_process<T, NC>(w[1], raw, beta);
-    p_load<T, NC, NSF>(raw, b, clampr(jb + D, N), P_, t, nnoff(clampr(jb + D, N)));
+    p_load<T, NC, NSF>(raw, b, clampr(jb + D, N), P_, t, nnoff(clampr(jb + D, N)), hmem);
     p_process<T, NC>(w[2], raw, beta);
     rp[1].load(pv, hp, (long long)jb * P_ + t.ic, jb, N, P_, t);
+    // The first rows' edges; row jb + D's are read at the first row's start.
+    put_edges(w[0], S0());
+    put_edges(w[1], S1());
+    put_edges(w[2], S2());
+    if (ep.lds) edge_barrier();
+    get_edges(w[0], S0());
+    get_edges(w[1], S1());
 
     // Row j: one batch of loads (the fields of row j + 2D, the parameters of
     // row j + D), the quartic of row j from registers, the batch into the
@@ -545,10 +769,15 @@ __device__ __forceinline__ void poly_body(const StepArgs& a, const DevString& s,
       PRow<T, NC>& wpl = w[UP ? IP : IN];  // row j + 1
       const bool live = store && (UP ? j >= t.j0 : j < t.j1);
       if constexpr (BATCH) {
-        p_load<T, NC, NSF>(raw, b, clampr(j + 2 * D, N), P_, t, nnoff(clampr(j + 2 * D, N)));
+        p_load<T, NC, NSF>(raw, b, clampr(j + 2 * D, N), P_, t, nnoff(clampr(j + 2 * D, N)),
+                           hmem);
         const int r1 = clampr(j + D, N);
         rp[IN].load(pv, hp, (long long)r1 * P_ + t.ic, r1, N, P_, t);
       }
+      // Row j + D's edges, stored by the neighbours when they processed it
+      // one row ago: row j + D is the current row at the next row.
+      if (ep.lds) edge_barrier();
+      get_edges(w[IN], in_tag);
       __builtin_amdgcn_sched_barrier(0);
 
       PolyNbhd<T, NC> nb;
@@ -606,6 +835,7 @@ __device__ __forceinline__ void poly_body(const StepArgs& a, const DevString& s,
       __builtin_amdgcn_sched_barrier(0);
       if constexpr (BATCH) {
         p_process<T, NC>(w[IP], raw, beta);
+        put_edges(w[IP], ip_tag);
         pin_row<T, NC>(w[IP]);
         if constexpr (PL) {
 #pragma unroll
