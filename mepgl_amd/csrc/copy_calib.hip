@@ -14,6 +14,15 @@
 // Bytes per lane: 4 (one dword: the mixed-precision stencil kernels' load
 // width with one column per lane), 8 (dwordx2: FP64 with one column per
 // lane, or mixed with two), 16 (dwordx4, the default).
+//   copy_calib [images] [repeats] step [4|8] [segment rows]
+// step: the access pattern of one NLCG step of the relaxation kernels on
+// `images` one-component images of 256 x 256 sites (4 planes interleaved by
+// row), without their arithmetic: a G-like launch reads x, d and g_old and
+// writes x' and g, a P-like launch reads g, d and x' and writes d', in the
+// kernels' alternating order over the same five field sets.  Each
+// 4-wavefront workgroup marches `segment rows` rows of four adjacent 64-column
+// strips (one wavefront per strip, one element per lane and plane), as the
+// mixed-precision kernels do; edge columns are not loaded, as there.
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -110,6 +119,82 @@ __global__ __launch_bounds__(256) void stream_kernel(const typename Elem<W>::T* 
   }
 }
 
+// One row segment of one workgroup: per row, every lane loads its column of
+// the 4 planes of R field sets and stores the 4 planes of NW field sets.
+template <class T, int R, int NW>
+struct StepBufs {
+  const T* in[R];
+  T* out[NW];
+};
+
+template <class T, int R, int NW>
+__global__ __launch_bounds__(256) void step_kernel(StepBufs<T, R, NW> p, int h) {
+  constexpr long long kRow = 4 * 256;  // elements per row of one field set
+  for (int r = 0; r < h; ++r) {
+    const long long base = ((long long)blockIdx.x * h + r) * kRow + threadIdx.x;
+    T v[R][4];
+#pragma unroll
+    for (int k = 0; k < R; ++k)
+#pragma unroll
+      for (int q = 0; q < 4; ++q) v[k][q] = p.in[k][base + q * 256];
+#pragma unroll
+    for (int k = 0; k < NW; ++k)
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        __builtin_nontemporal_store(v[0][q] + (T)(k + 1) * v[1][q] - v[R - 1][q],
+                                    p.out[k] + base + q * 256);
+  }
+}
+
+template <class T>
+static int run_step(long long images, int reps, int h) {
+  const long long rows = images * 256;
+  if (rows % h) {
+    fprintf(stderr, "copy_calib: segment rows must divide %lld\n", rows);
+    return 2;
+  }
+  const long long n = rows * 4 * 256;
+  T* f[5] = {};
+  for (auto& q : f) {
+    if (hipMalloc(&q, sizeof(T) * n) != hipSuccess) return 1;
+    if (hipMemset(q, 0, sizeof(T) * n) != hipSuccess) return 1;
+  }
+  T *x[2] = {f[0], f[1]}, *g[2] = {f[2], f[3]}, *d = f[4];
+  const unsigned grid = (unsigned)(rows / h);
+  hipEvent_t ev[3];
+  for (auto& e : ev)
+    if (hipEventCreate(&e) != hipSuccess) return 1;
+  double tg = 0, tp = 0;
+  const int warm = 20;
+  for (int it = 0; it < reps + warm; ++it) {
+    const int i = it & 1;
+    StepBufs<T, 3, 2> pg{{x[i], d, g[i]}, {x[1 - i], g[1 - i]}};
+    StepBufs<T, 3, 1> pp{{g[1 - i], d, x[1 - i]}, {d}};
+    if (hipEventRecord(ev[0], 0) != hipSuccess) return 1;
+    hipLaunchKernelGGL((step_kernel<T, 3, 2>), dim3(grid), dim3(256), 0, 0, pg, h);
+    if (hipEventRecord(ev[1], 0) != hipSuccess) return 1;
+    hipLaunchKernelGGL((step_kernel<T, 3, 1>), dim3(grid), dim3(256), 0, 0, pp, h);
+    if (hipEventRecord(ev[2], 0) != hipSuccess) return 1;
+    if (hipEventSynchronize(ev[2]) != hipSuccess) return 1;
+    float a = 0, c = 0;
+    if (hipEventElapsedTime(&a, ev[0], ev[1]) != hipSuccess) return 1;
+    if (hipEventElapsedTime(&c, ev[1], ev[2]) != hipSuccess) return 1;
+    if (it >= warm) {
+      tg += a;
+      tp += c;
+    }
+  }
+  tg /= reps;
+  tp /= reps;
+  const double bytes = (double)sizeof(T) * n;
+  printf("step %lld images, %zu bytes per element, %d-row segments: G-like %.2f us (%.1f GB/s), "
+         "P-like %.2f us (%.1f GB/s)\n",
+         images, sizeof(T), h, tg * 1e3, 5.0 * bytes / (tg * 1e-3) / 1e9, tp * 1e3,
+         4.0 * bytes / (tp * 1e-3) / 1e9);
+  for (auto* q : f) (void)hipFree(q);
+  return 0;
+}
+
 template <int W>
 static int run(long long mib, int reps, const char* mode, int waves) {
   using T = typename Elem<W>::T;
@@ -175,6 +260,18 @@ int main(int argc, char** argv) {
   const char* mode = argc > 3 ? argv[3] : "copy";
   const int width = argc > 4 ? atoi(argv[4]) : 16;
   const int waves = argc > 5 ? atoi(argv[5]) : 4;
+  if (!strcmp(mode, "step")) {
+    // mib: images; waves: segment rows
+    const int h = argc > 5 ? atoi(argv[5]) : 16;
+    if (mib <= 0 || mib > 256 || reps <= 0 || h < 1 || h > 256) {
+      fprintf(stderr, "copy_calib step: 1..256 images, repeats > 0, 1..256 segment rows\n");
+      return 2;
+    }
+    if (width == 4) return run_step<float>(mib, reps, h);
+    if (width == 8) return run_step<double>(mib, reps, h);
+    fprintf(stderr, "copy_calib step: bytes per element must be 4 or 8\n");
+    return 2;
+  }
   if (mib <= 0 || mib > 16384 || reps <= 0 || waves < 1 || waves > 8) {
     fprintf(stderr, "copy_calib: MiB in 1..16384, repeats > 0, 1..8 wavefronts per SIMD\n");
     return 2;
