@@ -21,6 +21,7 @@ SCALARS = {
     "winding_1": 12, "winding_2": 13,
 }
 CONTROL_STOP, CONTROL_RELAXATION_STEPS, CONTROL_NLCG, CONTROL_FOLD_LINE_SEARCH = 0, 1, 2, 3
+CONTROL_SPLIT_IMAGES = 4
 KERNEL_GRAD, KERNEL_POLY = 0, 1
 
 EXPORTS = [

@@ -55,6 +55,13 @@ struct mepgl_ctx {
   unsigned flags = 0;
   int device = 0;
   hipStream_t stream = nullptr;
+  // Second stream of the image halves (split_images): between
+  // reparametrisations every image's NLCG state is independent of the
+  // others', so the NLCG steps of the two halves of each class's images run
+  // as two launch sequences side by side, one P beside the other half's G.
+  hipStream_t stream2 = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  bool split_images = true;  // MEPGL_CONTROL_SPLIT_IMAGES
   std::vector<StringHost> strings;
   int total_images = 0;
   DevString* d_strings = nullptr;
@@ -66,8 +73,10 @@ struct mepgl_ctx {
   // Items sorted by class (components, fixed A, parameter planes); G and P
   // launch once per non-empty class so each launch runs one specialised
   // kernel.
+  // `half`: items of the first half of the class's images (an image
+  // boundary), the rest being the second half.
   struct Group {
-    int begin, count, nc, nsf, pl;
+    int begin, count, nc, nsf, pl, half;
   };
   std::vector<Group> groups;
   double** d_jb = nullptr;
@@ -216,25 +225,30 @@ hipError_t record_timing_event(mepgl_ctx* c) {
   return hipEventRecord(t.pool[t.used++], c->stream);
 }
 
-int run_step_group(mepgl_ctx* c, const StepArgs& a, int which);
+int run_step_group(mepgl_ctx* c, const StepArgs& a, int which, hipStream_t st);
 
-// Launches `which` once per item class.
-int run_step(mepgl_ctx* c, const StepArgs& a0, int which) {
+// Launches `which` once per item class: on all of its images (part -1) on
+// the context's stream, or on the first (part 0) or second (part 1) half of
+// its images on the first or second stream.
+int run_step(mepgl_ctx* c, const StepArgs& a0, int which, int part = -1) {
   if (c->dry) return MEPGL_OK;
   for (const auto& g : c->groups) {
     StepArgs a = a0;
-    a.items = c->d_items + g.begin;
-    a.nitems = g.count;
+    const int b = part == 1 ? g.half : 0;
+    const int e = part == 0 ? g.half : g.count;
+    if (e <= b) continue;
+    a.items = c->d_items + g.begin + b;
+    a.nitems = e - b;
     a.nc = g.nc;
     a.nsf = g.nsf;
     a.pl = g.pl;
-    int rc = run_step_group(c, a, which);
+    int rc = run_step_group(c, a, which, part == 1 ? c->stream2 : c->stream);
     if (rc) return rc;
   }
   return MEPGL_OK;
 }
 
-int run_step_group(mepgl_ctx* c, const StepArgs& a, int which) {
+int run_step_group(mepgl_ctx* c, const StepArgs& a, int which, hipStream_t st) {
   if (c->timing.on && which != kKernelUpdate) {
     const size_t s0 = c->timing.used;
     hipError_t e = record_timing_event(c);
@@ -246,8 +260,24 @@ int run_step_group(mepgl_ctx* c, const StepArgs& a, int which) {
     c->timing.marks.push_back({tid, {s0, s0 + 1}});
     return MEPGL_OK;
   }
-  hipError_t e = launch_step(a, which, c->fp32(), c->stream);
+  hipError_t e = launch_step(a, which, c->fp32(), st);
   if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, std::string("launch: ") + hipGetErrorString(e));
+  return MEPGL_OK;
+}
+
+// The second stream starts after the work queued so far on the first
+// (fork) and the first continues after the second's work (join).
+int fork_halves(mepgl_ctx* c) {
+  if (c->dry) return MEPGL_OK;
+  HIP_TRY(c, hipEventRecord(c->fork_ev, c->stream));
+  HIP_TRY(c, hipStreamWaitEvent(c->stream2, c->fork_ev, 0));
+  return MEPGL_OK;
+}
+
+int join_halves(mepgl_ctx* c) {
+  if (c->dry) return MEPGL_OK;
+  HIP_TRY(c, hipEventRecord(c->join_ev, c->stream2));
+  HIP_TRY(c, hipStreamWaitEvent(c->stream, c->join_ev, 0));
   return MEPGL_OK;
 }
 
@@ -348,7 +378,10 @@ bool fold_line_search(const mepgl_ctx* c) {
 // `more`: another step follows at once, so P's line search may be left to
 // that step's G (every G wavefront scans the table from P's partials), which
 // saves the finishing launch and one kernel boundary per step.
-int nlcg_step(mepgl_ctx* c, int k, int nlcg, int maxwell, bool more = false) {
+// `split`: the two halves of the images run on the two streams (the
+// caller forked them); a step that ends with the finishing launch joins them
+// first, since that launch covers every image.
+int nlcg_step(mepgl_ctx* c, int k, int nlcg, int maxwell, bool more = false, bool split = false) {
   const int gcur = (k % 2 == 0) ? 1 : 0;
   StepArgs a = step_args(c);
   a.gcur = gcur;
@@ -365,7 +398,8 @@ int nlcg_step(mepgl_ctx* c, int k, int nlcg, int maxwell, bool more = false) {
   if (!a.ls_fold) {
     if (int rc = settle_ls(c)) return rc;
   }
-  int rc = run_step(c, a, kKernelGrad);
+  int rc = split ? run_step(c, a, kKernelGrad, 0) : run_step(c, a, kKernelGrad);
+  if (!rc && split) rc = run_step(c, a, kKernelGrad, 1);
   if (rc) return rc;
   c->ls_deferred = false;
   if (c->pending) {
@@ -379,12 +413,15 @@ int nlcg_step(mepgl_ctx* c, int k, int nlcg, int maxwell, bool more = false) {
   a.dout = 1 - c->dp;
   // P totals the gradient partials into beta itself (there is no finishing
   // launch after G); the quartic totals and the line search follow P.
-  rc = run_step(c, a, kKernelPoly);
+  rc = split ? run_step(c, a, kKernelPoly, 0) : run_step(c, a, kKernelPoly);
+  if (!rc && split) rc = run_step(c, a, kKernelPoly, 1);
   if (rc) return rc;
-  if (more && fold_line_search(c))
+  if (more && fold_line_search(c)) {
     c->ls_deferred = true;
-  else if ((rc = run_finish(c, finish_args(c), kKernelPoly)))
-    return rc;
+  } else {
+    if (split && (rc = join_halves(c))) return rc;
+    if ((rc = run_finish(c, finish_args(c), kKernelPoly))) return rc;
+  }
   c->dp = 1 - c->dp;
   c->pending = true;
   return MEPGL_OK;
@@ -449,17 +486,24 @@ int build_items(mepgl_ctx* c) {
     for (int nsf = 0; nsf <= 1; ++nsf)
       for (int pl = 0; pl <= 1; ++pl) {
         const int begin = (int)items.size();
+        std::vector<int> image_ends;  // item index after each image of the class
         for (size_t si = 0; si < c->strings.size(); ++si) {
           const DevString& s = c->strings[si].dev;
           const int spl = (s.planes || s.h.p) ? 1 : 0;
           if (s.nc != nc || s.nsf != nsf || spl != pl) continue;
           const std::vector<int> flags = tile_flags(c->strings[si]);
-          for (int n = 0; n < s.F; ++n)
+          for (int n = 0; n < s.F; ++n) {
             for (int t = 0; t < s.tiles_x * s.tiles_y; ++t)
               items.push_back({(int)si, n, t, flags[t]});
+            image_ends.push_back((int)items.size() - begin);
+          }
         }
         const int count = (int)items.size() - begin;
-        if (count > 0) c->groups.push_back({begin, count, nc, nsf, pl});
+        // The image boundary nearest to half the class's items.
+        int half = count;
+        for (int e : image_ends)
+          if (e < count && std::abs(2 * e - count) < std::abs(2 * half - count)) half = e;
+        if (count > 0) c->groups.push_back({begin, count, nc, nsf, pl, half});
       }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   if (c->d_items) (void)hipFree(c->d_items);
@@ -830,8 +874,16 @@ int outer_iteration(mepgl_ctx* c, int relax, char mode, int nlcg) {
     if ((rc = redistribute_all(c, mode == 'C'))) return rc;
   }
   const bool maxwell = mode == 'M';
+  // The steps of the two image halves run side by side when the line search
+  // stays inside G until the last step (so nothing but that step's
+  // finishing launch covers every image); per-kernel timing keeps one stream.
+  const bool split = c->split_images && relax > 1 && fold_line_search(c) && !c->timing.on;
+  if (split && (rc = fork_halves(c))) return rc;
   for (int k = 0; k < relax; ++k)
-    if ((rc = nlcg_step(c, k, nlcg, maxwell ? 1 : 0, k + 1 < relax))) return rc;
+    if ((rc = nlcg_step(c, k, nlcg, maxwell ? 1 : 0, k + 1 < relax, split))) {
+      if (split) (void)join_halves(c);
+      return rc;
+    }
   return MEPGL_OK;
 }
 
@@ -1005,6 +1057,12 @@ int mepgl_create(const mepgl_string_desc* strings, int n_strings, int precision,
   if (e != hipSuccess) return MEPGL_ERR_HIP;
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) return MEPGL_ERR_HIP;
+  if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess) {
+    mepgl_destroy(c.release());
+    return MEPGL_ERR_HIP;
+  }
   if (hipHostMalloc((void**)&c->health, 64, hipHostMallocDefault) != hipSuccess) {
     c->health = nullptr;
     mepgl_destroy(c.release());
@@ -1176,6 +1234,9 @@ void mepgl_destroy(mepgl_ctx* c) {
   for (auto e : c->iter_done)
     if (e) (void)hipEventDestroy(e);
   if (c->health) (void)hipHostFree(c->health);
+  if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
+  if (c->join_ev) (void)hipEventDestroy(c->join_ev);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1685,6 +1746,12 @@ int mepgl_set_control(mepgl_ctx* c, int key, int value) {
         destroy_graphs(c);
       }
       break;
+    case MEPGL_CONTROL_SPLIT_IMAGES:
+      if (c->split_images != (value != 0)) {
+        c->split_images = value != 0;
+        destroy_graphs(c);
+      }
+      break;
     default: return MEPGL_ERR_ARG;
   }
   return MEPGL_OK;
@@ -1697,6 +1764,7 @@ int mepgl_get_control(mepgl_ctx* c, int key, int* value) {
     case MEPGL_CONTROL_RELAXATION_STEPS: *value = c->relax_steps.load(); break;
     case MEPGL_CONTROL_NLCG: *value = c->nlcg.load(); break;
     case MEPGL_CONTROL_FOLD_LINE_SEARCH: *value = c->fold_ls ? 1 : 0; break;
+    case MEPGL_CONTROL_SPLIT_IMAGES: *value = c->split_images ? 1 : 0; break;
     default: return MEPGL_ERR_ARG;
   }
   return MEPGL_OK;
