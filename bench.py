@@ -70,10 +70,16 @@ def warm_up(sv, iters):
 
 
 def run_path(strings, precision, steps, warmup, profile_iters=3, segment_rows=0, tune=True,
-             tune_steps=10):
+             tune_steps=10, split=True):
+    from mepgl_amd import _lib
     from mepgl_amd.solver import Solver
 
     sv = Solver(strings, precision=precision, segment_rows=segment_rows)
+    # Split: the steps of two halves of the images run on two streams (the
+    # library's default).  The per-kernel timing pass below always runs one
+    # launch sequence.
+    if not split:
+        sv.set_control(_lib.CONTROL_SPLIT_IMAGES, 0)
     iters = max(steps // R, 1)
     w_iters = max(warmup // R, 1)
     # The warm-up steps, on a device brought to its steady speed before the
@@ -195,6 +201,15 @@ def roofline(strings, r, sbytes):
         return None
     # Both step kernels' figures, the dominant one included.
     out[1]["step_kernels"] = others
+    # The step as a whole: G's and P's algorithmic bytes over the timed time
+    # per step (reparametrisation and finishing launches included).  With
+    # the image halves on two streams, one half's G and the other's P overlap,
+    # so this is the figure that sees that overlap.
+    if r.get("steps"):
+        step_bytes = sum(a * 9 * c_f(s) * sbytes for a, s in zip(act, strings))
+        step_s = r["seconds"] / r["steps"]
+        out[1]["step_achieved"] = step_bytes / step_s / 1e9
+        out[1]["step_frac"] = out[1]["step_achieved"] / HBM_PEAK_GBS
     return out[1]
 
 
@@ -359,6 +374,8 @@ def main(argv=None):
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--precision", default="both", choices=("both", "fp64", "mixed"))
     ap.add_argument("--segment-rows", type=int, default=0)
+    ap.add_argument("--no-split", action="store_true",
+                    help="one launch sequence for all images (profiling: no overlapping kernels)")
     ap.add_argument("--legs", default=",".join(LEGS),
                     help="further workloads run after the headline one ('' for none)")
     ap.add_argument("--leg-steps", type=int, default=0,
@@ -385,7 +402,7 @@ def main(argv=None):
     precs = ("fp64", "mixed") if args.precision == "both" else (args.precision,)
     for prec in precs:
         res[prec] = run_path(strings, prec, args.steps, args.warmup,
-                             segment_rows=args.segment_rows)
+                             segment_rows=args.segment_rows, split=not args.no_split)
         print(f"# {prec}: {res[prec]['steps']} steps in {res[prec]['seconds']:.4f} s", flush=True)
     if args.precision != "both":
         r = res[args.precision]
@@ -436,7 +453,7 @@ def main(argv=None):
         "config": {"workload": args.workload, "strings": len(strings),
                    "images": sum(s.F for s in strings), "N": strings[0].N,
                    "components": strings[0].ncomp, "mode": "L", "reparam_every": R,
-                   "segment_rows": rows_m,
+                   "segment_rows": rows_m, "split_images": not args.no_split,
                    "segment_rows_fp64": fp64["info"]["tile_rows"]},
         "fp64": {"value": value64, "value_active_cells": value64 * active,
                  "ms_per_step": 1e3 * fp64["seconds"] / fp64["steps"], "roofline": roof64},

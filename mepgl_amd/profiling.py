@@ -111,7 +111,9 @@ def counter_per_dispatch(out_dir, counter):
 
 
 def _bench_cmd(a):
-    return [sys.executable, os.path.join(ROOT, "bench.py"), "--no-cpu", "--legs", "",
+    # One launch sequence (--no-split): the kernels of the profiled run do not
+    # overlap, so each launch's duration is its own.
+    return [sys.executable, os.path.join(ROOT, "bench.py"), "--no-cpu", "--no-split", "--legs", "",
             "--workload", a.workload, "--precision", a.precision, "--segment-rows", str(a.rows),
             "--steps", str(a.steps), "--warmup", str(a.warmup)]
 

@@ -111,6 +111,12 @@ def test_bench_roofline_and_traffic_lookup():
     assert r["algorithmic_bytes_per_step"] == cells * 5 * 4 * 4
     assert r["achieved"] == pytest.approx(cells * 80 / 0.05e-3 / 1e9)
     assert r["frac"] == pytest.approx(r["achieved"] / 8000.0)
+    assert "step_achieved" not in r
+    # With the timed run's time per step: G's and P's bytes (9 C_f s per
+    # image*cell) over it.
+    rs = bench.roofline([s], {"times": times, "seconds": 0.02, "steps": 200}, 4)
+    assert rs["step_achieved"] == pytest.approx(cells * 9 * 4 * 4 / 0.1e-3 / 1e9)
+    assert rs["step_frac"] == pytest.approx(rs["step_achieved"] / 8000.0)
     # Two classes: one launch per class and step, bytes per string's C_f.
     s2 = configs.two_band_entry(64, 0.25, 5, 0.8, seed=3)
     r2 = bench.roofline([s, s2], {"times": times}, 4)
@@ -164,8 +170,10 @@ def test_bench_json_line_contract(monkeypatch, capsys):
     assert d["n_gpus"] == 1 and d["steps"] == 20 and d["warmup"] == 5
     assert d["ms_per_step"] == pytest.approx(0.135)
     roof = d["roofline"]
-    for key in ("bound", "achieved", "peak", "unit", "frac", "traffic", "step_kernels"):
+    for key in ("bound", "achieved", "peak", "unit", "frac", "traffic", "step_kernels",
+                "step_achieved", "step_frac"):
         assert key in roof, key
+    assert d["config"]["split_images"] is True
     assert set(roof["step_kernels"]) == {"grad_update", "poly"}
     assert d["config"]["workload"] == "vortex_entry_small"
     # The headline workload is not repeated as a leg.
