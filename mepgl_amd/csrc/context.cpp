@@ -62,6 +62,10 @@ struct mepgl_ctx {
   hipStream_t stream2 = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   bool split_images = true;  // MEPGL_CONTROL_SPLIT_IMAGES
+  // Global ids of each half's images, for the finishing launches of a split
+  // sequence.
+  int* d_half_imgs[2] = {nullptr, nullptr};
+  int half_images[2] = {0, 0};
   std::vector<StringHost> strings;
   int total_images = 0;
   DevString* d_strings = nullptr;
@@ -307,11 +311,16 @@ int run_aux(mepgl_ctx* c, const AuxArgs& a, int which, int arg) {
   return MEPGL_OK;
 }
 
-FinishArgs finish_args(mepgl_ctx* c) {
+// `part` 0 or 1: only that half's images (a split sequence).
+FinishArgs finish_args(mepgl_ctx* c, int part = -1) {
   FinishArgs f{};
   f.strings = c->d_strings;
   f.img = c->d_img;
   f.nimages = c->total_images;
+  if (part >= 0) {
+    f.gids = c->d_half_imgs[part];
+    f.nimages = c->half_images[part];
+  }
   f.sc = c->d_sc;
   f.ls = c->d_ls;
   f.out = c->d_out;
@@ -324,12 +333,12 @@ FinishArgs finish_args(mepgl_ctx* c) {
 // G, is no longer made: P reduces beta itself).
 constexpr int kTimeFinishPoly = 4;
 
-int run_finish(mepgl_ctx* c, const FinishArgs& f, int which) {
+int run_finish(mepgl_ctx* c, const FinishArgs& f, int which, hipStream_t st = nullptr) {
   if (c->dry) return MEPGL_OK;
   const bool timed = c->timing.on && which == kKernelPoly;
   const size_t s0 = c->timing.used;
   hipError_t e = timed ? record_timing_event(c) : hipSuccess;
-  if (e == hipSuccess) e = launch_finish(f, which, c->fp32(), c->stream);
+  if (e == hipSuccess) e = launch_finish(f, which, c->fp32(), st ? st : c->stream);
   if (timed && e == hipSuccess) e = record_timing_event(c);
   if (e != hipSuccess) return fail(c, MEPGL_ERR_HIP, std::string("launch: ") + hipGetErrorString(e));
   if (timed)
@@ -378,9 +387,9 @@ bool fold_line_search(const mepgl_ctx* c) {
 // `more`: another step follows at once, so P's line search may be left to
 // that step's G (every G wavefront scans the table from P's partials), which
 // saves the finishing launch and one kernel boundary per step.
-// `split`: the two halves of the images run on the two streams (the
-// caller forked them); a step that ends with the finishing launch joins them
-// first, since that launch covers every image.
+// `split`: the two halves of the images run on the two streams (the caller
+// forked them and joins them); each half's finishing launch covers its own
+// images.
 int nlcg_step(mepgl_ctx* c, int k, int nlcg, int maxwell, bool more = false, bool split = false) {
   const int gcur = (k % 2 == 0) ? 1 : 0;
   StepArgs a = step_args(c);
@@ -418,9 +427,12 @@ int nlcg_step(mepgl_ctx* c, int k, int nlcg, int maxwell, bool more = false, boo
   if (rc) return rc;
   if (more && fold_line_search(c)) {
     c->ls_deferred = true;
-  } else {
-    if (split && (rc = join_halves(c))) return rc;
-    if ((rc = run_finish(c, finish_args(c), kKernelPoly))) return rc;
+  } else if (split) {
+    if ((rc = run_finish(c, finish_args(c, 0), kKernelPoly, c->stream)) ||
+        (rc = run_finish(c, finish_args(c, 1), kKernelPoly, c->stream2)))
+      return rc;
+  } else if ((rc = run_finish(c, finish_args(c), kKernelPoly))) {
+    return rc;
   }
   c->dp = 1 - c->dp;
   c->pending = true;
@@ -481,12 +493,14 @@ std::vector<int> tile_flags(const StringHost& sh) {
 
 int build_items(mepgl_ctx* c) {
   std::vector<WorkItem> items;
+  std::vector<int> half_gids[2];
   c->groups.clear();
   for (int nc = 1; nc <= 2; ++nc)
     for (int nsf = 0; nsf <= 1; ++nsf)
       for (int pl = 0; pl <= 1; ++pl) {
         const int begin = (int)items.size();
         std::vector<int> image_ends;  // item index after each image of the class
+        std::vector<int> image_gids;  // its global image id
         for (size_t si = 0; si < c->strings.size(); ++si) {
           const DevString& s = c->strings[si].dev;
           const int spl = (s.planes || s.h.p) ? 1 : 0;
@@ -496,6 +510,7 @@ int build_items(mepgl_ctx* c) {
             for (int t = 0; t < s.tiles_x * s.tiles_y; ++t)
               items.push_back({(int)si, n, t, flags[t]});
             image_ends.push_back((int)items.size() - begin);
+            image_gids.push_back(s.img0 + n);
           }
         }
         const int count = (int)items.size() - begin;
@@ -504,6 +519,8 @@ int build_items(mepgl_ctx* c) {
         for (int e : image_ends)
           if (e < count && std::abs(2 * e - count) < std::abs(2 * half - count)) half = e;
         if (count > 0) c->groups.push_back({begin, count, nc, nsf, pl, half});
+        for (size_t q = 0; q < image_ends.size(); ++q)
+          half_gids[image_ends[q] <= half ? 0 : 1].push_back(image_gids[q]);
       }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   if (c->d_items) (void)hipFree(c->d_items);
@@ -512,6 +529,15 @@ int build_items(mepgl_ctx* c) {
   HIP_TRY(c, hipMalloc(&c->d_items, sizeof(WorkItem) * std::max<size_t>(items.size(), 1)));
   HIP_TRY(c, hipMemcpyAsync(c->d_items, items.data(), sizeof(WorkItem) * items.size(),
                             hipMemcpyHostToDevice, c->stream));
+  for (int h = 0; h < 2; ++h) {
+    if (c->d_half_imgs[h]) (void)hipFree(c->d_half_imgs[h]);
+    c->d_half_imgs[h] = nullptr;
+    c->half_images[h] = (int)half_gids[h].size();
+    HIP_TRY(c, hipMalloc(&c->d_half_imgs[h], sizeof(int) * std::max<size_t>(half_gids[h].size(), 1)));
+    if (!half_gids[h].empty())
+      HIP_TRY(c, hipMemcpyAsync(c->d_half_imgs[h], half_gids[h].data(),
+                                sizeof(int) * half_gids[h].size(), hipMemcpyHostToDevice, c->stream));
+  }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return MEPGL_OK;
 }
@@ -874,16 +900,16 @@ int outer_iteration(mepgl_ctx* c, int relax, char mode, int nlcg) {
     if ((rc = redistribute_all(c, mode == 'C'))) return rc;
   }
   const bool maxwell = mode == 'M';
-  // The steps of the two image halves run side by side when the line search
-  // stays inside G until the last step (so nothing but that step's
-  // finishing launch covers every image); per-kernel timing keeps one stream.
-  const bool split = c->split_images && relax > 1 && fold_line_search(c) && !c->timing.on;
+  // The steps of the two image halves run side by side (per-kernel timing
+  // keeps one stream).
+  const bool split = c->split_images && relax > 1 && !c->timing.on;
   if (split && (rc = fork_halves(c))) return rc;
   for (int k = 0; k < relax; ++k)
     if ((rc = nlcg_step(c, k, nlcg, maxwell ? 1 : 0, k + 1 < relax, split))) {
       if (split) (void)join_halves(c);
       return rc;
     }
+  if (split && (rc = join_halves(c))) return rc;
   return MEPGL_OK;
 }
 
@@ -1234,6 +1260,8 @@ void mepgl_destroy(mepgl_ctx* c) {
   for (auto e : c->iter_done)
     if (e) (void)hipEventDestroy(e);
   if (c->health) (void)hipHostFree(c->health);
+  for (int* p : c->d_half_imgs)
+    if (p) (void)hipFree(p);
   if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
   if (c->join_ev) (void)hipEventDestroy(c->join_ev);
   if (c->stream2) (void)hipStreamDestroy(c->stream2);

@@ -29,7 +29,8 @@ template <bool COMP>
 __global__ __launch_bounds__(kFinishPolyBlock) void k_finish_poly(FinishArgs f) {
   __shared__ double s_q[5];
   __shared__ int s_first[kFinishPolyBlock / kWave];
-  const int2 si = f.img[blockIdx.x];
+  const int gid = f.gids ? f.gids[blockIdx.x] : (int)blockIdx.x;
+  const int2 si = f.img[gid];
   const DevString& s = f.strings[si.x];
   const LineSearchTable& t = *f.ls;
   const int tid = threadIdx.x;
@@ -69,7 +70,7 @@ __global__ __launch_bounds__(kFinishPolyBlock) void k_finish_poly(FinishArgs f) 
 #pragma unroll
     for (int w = 0; w < kFinishPolyBlock / kWave; ++w) fr = min(fr, s_first[w]);
     const int first_rise = fr < kLsSteps ? fr : kLsSteps - 1;
-    ImageScalars& sc = f.sc[blockIdx.x];
+    ImageScalars& sc = f.sc[gid];
     sc.fenergy = A;
     sc.poly[0] = B;
     sc.poly[1] = C;

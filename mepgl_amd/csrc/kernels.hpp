@@ -74,6 +74,7 @@ struct AuxArgs {
 struct FinishArgs {
   const DevString* strings;
   const int2* img;   // global image id -> (string, image)
+  const int* gids;   // the images of a launch (global ids); null: every image, in order
   int nimages;
   ImageScalars* sc;
   const LineSearchTable* ls;

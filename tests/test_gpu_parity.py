@@ -640,26 +640,29 @@ def test_sites_without_bonds_across_height_changes(oracle):
         assert np.max(np.abs(g - w)) <= 1e-10 * np.max(np.abs(w))
 
 
+@pytest.mark.parametrize("fold", [1, 0])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("use_graph", [True, False])
-def test_split_images_equals_one_sequence(precision, use_graph):
+def test_split_images_equals_one_sequence(precision, use_graph, fold):
     """The NLCG steps of an outer iteration launched as two halves of each
     class's images on two streams (MEPGL_CONTROL_SPLIT_IMAGES = 1, the
     default) and as one launch sequence (0) give the same fields, direction
     and per-image scalars bit for bit: modes L, C and S, graph replay and
     eager launches, three strings in two classes (one class of 5 + 4 images,
-    one of 5, both split unevenly)."""
+    one of 5, both split unevenly), with the line search in G or in a
+    finishing launch per half after every P."""
     from mepgl_amd import _lib
 
     strings = [small_string(N=40, F=5, seed=51),
                small_string(N=40, F=4, seed=52),
                small_string(N=24, F=5, seed=53, multicomponent=True, masked=True)]
-    say(f"[gpu] split images vs one sequence: {precision} graph={use_graph}")
+    say(f"[gpu] split images vs one sequence: {precision} graph={use_graph} fold={fold}")
     names = ("fenergy", "alpha", "ls_index", "g_norm2", "beta", "poly")
     out = []
     for split in (1, 0):
         with solver_for(strings, precision=precision, use_graph=use_graph) as sv:
             sv.set_control(_lib.CONTROL_SPLIT_IMAGES, split)
+            sv.set_control(_lib.CONTROL_FOLD_LINE_SEARCH, fold)
             sv.loop(2, 6, "L")
             sv.loop(1, 5, "C")
             sv.loop(1, 4, "S")
