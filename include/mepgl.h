@@ -247,8 +247,8 @@ int mepgl_set_scalars(mepgl_ctx *ctx, int string, int which, const double *value
  * two halves at an image boundary, and the halves' steps run as two launch
  * sequences on two streams, so that one half's P (mostly arithmetic) runs
  * beside the other half's G (mostly memory traffic).  This applies when an
- * outer iteration has more than one step and P's line search stays inside G
- * (MEPGL_CONTROL_FOLD_LINE_SEARCH), and not while per-kernel timing is on.
+ * outer iteration has more than one step, and not while per-kernel timing is
+ * on.
  * 0: one launch sequence.  Results are identical bit for bit.  Set it between
  * calls, from the calling thread (a change drops the captured graphs). */
 #define MEPGL_CONTROL_SPLIT_IMAGES 4
