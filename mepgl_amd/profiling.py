@@ -2,8 +2,8 @@
 roofline fractions at a fixed segment height.
 
     python -m mepgl_amd.profiling run --workload vortex_entry_256 \
-        --precision mixed --rows 16 --out gpurun_out/prof/c2_mixed_16
-    python -m mepgl_amd.profiling merge gpurun_out/prof/c2_mixed_16/summary.json
+        --precision mixed --rows 16 --out prof/vortex_entry_256_mixed_16
+    python -m mepgl_amd.profiling merge prof/vortex_entry_256_mixed_16/summary.json
 
 `run` (on a GPU host) profiles `bench.py` at a fixed segment height (no
 autotuning, so every launch has the same shape) in three rocprofv3 passes:
