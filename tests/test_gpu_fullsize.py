@@ -227,6 +227,39 @@ def test_two_band_mixed_at_relaxed_state():
     assert np.all(e32_after <= e64 + 2e-7 * np.abs(e64)), np.max((e32_after - e64) / np.abs(e64))
 
 
+def test_thin_film_mixed_at_relaxed_state():
+    """The thin-film workload's mixed path against FP64 on one relaxed state,
+    as for two_band_512: one replica (33 x 1024^2, A fixed, the disk with a
+    notch) relaxed in FP64 for 100 outer iterations of mode L, then evaluated
+    by a mixed context.  Energies within 1e-9 relative per image and barriers
+    within 1e-7 relative of the FP64 values; 20 mixed outer iterations from
+    there raise no image's energy above the FP64 one by more than 2e-7
+    relative."""
+    import dataclasses
+
+    from mepgl_amd import configs, observables
+
+    s = configs.thin_film_disk(0)
+    say("[gpu] thin_film_disk_1024 replica 0: FP64 relaxed 100 outer iterations, then mixed")
+    with solver([s], precision="fp64") as sv:
+        sv.loop(100, 10, "L")
+        e64 = sv.computeFreeEnergy()[0]
+        a, p1, p2 = sv.download(0)
+    relaxed = dataclasses.replace(s, a=a, psi_1=p1, psi_2=p2)
+    with solver([relaxed], precision="mixed") as sv:
+        e32 = sv.computeFreeEnergy()[0]
+        sv.loop(20, 10, "L")
+        e32_after = sv.computeFreeEnergy()[0]
+    b64, b32 = observables.barriers(e64), observables.barriers(e32)
+    say(f"[gpu] thin_film barriers fp64 {b64} mixed {b32}; "
+        f"max energy rel diff {np.max(np.abs(e32 - e64) / np.abs(e64)):.3e}")
+    assert np.max(np.abs(e32 - e64) / np.abs(e64)) < 1e-9
+    for k in range(2):
+        if b64[k]:
+            assert abs(b32[k] - b64[k]) <= 1e-7 * abs(b64[k]), (k, b64, b32)
+    assert np.all(e32_after <= e64 + 2e-7 * np.abs(e64)), np.max((e32_after - e64) / np.abs(e64))
+
+
 def test_thin_film_mixed_replicas(oracle):
     """The thin-film workload as specified: 4 seeded replicas in one mixed
     context.  Batched equals each replica's own context bit for bit (same
