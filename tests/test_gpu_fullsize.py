@@ -260,6 +260,37 @@ def test_thin_film_mixed_at_relaxed_state():
     assert np.all(e32_after <= e64 + 2e-7 * np.abs(e64)), np.max((e32_after - e64) / np.abs(e64))
 
 
+def test_sweep_mixed_at_relaxed_state():
+    """The batched sweep's mixed path against FP64 on relaxed states: six
+    strings of sweep_48 (uneven N and F, one and two components) relaxed
+    together in one FP64 context for 60 outer iterations of mode L, then
+    evaluated by one mixed context.  Per string, energies within 1e-9
+    relative per image and barriers within 1e-7 relative (absolute 1e-9 of
+    the energy scale for a zero barrier)."""
+    import dataclasses
+
+    from mepgl_amd import configs, observables
+
+    strings = configs.sweep_48()[:6]
+    say("[gpu] sweep_48 strings 0-5: FP64 relaxed 60 outer iterations, evaluated in mixed")
+    with solver(strings, precision="fp64") as sv:
+        sv.loop(60, 10, "L")
+        e64 = sv.computeFreeEnergy()
+        relaxed = []
+        for i, s in enumerate(strings):
+            a, p1, p2 = sv.download(i)
+            relaxed.append(dataclasses.replace(s, a=a, psi_1=p1,
+                                               psi_2=p2 if s.multicomponent else None))
+    with solver(relaxed, precision="mixed") as sv:
+        e32 = sv.computeFreeEnergy()
+    for i, (m, d) in enumerate(zip(e32, e64)):
+        assert np.max(np.abs(m - d) / np.abs(d)) < 1e-9, i
+        bm, bd = observables.barriers(m), observables.barriers(d)
+        for k in range(2):
+            assert abs(bm[k] - bd[k]) <= max(1e-7 * abs(bd[k]), 1e-9 * np.max(np.abs(d))), \
+                (i, k, bm, bd)
+
+
 def test_thin_film_mixed_replicas(oracle):
     """The thin-film workload as specified: 4 seeded replicas in one mixed
     context.  Batched equals each replica's own context bit for bit (same
