@@ -264,9 +264,10 @@ def test_sweep_mixed_at_relaxed_state():
     """The batched sweep's mixed path against FP64 on relaxed states: six
     strings of sweep_48 (uneven N and F, one and two components) relaxed
     together in one FP64 context for 60 outer iterations of mode L, then
-    evaluated by one mixed context.  Per string, energies within 1e-9
-    relative per image and barriers within 1e-7 relative (absolute 1e-9 of
-    the energy scale for a zero barrier)."""
+    evaluated by one mixed context.  Per string, energies within 1e-8
+    relative per image (the float rounding of the relaxed fields moves them
+    by about 1e-9) and barriers within 1e-7 relative (absolute 1e-9 of the
+    energy scale for a zero barrier)."""
     import dataclasses
 
     from mepgl_amd import configs, observables
@@ -284,7 +285,7 @@ def test_sweep_mixed_at_relaxed_state():
     with solver(relaxed, precision="mixed") as sv:
         e32 = sv.computeFreeEnergy()
     for i, (m, d) in enumerate(zip(e32, e64)):
-        assert np.max(np.abs(m - d) / np.abs(d)) < 1e-9, i
+        assert np.max(np.abs(m - d) / np.abs(d)) < 1e-8, i
         bm, bd = observables.barriers(m), observables.barriers(d)
         for k in range(2):
             assert abs(bm[k] - bd[k]) <= max(1e-7 * abs(bd[k]), 1e-9 * np.max(np.abs(d))), \
