@@ -21,8 +21,15 @@ the leg's segment height (fixed for the single-class legs), and for the
 48-string sweep the batched context against one context per string in both
 precisions.
 
+The timed steps run the two halves of the images on two streams (the
+library's default); `--no-split` keeps one launch sequence, as the profiled
+runs do.  `roofline` describes the dominant step kernel from a per-kernel
+timing pass (one launch sequence, HIP events); `roofline.step_achieved` is G's
+and P's algorithmic bytes per step over the timed time per step, where the
+halves overlap.
+
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
-                    [--legs a,b,c] [--leg-steps K2]
+                    [--legs a,b,c] [--leg-steps K2] [--no-split]
 """
 from __future__ import annotations
 
