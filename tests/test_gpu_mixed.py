@@ -182,3 +182,32 @@ def test_solver_fp64_check(oracle):
     # every image's energy, which the statement counts as an upper bound.
     assert r["fenergy_rel_err"].shape == (5,) and np.all(np.isfinite(r["fenergy_rel_err"]))
     assert np.all(r["fenergy_fp64"] < r["fenergy_mixed"])
+
+
+@pytest.mark.parametrize("N,multicomponent,masked", [(256, False, True), (250, True, False),
+                                                     (200, False, False), (320, True, True)])
+def test_mixed_edge_exchange_against_oracle(N, multicomponent, masked, oracle):
+    """Mixed-precision G and P trade the edge columns of the four strips of a
+    workgroup through LDS when the four are one segment of one image.  Rows
+    of 4 strips (N = 256, 250: the last strip partial, its lane 63 taking its
+    own clamped column), 4 strips of which the last holds 8 columns (N = 200)
+    and 5 strips (N = 320: groups that straddle two segments load their
+    halos), with and without a domain mask: after two outer iterations of
+    mode L the fields and energies agree with the FP64 oracle within the mixed
+    tolerance (energies 1e-5, fields norm-wise 1e-4).  A wrong halo value
+    would put O(1) errors on the strip edges."""
+    from mepgl_amd.solver import Solver
+
+    s = small_string(N=N, F=4, seed=61 + N, multicomponent=multicomponent, masked=masked)
+    say(f"[gpu] mixed edge exchange N={N} mc={multicomponent} masked={masked}")
+    with Solver([s], precision="mixed", segment_rows=16) as sv:
+        sv.loop(2, 3, "L")
+        e = sv.computeFreeEnergy()[0]
+        a, p1, p2 = sv.download(0)
+    ref = oracle.OracleString(s)
+    ref.loop(2, 3, "L")
+    assert rel_err(e, ref.free_energy()) < 1e-5
+    assert normwise(p1, ref.psi_1) < 1e-4
+    assert normwise(a, ref.a) < 1e-4
+    if multicomponent:
+        assert normwise(p2, ref.psi_2) < 1e-4
