@@ -252,6 +252,12 @@ int mepgl_set_scalars(mepgl_ctx *ctx, int string, int which, const double *value
  * 0: one launch sequence.  Results are identical bit for bit.  Set it between
  * calls, from the calling thread (a change drops the captured graphs). */
 #define MEPGL_CONTROL_SPLIT_IMAGES 4
+/* No counterpart in the reference: in mixed precision, the four strips of a
+ * G or P workgroup trade the columns just outside each strip through LDS
+ * when they are one segment of one image (1, the default), instead of
+ * loading them (0).  Results are identical bit for bit.  Set it between
+ * calls (a change drops the captured graphs). */
+#define MEPGL_CONTROL_EDGE_EXCHANGE 5
 int mepgl_set_control(mepgl_ctx *ctx, int key, int value);
 int mepgl_get_control(mepgl_ctx *ctx, int key, int *value);
 

@@ -22,6 +22,7 @@ SCALARS = {
 }
 CONTROL_STOP, CONTROL_RELAXATION_STEPS, CONTROL_NLCG, CONTROL_FOLD_LINE_SEARCH = 0, 1, 2, 3
 CONTROL_SPLIT_IMAGES = 4
+CONTROL_EDGE_EXCHANGE = 5
 KERNEL_GRAD, KERNEL_POLY = 0, 1
 
 EXPORTS = [

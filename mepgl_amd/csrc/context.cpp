@@ -62,6 +62,7 @@ struct mepgl_ctx {
   hipStream_t stream2 = nullptr;
   hipEvent_t fork_ev = nullptr, join_ev = nullptr;
   bool split_images = true;  // MEPGL_CONTROL_SPLIT_IMAGES
+  bool edge_exchange = true; // MEPGL_CONTROL_EDGE_EXCHANGE
   // Global ids of each half's images, for the finishing launches of a split
   // sequence.
   int* d_half_imgs[2] = {nullptr, nullptr};
@@ -196,6 +197,7 @@ StepArgs step_args(mepgl_ctx* c) {
   a.sc = c->d_sc;
   a.ls = c->d_ls;
   a.nlcg = 1;
+  a.edges = c->edge_exchange ? 1 : 0;
   a.health = c->health;
   return a;
 }
@@ -1780,6 +1782,12 @@ int mepgl_set_control(mepgl_ctx* c, int key, int value) {
         destroy_graphs(c);
       }
       break;
+    case MEPGL_CONTROL_EDGE_EXCHANGE:
+      if (c->edge_exchange != (value != 0)) {
+        c->edge_exchange = value != 0;
+        destroy_graphs(c);
+      }
+      break;
     default: return MEPGL_ERR_ARG;
   }
   return MEPGL_OK;
@@ -1793,6 +1801,7 @@ int mepgl_get_control(mepgl_ctx* c, int key, int* value) {
     case MEPGL_CONTROL_NLCG: *value = c->nlcg.load(); break;
     case MEPGL_CONTROL_FOLD_LINE_SEARCH: *value = c->fold_ls ? 1 : 0; break;
     case MEPGL_CONTROL_SPLIT_IMAGES: *value = c->split_images ? 1 : 0; break;
+    case MEPGL_CONTROL_EDGE_EXCHANGE: *value = c->edge_exchange ? 1 : 0; break;
     default: return MEPGL_ERR_ARG;
   }
   return MEPGL_OK;

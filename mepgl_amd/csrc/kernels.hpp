@@ -46,6 +46,7 @@ struct StepArgs {
   int scratch;      // G writes the diagnostic gradient buffer only
   int nc, nsf;      // class of every item in `items` (components, fixed A)
   int pl;           // ... and whether its strings have parameter planes
+  int edges;        // mixed precision: workgroups trade edge columns through LDS
   int* health;      // set to 1 when a step's gradient totals are not finite
 };
 

@@ -131,6 +131,7 @@ __device__ __forceinline__ EdgePlan edge_plan(const StepArgs& a, const DevString
                                               const WorkItem& it) {
   EdgePlan e;
   if constexpr (W > 1) {
+    if (!a.edges) return e;
     const int groups = (a.nitems + W - 1) / W;
     const int kb = xcd_item(blockIdx.x, groups) * W;
     if (kb + W > a.nitems) return e;

@@ -211,3 +211,32 @@ def test_mixed_edge_exchange_against_oracle(N, multicomponent, masked, oracle):
     assert normwise(a, ref.a) < 1e-4
     if multicomponent:
         assert normwise(p2, ref.psi_2) < 1e-4
+
+
+@pytest.mark.parametrize("N,multicomponent,masked", [(256, False, True), (250, True, False),
+                                                     (320, True, True)])
+def test_mixed_edge_exchange_equals_halo_loads(N, multicomponent, masked):
+    """The edge exchange (MEPGL_CONTROL_EDGE_EXCHANGE = 1, the default) and
+    halo loads from memory (0) give the same fields, direction and per-image
+    scalars bit for bit through graph replays of modes L and C: the traded
+    values are the neighbours' own, formed by the same operations from the
+    same inputs."""
+    from mepgl_amd import _lib
+    from mepgl_amd.solver import Solver
+
+    s = small_string(N=N, F=5, seed=71 + N, multicomponent=multicomponent, masked=masked)
+    say(f"[gpu] edge exchange vs halo loads N={N} mc={multicomponent} masked={masked}")
+    names = ("fenergy", "alpha", "ls_index", "g_norm2", "beta", "poly")
+    out = []
+    for edges in (1, 0):
+        with Solver([s], precision="mixed", segment_rows=16) as sv:
+            sv.set_control(_lib.CONTROL_EDGE_EXCHANGE, edges)
+            sv.loop(2, 4, "L")
+            sv.loop(1, 3, "C")
+            out.append((sv.download(0), sv.download_state(0, "d"),
+                        [sv.scalars(0, k) for k in names]))
+    (f1, d1, s1), (f0, d0, s0) = out
+    for x, y in zip(f1 + d1, f0 + d0):
+        np.testing.assert_array_equal(x, y)
+    for x, y in zip(s1, s0):
+        np.testing.assert_array_equal(x, y)
