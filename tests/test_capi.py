@@ -32,6 +32,35 @@ def test_library_exports_all_declared_symbols():
     assert sorted(_lib.EXPORTS) == header_functions()
 
 
+def header_defines():
+    text = open(os.path.join(ROOT, "include", "mepgl.h")).read()
+    return {k: int(v) for k, v in
+            re.findall(r"#define (MEPGL_[A-Z_0-9]+)\s+\(?(-?\d+)\)?u?\b", text)}
+
+
+def test_python_constants_match_the_header():
+    """The ctypes binding's constants are the header's: error codes,
+    precisions, flags, scalar kinds and control keys."""
+    from mepgl_amd import _lib
+
+    d = header_defines()
+    pairs = {"OK": "MEPGL_OK", "ERR_ARG": "MEPGL_ERR_ARG", "ERR_HIP": "MEPGL_ERR_HIP",
+             "ERR_NONFINITE": "MEPGL_ERR_NONFINITE", "ERR_STATE": "MEPGL_ERR_STATE",
+             "ERR_NOMEM": "MEPGL_ERR_NOMEM", "FP64": "MEPGL_FP64", "MIXED": "MEPGL_MIXED",
+             "POLY_EXACT": "MEPGL_POLY_EXACT", "CUBIC_TEXTBOOK": "MEPGL_CUBIC_TEXTBOOK",
+             "NO_GRAPH": "MEPGL_NO_GRAPH", "GAUGE_ALIGN": "MEPGL_GAUGE_ALIGN",
+             "ROLLBACK": "MEPGL_ROLLBACK", "GAUGE_ALIGN_LOCAL": "MEPGL_GAUGE_ALIGN_LOCAL"}
+    for py, c in pairs.items():
+        assert getattr(_lib, py) == d[c], (py, c)
+    controls = {k[len("MEPGL_CONTROL_"):]: v for k, v in d.items()
+                if k.startswith("MEPGL_CONTROL_")}
+    assert len(controls) >= 6
+    for name, v in controls.items():
+        assert getattr(_lib, "CONTROL_" + name) == v, name
+    for name, v in _lib.SCALARS.items():
+        assert d["MEPGL_SCALAR_" + name.upper()] == v, name
+
+
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     from mepgl_amd import _lib
 
