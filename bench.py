@@ -129,6 +129,8 @@ def run_path(strings, precision, steps, warmup, profile_iters=3, segment_rows=0,
     info = sv.info(0)
     info["rows"] = rows if rows is not None else [sv.info(i)["tile_rows"]
                                                   for i in range(len(strings))]
+    # Image parts of the split step sequence (0: one sequence).
+    info["parts"] = sv.get_control(_lib.CONTROL_SPLIT_IMAGES)
     sv.close()
     return dict(seconds=dt, steps=iters * R, energies=energies, times=times, info=info,
                 profile_steps=profile_iters * R)
@@ -461,7 +463,9 @@ def main(argv=None):
                    "images": sum(s.F for s in strings), "N": strings[0].N,
                    "components": strings[0].ncomp, "mode": "L", "reparam_every": R,
                    "segment_rows": rows_m, "split_images": not args.no_split,
-                   "segment_rows_fp64": fp64["info"]["tile_rows"]},
+                   "image_parts": mixed["info"]["parts"],
+                   "segment_rows_fp64": fp64["info"]["tile_rows"],
+                   "image_parts_fp64": fp64["info"]["parts"]},
         "fp64": {"value": value64, "value_active_cells": value64 * active,
                  "ms_per_step": 1e3 * fp64["seconds"] / fp64["steps"], "roofline": roof64},
         "mixed_over_fp64": value / value64,

@@ -243,14 +243,18 @@ int mepgl_set_scalars(mepgl_ctx *ctx, int string, int which, const double *value
  * graphs). */
 #define MEPGL_CONTROL_FOLD_LINE_SEARCH 3
 /* No counterpart in the reference: how the NLCG steps of an outer iteration
- * are launched.  1 (the default): the images of each class are split into
- * two halves at an image boundary, and the halves' steps run as two launch
- * sequences on two streams, so that one half's P (mostly arithmetic) runs
- * beside the other half's G (mostly memory traffic).  This applies when an
- * outer iteration has more than one step, and not while per-kernel timing is
- * on.
- * 0: one launch sequence.  Results are identical bit for bit.  Set it between
- * calls, from the calling thread (a change drops the captured graphs). */
+ * are launched.  Split (the default): the images of each class are cut into
+ * parts at image boundaries, and the parts' steps run as launch sequences on
+ * their own streams, so that one part's P (mostly arithmetic) runs beside
+ * another part's G (mostly memory traffic).  A context starts with 2 parts;
+ * mepgl_autotune picks 2 or 4 together with the segment heights.  Splitting
+ * applies when an outer iteration has more than one step, and not while
+ * per-kernel timing is on.
+ * 0: one launch sequence.  1: split, with the current part count.  2 to 4:
+ * split into that many parts.  Other values: MEPGL_ERR_ARG.
+ * mepgl_get_control returns the part count, or 0 when not split.  Results are
+ * identical bit for bit.  Set it between calls, from the calling thread (a
+ * change drops the captured graphs). */
 #define MEPGL_CONTROL_SPLIT_IMAGES 4
 /* No counterpart in the reference: in mixed precision, the four strips of a
  * G or P workgroup trade the columns just outside each strip through LDS

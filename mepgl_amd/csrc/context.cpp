@@ -21,6 +21,16 @@
 
 using namespace mepgl;
 
+// Most parts of each class's images whose NLCG steps run side by side on
+// their own streams (MEPGL_CONTROL_SPLIT_IMAGES).  The best count depends on
+// the workload, so the autotuner picks 2 or 4 with the segment heights.  On
+// one device, vortex_entry_256 in mixed precision (64 images of 256^2) ran a
+// step in 0.1109 ms with four parts at 16-row segments, against 0.1117 ms
+// with two at their best height (22 rows).  Four parts made the same
+// workload's FP64 steps 0.8 % slower, and thin_film_disk_1024's mixed steps
+// 1.4 % slower.
+constexpr int kMaxParts = 4;
+
 namespace {
 
 struct Alloc {
@@ -55,18 +65,20 @@ struct mepgl_ctx {
   unsigned flags = 0;
   int device = 0;
   hipStream_t stream = nullptr;
-  // Second stream of the image halves (split_images): between
-  // reparametrisations every image's NLCG state is independent of the
-  // others', so the NLCG steps of the two halves of each class's images run
-  // as two launch sequences side by side, one P beside the other half's G.
-  hipStream_t stream2 = nullptr;
-  hipEvent_t fork_ev = nullptr, join_ev = nullptr;
+  // Streams of the image parts (split_images): between reparametrisations
+  // every image's NLCG state is independent of the others', so the NLCG
+  // steps of `parts` parts of each class's images run as launch sequences
+  // side by side, one part's P (mostly arithmetic) beside another part's G
+  // (mostly memory traffic).  Part 0 runs on `stream`, part p on xs[p].
+  hipStream_t xs[kMaxParts] = {};
+  hipEvent_t fork_ev = nullptr, join_evs[kMaxParts] = {};
   bool split_images = true;  // MEPGL_CONTROL_SPLIT_IMAGES
+  int parts = 2;             // parts of a split sequence, 2..kMaxParts
   bool edge_exchange = true; // MEPGL_CONTROL_EDGE_EXCHANGE
-  // Global ids of each half's images, for the finishing launches of a split
+  // Global ids of each part's images, for the finishing launches of a split
   // sequence.
-  int* d_half_imgs[2] = {nullptr, nullptr};
-  int half_images[2] = {0, 0};
+  int* d_part_imgs[kMaxParts] = {};
+  int part_images[kMaxParts] = {};
   std::vector<StringHost> strings;
   int total_images = 0;
   DevString* d_strings = nullptr;
@@ -78,10 +90,11 @@ struct mepgl_ctx {
   // Items sorted by class (components, fixed A, parameter planes); G and P
   // launch once per non-empty class so each launch runs one specialised
   // kernel.
-  // `half`: items of the first half of the class's images (an image
-  // boundary), the rest being the second half.
+  // `cut`: item offsets of the parts of the class's images, each at an
+  // image boundary; part p is items [cut[p], cut[p + 1]).
   struct Group {
-    int begin, count, nc, nsf, pl, half;
+    int begin, count, nc, nsf, pl;
+    int cut[kMaxParts + 1];
   };
   std::vector<Group> groups;
   double** d_jb = nullptr;
@@ -233,22 +246,24 @@ hipError_t record_timing_event(mepgl_ctx* c) {
 
 int run_step_group(mepgl_ctx* c, const StepArgs& a, int which, hipStream_t st);
 
+hipStream_t part_stream(const mepgl_ctx* c, int part);
+
 // Launches `which` once per item class: on all of its images (part -1) on
-// the context's stream, or on the first (part 0) or second (part 1) half of
-// its images on the first or second stream.
+// the context's stream, or on part `part` of its images on that part's
+// stream.
 int run_step(mepgl_ctx* c, const StepArgs& a0, int which, int part = -1) {
   if (c->dry) return MEPGL_OK;
   for (const auto& g : c->groups) {
     StepArgs a = a0;
-    const int b = part == 1 ? g.half : 0;
-    const int e = part == 0 ? g.half : g.count;
+    const int b = part < 0 ? 0 : g.cut[part];
+    const int e = part < 0 ? g.count : g.cut[part + 1];
     if (e <= b) continue;
     a.items = c->d_items + g.begin + b;
     a.nitems = e - b;
     a.nc = g.nc;
     a.nsf = g.nsf;
     a.pl = g.pl;
-    int rc = run_step_group(c, a, which, part == 1 ? c->stream2 : c->stream);
+    int rc = run_step_group(c, a, which, part_stream(c, part));
     if (rc) return rc;
   }
   return MEPGL_OK;
@@ -273,17 +288,23 @@ int run_step_group(mepgl_ctx* c, const StepArgs& a, int which, hipStream_t st) {
 
 // The second stream starts after the work queued so far on the first
 // (fork) and the first continues after the second's work (join).
-int fork_halves(mepgl_ctx* c) {
+hipStream_t part_stream(const mepgl_ctx* c, int part) {
+  return part >= 1 ? c->xs[part] : c->stream;
+}
+
+int fork_parts(mepgl_ctx* c) {
   if (c->dry) return MEPGL_OK;
   HIP_TRY(c, hipEventRecord(c->fork_ev, c->stream));
-  HIP_TRY(c, hipStreamWaitEvent(c->stream2, c->fork_ev, 0));
+  for (int p = 1; p < c->parts; ++p) HIP_TRY(c, hipStreamWaitEvent(c->xs[p], c->fork_ev, 0));
   return MEPGL_OK;
 }
 
-int join_halves(mepgl_ctx* c) {
+int join_parts(mepgl_ctx* c) {
   if (c->dry) return MEPGL_OK;
-  HIP_TRY(c, hipEventRecord(c->join_ev, c->stream2));
-  HIP_TRY(c, hipStreamWaitEvent(c->stream, c->join_ev, 0));
+  for (int p = 1; p < c->parts; ++p) {
+    HIP_TRY(c, hipEventRecord(c->join_evs[p], c->xs[p]));
+    HIP_TRY(c, hipStreamWaitEvent(c->stream, c->join_evs[p], 0));
+  }
   return MEPGL_OK;
 }
 
@@ -313,15 +334,15 @@ int run_aux(mepgl_ctx* c, const AuxArgs& a, int which, int arg) {
   return MEPGL_OK;
 }
 
-// `part` 0 or 1: only that half's images (a split sequence).
+// `part` in [0, parts): only that part's images (a split sequence).
 FinishArgs finish_args(mepgl_ctx* c, int part = -1) {
   FinishArgs f{};
   f.strings = c->d_strings;
   f.img = c->d_img;
   f.nimages = c->total_images;
   if (part >= 0) {
-    f.gids = c->d_half_imgs[part];
-    f.nimages = c->half_images[part];
+    f.gids = c->d_part_imgs[part];
+    f.nimages = c->part_images[part];
   }
   f.sc = c->d_sc;
   f.ls = c->d_ls;
@@ -389,9 +410,8 @@ bool fold_line_search(const mepgl_ctx* c) {
 // `more`: another step follows at once, so P's line search may be left to
 // that step's G (every G wavefront scans the table from P's partials), which
 // saves the finishing launch and one kernel boundary per step.
-// `split`: the two halves of the images run on the two streams (the caller
-// forked them and joins them); each half's finishing launch covers its own
-// images.
+// `split`: the image parts run on their streams (the caller forked them and
+// joins them); each part's finishing launch covers its own images.
 int nlcg_step(mepgl_ctx* c, int k, int nlcg, int maxwell, bool more = false, bool split = false) {
   const int gcur = (k % 2 == 0) ? 1 : 0;
   StepArgs a = step_args(c);
@@ -409,8 +429,8 @@ int nlcg_step(mepgl_ctx* c, int k, int nlcg, int maxwell, bool more = false, boo
   if (!a.ls_fold) {
     if (int rc = settle_ls(c)) return rc;
   }
-  int rc = split ? run_step(c, a, kKernelGrad, 0) : run_step(c, a, kKernelGrad);
-  if (!rc && split) rc = run_step(c, a, kKernelGrad, 1);
+  int rc = MEPGL_OK;
+  for (int p = split ? 0 : -1; p < (split ? c->parts : 0) && !rc; ++p) rc = run_step(c, a, kKernelGrad, p);
   if (rc) return rc;
   c->ls_deferred = false;
   if (c->pending) {
@@ -424,15 +444,13 @@ int nlcg_step(mepgl_ctx* c, int k, int nlcg, int maxwell, bool more = false, boo
   a.dout = 1 - c->dp;
   // P totals the gradient partials into beta itself (there is no finishing
   // launch after G); the quartic totals and the line search follow P.
-  rc = split ? run_step(c, a, kKernelPoly, 0) : run_step(c, a, kKernelPoly);
-  if (!rc && split) rc = run_step(c, a, kKernelPoly, 1);
+  for (int p = split ? 0 : -1; p < (split ? c->parts : 0) && !rc; ++p) rc = run_step(c, a, kKernelPoly, p);
   if (rc) return rc;
   if (more && fold_line_search(c)) {
     c->ls_deferred = true;
   } else if (split) {
-    if ((rc = run_finish(c, finish_args(c, 0), kKernelPoly, c->stream)) ||
-        (rc = run_finish(c, finish_args(c, 1), kKernelPoly, c->stream2)))
-      return rc;
+    for (int p = 0; p < c->parts; ++p)
+      if ((rc = run_finish(c, finish_args(c, p), kKernelPoly, part_stream(c, p)))) return rc;
   } else if ((rc = run_finish(c, finish_args(c), kKernelPoly))) {
     return rc;
   }
@@ -495,7 +513,7 @@ std::vector<int> tile_flags(const StringHost& sh) {
 
 int build_items(mepgl_ctx* c) {
   std::vector<WorkItem> items;
-  std::vector<int> half_gids[2];
+  std::vector<int> part_gids[kMaxParts];
   c->groups.clear();
   for (int nc = 1; nc <= 2; ++nc)
     for (int nsf = 0; nsf <= 1; ++nsf)
@@ -516,13 +534,25 @@ int build_items(mepgl_ctx* c) {
           }
         }
         const int count = (int)items.size() - begin;
-        // The image boundary nearest to half the class's items.
-        int half = count;
-        for (int e : image_ends)
-          if (e < count && std::abs(2 * e - count) < std::abs(2 * half - count)) half = e;
-        if (count > 0) c->groups.push_back({begin, count, nc, nsf, pl, half});
-        for (size_t q = 0; q < image_ends.size(); ++q)
-          half_gids[image_ends[q] <= half ? 0 : 1].push_back(image_gids[q]);
+        // Cut k is the image boundary nearest to k / parts of the class's
+        // items, at or after cut k - 1 (a part may be empty).
+        const int np = c->parts;
+        mepgl_ctx::Group g{begin, count, nc, nsf, pl, {}};
+        g.cut[0] = 0;
+        for (int k = np; k <= kMaxParts; ++k) g.cut[k] = count;
+        for (int k = 1; k < np; ++k) {
+          int best = g.cut[k - 1];
+          for (int e : image_ends)
+            if (e >= g.cut[k - 1] && std::abs(np * e - k * count) < std::abs(np * best - k * count))
+              best = e;
+          g.cut[k] = best;
+        }
+        if (count > 0) c->groups.push_back(g);
+        for (size_t q = 0; q < image_ends.size(); ++q) {
+          int p = 0;
+          while (p + 1 < np && image_ends[q] > g.cut[p + 1]) ++p;
+          part_gids[p].push_back(image_gids[q]);
+        }
       }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   if (c->d_items) (void)hipFree(c->d_items);
@@ -531,14 +561,14 @@ int build_items(mepgl_ctx* c) {
   HIP_TRY(c, hipMalloc(&c->d_items, sizeof(WorkItem) * std::max<size_t>(items.size(), 1)));
   HIP_TRY(c, hipMemcpyAsync(c->d_items, items.data(), sizeof(WorkItem) * items.size(),
                             hipMemcpyHostToDevice, c->stream));
-  for (int h = 0; h < 2; ++h) {
-    if (c->d_half_imgs[h]) (void)hipFree(c->d_half_imgs[h]);
-    c->d_half_imgs[h] = nullptr;
-    c->half_images[h] = (int)half_gids[h].size();
-    HIP_TRY(c, hipMalloc(&c->d_half_imgs[h], sizeof(int) * std::max<size_t>(half_gids[h].size(), 1)));
-    if (!half_gids[h].empty())
-      HIP_TRY(c, hipMemcpyAsync(c->d_half_imgs[h], half_gids[h].data(),
-                                sizeof(int) * half_gids[h].size(), hipMemcpyHostToDevice, c->stream));
+  for (int h = 0; h < kMaxParts; ++h) {
+    if (c->d_part_imgs[h]) (void)hipFree(c->d_part_imgs[h]);
+    c->d_part_imgs[h] = nullptr;
+    c->part_images[h] = (int)part_gids[h].size();
+    HIP_TRY(c, hipMalloc(&c->d_part_imgs[h], sizeof(int) * std::max<size_t>(part_gids[h].size(), 1)));
+    if (!part_gids[h].empty())
+      HIP_TRY(c, hipMemcpyAsync(c->d_part_imgs[h], part_gids[h].data(),
+                                sizeof(int) * part_gids[h].size(), hipMemcpyHostToDevice, c->stream));
   }
   HIP_TRY(c, hipStreamSynchronize(c->stream));
   return MEPGL_OK;
@@ -902,16 +932,16 @@ int outer_iteration(mepgl_ctx* c, int relax, char mode, int nlcg) {
     if ((rc = redistribute_all(c, mode == 'C'))) return rc;
   }
   const bool maxwell = mode == 'M';
-  // The steps of the two image halves run side by side (per-kernel timing
+  // The steps of the image parts run side by side (per-kernel timing
   // keeps one stream).
   const bool split = c->split_images && relax > 1 && !c->timing.on;
-  if (split && (rc = fork_halves(c))) return rc;
+  if (split && (rc = fork_parts(c))) return rc;
   for (int k = 0; k < relax; ++k)
     if ((rc = nlcg_step(c, k, nlcg, maxwell ? 1 : 0, k + 1 < relax, split))) {
-      if (split) (void)join_halves(c);
+      if (split) (void)join_parts(c);
       return rc;
     }
-  if (split && (rc = join_halves(c))) return rc;
+  if (split && (rc = join_parts(c))) return rc;
   return MEPGL_OK;
 }
 
@@ -1085,9 +1115,11 @@ int mepgl_create(const mepgl_string_desc* strings, int n_strings, int precision,
   if (e != hipSuccess) return MEPGL_ERR_HIP;
   e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) return MEPGL_ERR_HIP;
-  if (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->join_ev, hipEventDisableTiming) != hipSuccess) {
+  bool made = hipEventCreateWithFlags(&c->fork_ev, hipEventDisableTiming) == hipSuccess;
+  for (int p = 1; p < kMaxParts && made; ++p)
+    made = hipStreamCreateWithFlags(&c->xs[p], hipStreamNonBlocking) == hipSuccess &&
+           hipEventCreateWithFlags(&c->join_evs[p], hipEventDisableTiming) == hipSuccess;
+  if (!made) {
     mepgl_destroy(c.release());
     return MEPGL_ERR_HIP;
   }
@@ -1262,11 +1294,13 @@ void mepgl_destroy(mepgl_ctx* c) {
   for (auto e : c->iter_done)
     if (e) (void)hipEventDestroy(e);
   if (c->health) (void)hipHostFree(c->health);
-  for (int* p : c->d_half_imgs)
+  for (int* p : c->d_part_imgs)
     if (p) (void)hipFree(p);
   if (c->fork_ev) (void)hipEventDestroy(c->fork_ev);
-  if (c->join_ev) (void)hipEventDestroy(c->join_ev);
-  if (c->stream2) (void)hipStreamDestroy(c->stream2);
+  for (int p = 1; p < kMaxParts; ++p) {
+    if (c->join_evs[p]) (void)hipEventDestroy(c->join_evs[p]);
+    if (c->xs[p]) (void)hipStreamDestroy(c->xs[p]);
+  }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
@@ -1777,6 +1811,13 @@ int mepgl_set_control(mepgl_ctx* c, int key, int value) {
       }
       break;
     case MEPGL_CONTROL_SPLIT_IMAGES:
+      if (value < 0 || value > kMaxParts) return MEPGL_ERR_ARG;
+      if (value >= 2 && value != c->parts) {
+        HIP_TRY(c, hipSetDevice(c->device));
+        c->parts = value;
+        c->split_images = true;
+        return apply_schedule(c);
+      }
       if (c->split_images != (value != 0)) {
         c->split_images = value != 0;
         destroy_graphs(c);
@@ -1800,7 +1841,7 @@ int mepgl_get_control(mepgl_ctx* c, int key, int* value) {
     case MEPGL_CONTROL_RELAXATION_STEPS: *value = c->relax_steps.load(); break;
     case MEPGL_CONTROL_NLCG: *value = c->nlcg.load(); break;
     case MEPGL_CONTROL_FOLD_LINE_SEARCH: *value = c->fold_ls ? 1 : 0; break;
-    case MEPGL_CONTROL_SPLIT_IMAGES: *value = c->split_images ? 1 : 0; break;
+    case MEPGL_CONTROL_SPLIT_IMAGES: *value = c->split_images ? c->parts : 0; break;
     case MEPGL_CONTROL_EDGE_EXCHANGE: *value = c->edge_exchange ? 1 : 0; break;
     default: return MEPGL_ERR_ARG;
   }
@@ -1886,29 +1927,63 @@ int mepgl_autotune(mepgl_ctx* c, int steps, int* rows_out) {
   }
   const bool timing = c->timing.on;
   c->timing.on = false;
-  // Candidate heights are timed in rounds that visit every candidate once,
-  // and each keeps its fastest round: neighbouring heights often differ by a
-  // few percent, about as much as one timing varies, and a drift of the
-  // clock over the tuning run then favours no candidate.
-  for (auto& cl : order) {
-    const int N = c->strings[cl.second[0]].dev.N;
-    std::vector<int> cand{c->strings[cl.second[0]].dev.seg_rows};
-    for (int h : kRowChoices) {
-      const int r = clamp_rows(h, N);
-      if (h > N && h != kRowChoices[0]) continue;
-      if (std::find(cand.begin(), cand.end(), r) == cand.end()) cand.push_back(r);
-    }
-    std::vector<double> best(cand.size(), INFINITY);
-    for (int round = 0; round < kTuneRounds && !rc; ++round)
-      for (size_t k = 0; k < cand.size() && !rc; ++k) {
-        for (int si : cl.second) c->strings[si].dev.seg_rows = cand[k];
-        double ms = 0;
-        if ((rc = time_schedule(c, sn, steps, &ms))) break;
-        best[k] = std::min(best[k], ms);
+  // The heights are tuned for each part count of a split sequence (2 and
+  // kMaxParts), and the count whose tuned schedule runs fastest is kept with
+  // its heights.
+  const int parts0 = c->parts;
+  std::vector<int> part_cand{c->parts};
+  if (c->split_images)
+    for (int p : {2, kMaxParts})
+      if (p != c->parts) part_cand.push_back(p);
+  double best_score = INFINITY;
+  int best_parts = c->parts;
+  std::vector<int> best_rows(c->strings.size());
+  for (int parts : part_cand) {
+    c->parts = parts;
+    // Candidate heights are timed in rounds that visit every candidate once,
+    // and each keeps its fastest round: neighbouring heights often differ by a
+    // few percent, about as much as one timing varies, and a drift of the
+    // clock over the tuning run then favours no candidate.
+    for (auto& cl : order) {
+      const int N = c->strings[cl.second[0]].dev.N;
+      std::vector<int> cand{c->strings[cl.second[0]].dev.seg_rows};
+      for (int h : kRowChoices) {
+        const int r = clamp_rows(h, N);
+        if (h > N && h != kRowChoices[0]) continue;
+        if (std::find(cand.begin(), cand.end(), r) == cand.end()) cand.push_back(r);
       }
+      std::vector<double> best(cand.size(), INFINITY);
+      for (int round = 0; round < kTuneRounds && !rc; ++round)
+        for (size_t k = 0; k < cand.size() && !rc; ++k) {
+          for (int si : cl.second) c->strings[si].dev.seg_rows = cand[k];
+          double ms = 0;
+          if ((rc = time_schedule(c, sn, steps, &ms))) break;
+          best[k] = std::min(best[k], ms);
+        }
+      if (rc) break;
+      const size_t pick = (size_t)(std::min_element(best.begin(), best.end()) - best.begin());
+      for (int si : cl.second) c->strings[si].dev.seg_rows = cand[pick];
+    }
     if (rc) break;
-    const size_t pick = (size_t)(std::min_element(best.begin(), best.end()) - best.begin());
-    for (int si : cl.second) c->strings[si].dev.seg_rows = cand[pick];
+    // This part count's score: the fastest round of its tuned schedule.
+    double score = INFINITY;
+    for (int round = 0; round < kTuneRounds && !rc; ++round) {
+      double ms = 0;
+      if ((rc = time_schedule(c, sn, steps, &ms))) break;
+      score = std::min(score, ms);
+    }
+    if (rc) break;
+    if (score < best_score) {
+      best_score = score;
+      best_parts = parts;
+      for (size_t si = 0; si < c->strings.size(); ++si) best_rows[si] = c->strings[si].dev.seg_rows;
+    }
+  }
+  if (rc) {
+    c->parts = parts0;
+  } else {
+    c->parts = best_parts;
+    for (size_t si = 0; si < c->strings.size(); ++si) c->strings[si].dev.seg_rows = best_rows[si];
   }
   c->timing.on = timing;
   int rc2 = apply_schedule(c);

@@ -309,6 +309,11 @@ class Solver:
     def set_control(self, key: int, value: int):
         self._check(self._L.mepgl_set_control(self._ctx, key, int(value)))
 
+    def get_control(self, key: int) -> int:
+        v = C.c_int(0)
+        self._check(self._L.mepgl_get_control(self._ctx, key, C.byref(v)))
+        return v.value
+
     def synchronize(self):
         self._check(self._L.mepgl_synchronize(self._ctx))
 

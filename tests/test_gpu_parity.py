@@ -643,23 +643,25 @@ def test_sites_without_bonds_across_height_changes(oracle):
 @pytest.mark.parametrize("fold", [1, 0])
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("use_graph", [True, False])
-def test_split_images_equals_one_sequence(precision, use_graph, fold):
-    """The NLCG steps of an outer iteration launched as two halves of each
-    class's images on two streams (MEPGL_CONTROL_SPLIT_IMAGES = 1, the
-    default) and as one launch sequence (0) give the same fields, direction
+@pytest.mark.parametrize("parts", [2, 4])
+def test_split_images_equals_one_sequence(precision, use_graph, fold, parts):
+    """The NLCG steps of an outer iteration launched as two or four parts of
+    each class's images on as many streams (MEPGL_CONTROL_SPLIT_IMAGES = 2 or
+    4) and as one launch sequence (0) give the same fields, direction
     and per-image scalars bit for bit: modes L, C and S, graph replay and
     eager launches, three strings in two classes (one class of 5 + 4 images,
     one of 5, both split unevenly), with the line search in G or in a
-    finishing launch per half after every P."""
+    finishing launch per part after every P."""
     from mepgl_amd import _lib
 
     strings = [small_string(N=40, F=5, seed=51),
                small_string(N=40, F=4, seed=52),
                small_string(N=24, F=5, seed=53, multicomponent=True, masked=True)]
-    say(f"[gpu] split images vs one sequence: {precision} graph={use_graph} fold={fold}")
+    say(f"[gpu] split images vs one sequence: {precision} graph={use_graph} fold={fold} "
+        f"parts={parts}")
     names = ("fenergy", "alpha", "ls_index", "g_norm2", "beta", "poly")
     out = []
-    for split in (1, 0):
+    for split in (parts, 0):
         with solver_for(strings, precision=precision, use_graph=use_graph) as sv:
             sv.set_control(_lib.CONTROL_SPLIT_IMAGES, split)
             sv.set_control(_lib.CONTROL_FOLD_LINE_SEARCH, fold)
@@ -668,6 +670,33 @@ def test_split_images_equals_one_sequence(precision, use_graph, fold):
             sv.loop(1, 4, "S")
             out.append([(sv.download(i), sv.download_state(i, "d"),
                          [sv.scalars(i, k) for k in names]) for i in range(3)])
+    for (f1, d1, s1), (f0, d0, s0) in zip(*out):
+        for x, y in zip(f1 + d1, f0 + d0):
+            np.testing.assert_array_equal(x, y)
+        for x, y in zip(s1, s0):
+            np.testing.assert_array_equal(x, y)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_split_images_with_empty_parts(precision):
+    """With four parts, classes of 2 and 3 images leave some parts empty (no
+    step launches, no finishing launch).  Split and one launch sequence
+    still give the same fields, direction and per-image scalars bit for bit,
+    in modes L and S, by graph replay."""
+    from mepgl_amd import _lib
+
+    strings = [small_string(N=40, F=2, seed=61),
+               small_string(N=24, F=3, seed=62, multicomponent=True)]
+    say(f"[gpu] split images with empty parts: {precision}")
+    names = ("fenergy", "alpha", "ls_index", "g_norm2", "beta", "poly")
+    out = []
+    for split in (4, 0):
+        with solver_for(strings, precision=precision, use_graph=True) as sv:
+            sv.set_control(_lib.CONTROL_SPLIT_IMAGES, split)
+            sv.loop(2, 5, "L")
+            sv.loop(1, 4, "S")
+            out.append([(sv.download(i), sv.download_state(i, "d"),
+                         [sv.scalars(i, k) for k in names]) for i in range(2)])
     for (f1, d1, s1), (f0, d0, s0) in zip(*out):
         for x, y in zip(f1 + d1, f0 + d0):
             np.testing.assert_array_equal(x, y)

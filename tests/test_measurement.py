@@ -150,7 +150,8 @@ def test_bench_json_line_contract(monkeypatch, capsys):
         rows = segment_rows or 16
         return dict(seconds=iters * bench.R * ms * 1e-3, steps=iters * bench.R,
                     energies=[np.linspace(1.0, 2.0, s.F) for s in strings], times=times,
-                    info={"tile_rows": rows, "rows": [rows] * len(strings)}, profile_steps=30)
+                    info={"tile_rows": rows, "rows": [rows] * len(strings), "parts": 2},
+                    profile_steps=30)
 
     monkeypatch.setattr(bench, "run_path", fake_run_path)
     monkeypatch.setitem(configs.WORKLOADS, "two_band_small",
