@@ -21,12 +21,13 @@ the leg's segment height (fixed for the single-class legs), and for the
 48-string sweep the batched context against one context per string in both
 precisions.
 
-The timed steps run the two halves of the images on two streams (the
-library's default); `--no-split` keeps one launch sequence, as the profiled
+The timed steps run the images in 2 or 4 parts on their own streams (the
+library's default; the autotuner picks the count, reported as
+`config.image_parts`); `--no-split` keeps one launch sequence, as the profiled
 runs do.  `roofline` describes the dominant step kernel from a per-kernel
 timing pass (one launch sequence, HIP events); `roofline.step_achieved` is G's
 and P's algorithmic bytes per step over the timed time per step, where the
-halves overlap.
+parts overlap.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
                     [--legs a,b,c] [--leg-steps K2] [--no-split]
@@ -82,9 +83,9 @@ def run_path(strings, precision, steps, warmup, profile_iters=3, segment_rows=0,
     from mepgl_amd.solver import Solver
 
     sv = Solver(strings, precision=precision, segment_rows=segment_rows)
-    # Split: the steps of two halves of the images run on two streams (the
-    # library's default).  The per-kernel timing pass below always runs one
-    # launch sequence.
+    # Split: the steps of 2 or 4 parts of the images run on their own streams
+    # (the library's default).  The per-kernel timing pass below always runs
+    # one launch sequence.
     if not split:
         sv.set_control(_lib.CONTROL_SPLIT_IMAGES, 0)
     iters = max(steps // R, 1)
@@ -212,8 +213,8 @@ def roofline(strings, r, sbytes):
     out[1]["step_kernels"] = others
     # The step as a whole: G's and P's algorithmic bytes over the timed time
     # per step (reparametrisation and finishing launches included).  With
-    # the image halves on two streams, one half's G and the other's P overlap,
-    # so this is the figure that sees that overlap.
+    # the image parts on their own streams, one part's G and another's P
+    # overlap, so this is the figure that sees that overlap.
     if r.get("steps"):
         step_bytes = sum(a * 9 * c_f(s) * sbytes for a, s in zip(act, strings))
         step_s = r["seconds"] / r["steps"]
